@@ -1,0 +1,161 @@
+/*
+ * spgemm_amd.h -- C ABI of the MI355X SpGEMM engine (libspgemm_amd.so).
+ *
+ * The engine replaces the local multiply each rank of malchioman/SpGEMM-hpc-thesis
+ * computes: A_local rows times the gathered B rows.  The data contract is the
+ * reference's CsrMatrix (src/common/csr_matrix.hpp:10-49): int64 row offsets and
+ * entry counts, int32 column indices, fp64 values.
+ *
+ * Results are bit-identical to the reference's serial order: every output
+ * coordinate is the left-to-right sum of a*b over A entries then B entries in
+ * stored order, columns ascending, exact zeros dropped.
+ *
+ * All entry points return an spg_status; spg_last_error() gives the message of
+ * the last failure on a context.  One context per host thread; calls on one
+ * context are synchronous with respect to the host unless noted.
+ */
+#ifndef SPGEMM_AMD_H
+#define SPGEMM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes.  The C++ shim (spgemm-hpc-thesis_amd/host/spgemm_local.hpp)
+ * maps them back to the reference's exception types:
+ *   SPG_EINVAL       -> std::invalid_argument ("SpGEMM requires A.cols == B.rows",
+ *                       src/common/spgemm_common.cpp:106-110; malformed CSR views)
+ *   SPG_EOVERFLOW    -> std::overflow_error (src/common/csr_matrix.hpp:13-41)
+ *   SPG_ERANGE       -> std::runtime_error ("A contains a column index outside B's
+ *                       row range", src/common/spgemm_common.cpp:141-143)
+ *   SPG_EMISSING_ROW -> fail("missing a remote sparse row required by A"),
+ *                       src/baselines/spgemm_exchange.cpp:82-85
+ *   SPG_ENOMEM, SPG_EDEVICE, SPG_ENODEVICE, SPG_ESTATE, SPG_EPARSE ->
+ *                       std::runtime_error / std::bad_alloc */
+typedef enum spg_status {
+    SPG_OK = 0,
+    SPG_EINVAL = 1,
+    SPG_EOVERFLOW = 2,
+    SPG_ENOMEM = 3,
+    SPG_EMISSING_ROW = 4,
+    SPG_ERANGE = 5,
+    SPG_EDEVICE = 6,
+    SPG_ENODEVICE = 7,
+    SPG_ESTATE = 8,
+    SPG_EPARSE = 9,
+    SPG_EBUSY = 10
+} spg_status;
+
+typedef enum spg_location { SPG_HOST = 0, SPG_DEVICE = 1 } spg_location;
+
+/* A borrowed CSR matrix (the reference's CsrMatrix, csr_matrix.hpp:43-49).
+ * row_ptr has rows+1 entries with row_ptr[0] == 0 and row_ptr[rows] == nnz.
+ * Pointers are host or device memory according to `location`. */
+typedef struct spg_csr_view {
+    int32_t rows;
+    int32_t cols;
+    int64_t nnz;
+    const int64_t *row_ptr;
+    const int32_t *col_idx;
+    const double *values;
+    int32_t location;
+} spg_csr_view;
+
+/* The gathered remote B rows (RemoteSparseRows, src/baselines/spgemm_exchange.hpp:8-20):
+ * slot s holds global B row global_rows[s] with entries
+ * [row_ptr[s], row_ptr[s+1]).  global_rows must be distinct. */
+typedef struct spg_halo_view {
+    int32_t n_rows;
+    const int32_t *global_rows;
+    const int64_t *row_ptr;
+    const int32_t *col_idx;
+    const double *values;
+    int32_t location;
+} spg_halo_view;
+
+typedef struct spg_context spg_context;
+
+/* Per-multiply statistics of the last completed product on a context. */
+typedef struct spg_stats {
+    int64_t products;          /* P = sum over A entries of nnz(B row): the flop basis /2
+                                  (scalarMultiplicationCount, spgemm_common.cpp:474-487) */
+    int64_t structural_nnz;    /* distinct output coordinates */
+    int64_t nnz;               /* stored output entries after exact-zero drop */
+    int64_t rows_per_bin[8];   /* rows routed to each kernel bin */
+    int32_t compacted;         /* 1 if exact zeros forced a compaction pass */
+    int32_t bins;              /* number of bins used */
+    double ms_total;           /* device time of the whole pipeline (events), if timing on */
+    double ms_symbolic;        /* device time of the symbolic phase, if timing on */
+    double ms_numeric;         /* device time of the numeric phase, if timing on */
+} spg_stats;
+
+/* ---- context ---------------------------------------------------------- */
+spg_status spg_create(spg_context **ctx, int device);
+spg_status spg_destroy(spg_context *ctx);
+const char *spg_status_string(spg_status status);
+const char *spg_last_error(const spg_context *ctx);
+/* Kernels of this context run on `stream` (a hipStream_t; NULL = the context's
+ * own stream). */
+spg_status spg_set_stream(spg_context *ctx, void *stream);
+/* Record HIP events around the phases of each multiply (spg_stats.ms_*). */
+spg_status spg_set_timing(spg_context *ctx, int enabled);
+spg_status spg_last_stats(const spg_context *ctx, spg_stats *out);
+
+/* ---- the drop-in local multiply ----------------------------------------
+ * Replaces
+ *   CsrMatrix spgemm(const CsrMatrix& localMatrixA, RowBlock localBBlock,
+ *                    const CsrMatrix& localMatrixB, const RemoteSparseRows& remoteBRows,
+ *                    MPI_Comm communicator);
+ * (src/baselines/spgemm_exchange.hpp:37-39, body spgemm_exchange.cpp:258-325),
+ * called at src/baselines/two_sided_spgemm.cpp:108-109,130-131,153-154,
+ * one_sided_get_spgemm.cpp:208-209,230-231,252-253 and
+ * one_sided_put_spgemm.cpp:327-328,352-354,378-380.
+ *
+ * B row k of the product is local row k - b_first_row when
+ * b_first_row <= k < b_first_row + B_local->rows, else the halo slot holding
+ * global row k (bRowForColumn, spgemm_exchange.cpp:75-87).  With halo == NULL
+ * and b_first_row == 0 this is the -np 1 product and A->cols must equal
+ * B_local->rows.  The product is computed on the device; *nnz_c receives
+ * nnz(C).  The result stays on the context until the next multiply. */
+spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B_local,
+                              int32_t b_first_row, const spg_halo_view *halo, int64_t *nnz_c);
+/* Copy the result of the last multiply into caller-owned host buffers of
+ * sizes A->rows+1, nnz_c and nnz_c (the shim resizes its std::vectors first). */
+spg_status spg_multiply_finish(spg_context *ctx, int64_t *row_ptr, int32_t *col_idx, double *values);
+/* Device view of the last result (pointers owned by the context, valid until
+ * the next multiply on it or spg_destroy). */
+spg_status spg_result_device(spg_context *ctx, spg_csr_view *out);
+
+/* ---- staged accumulation (trident::LocalAccumulator) -------------------
+ * Replaces trident::LocalAccumulator{LocalAccumulator(rows, cols);
+ * addProduct(a, b); finish();} (src/trident/common/matrix_blocks.hpp:23-32,
+ * matrix_blocks.cpp:152-200), called at src/trident/common/execution.cpp:74-89.
+ * Each output coordinate is acc += a*b over stages in call order, then A
+ * entries, then B entries in stored order; finish() drops exact zeros and
+ * sorts columns. */
+typedef struct spg_accumulator spg_accumulator;
+spg_status spg_accumulator_create(spg_context *ctx, int32_t rows, int32_t cols, spg_accumulator **acc);
+spg_status spg_accumulator_add(spg_accumulator *acc, const spg_csr_view *A, const spg_csr_view *B);
+spg_status spg_accumulator_finish(spg_accumulator *acc, int64_t *nnz_c);
+spg_status spg_accumulator_copy(spg_accumulator *acc, int64_t *row_ptr, int32_t *col_idx, double *values);
+spg_status spg_accumulator_destroy(spg_accumulator *acc);
+
+/* ---- host matrices: MatrixMarket and binary CSR -------------------------
+ * readMatrixMarket (src/common/matrix_market.cpp:28-144): coordinate
+ * real/integer/pattern, general/symmetric/skew-symmetric/hermitian, file order,
+ * duplicates and explicit zeros kept.  The matrix is owned by the library
+ * until spg_matrix_free. */
+typedef struct spg_matrix spg_matrix;
+spg_status spg_matrix_read_mm(const char *path, spg_matrix **out);
+spg_status spg_matrix_view(const spg_matrix *m, spg_csr_view *out);
+spg_status spg_matrix_free(spg_matrix *m);
+/* Last error of the host matrix helpers (thread-local). */
+const char *spg_matrix_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPGEMM_AMD_H */

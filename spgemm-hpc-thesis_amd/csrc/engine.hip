@@ -1,0 +1,499 @@
+// engine.hip -- host side of the MI355X SpGEMM engine and the C ABI of
+// include/spgemm_amd.h.
+//
+// The context owns growth-only device buffers (no allocation in steady state),
+// a stream, and a pinned mirror of the kernels' control block.  A multiply is:
+//   analyze (per-row work bound and bin)            -> sync 1: errors, bin counts
+//   bin lists, symbolic phase, scan -> C row_ptr   -> sync 2: nnz(C) for allocation
+//   numeric phase                                   -> sync 3: stored entries
+//   compaction, only when exact zeros were dropped.
+#include "internal.hpp"
+#include "spgemm_amd.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+using namespace spg;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    bool zero_on_grow = false;
+    hipError_t ensure(size_t want, hipStream_t stream) {
+        if (want <= bytes && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return e;
+            hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t n = want < 256 ? 256 : want;
+        n = (n + 255) & ~size_t(255);
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return e;
+        }
+        bytes = n;
+        if (zero_on_grow) launch_zero_bytes(p, n, stream);
+        return hipSuccess;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace
+
+struct spg_context {
+    int device = 0;
+    int cus = 256;
+    size_t total_mem = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    Control *h_ctrl = nullptr;  // pinned mirror
+    // uploaded operands (host inputs) and the halo directory
+    DevBuf a_ptr, a_col, a_val, b_ptr, b_col, b_val, dir_start, dir_end;
+    // workspace
+    DevBuf ub, bin, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc;
+    // results (ping-pong for compaction)
+    DevBuf rowptr[2], ccol[2], cval[2];
+    int cur = 0;
+    bool have_result = false;
+    int32_t res_rows = 0, res_cols = 0;
+    int64_t res_nnz = 0;
+    spg_stats stats{};
+};
+
+struct spg_accumulator {
+    spg_context *ctx = nullptr;
+    int32_t rows = 0, cols = 0;
+};
+
+namespace {
+
+spg_status set_err(spg_context *c, spg_status s, const std::string &msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+spg_status hip_err(spg_context *c, hipError_t e, const char *what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();
+    if (e == hipErrorOutOfMemory) return set_err(c, SPG_ENOMEM, m);
+    return set_err(c, SPG_EDEVICE, m);
+}
+
+#define SPG_HIP(call, what)                                  \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_err(ctx, e_, what); \
+    } while (0)
+
+spg_status check_host_csr(spg_context *ctx, const spg_csr_view *m, const char *name) {
+    if (m->rows < 0 || m->cols < 0 || m->nnz < 0) return set_err(ctx, SPG_EINVAL, std::string(name) + ": negative dimension");
+    if (!m->row_ptr) return set_err(ctx, SPG_EINVAL, std::string(name) + ": null row_ptr");
+    if (m->nnz > 0 && (!m->col_idx || !m->values))
+        return set_err(ctx, SPG_EINVAL, std::string(name) + ": null column/value arrays");
+    if (m->location != SPG_HOST) return SPG_OK;  // device views are trusted (documented)
+    if (m->row_ptr[0] != 0 || m->row_ptr[m->rows] != m->nnz)
+        return set_err(ctx, SPG_EINVAL, std::string(name) + ": row_ptr must start at 0 and end at nnz");
+    for (int32_t r = 0; r < m->rows; ++r)
+        if (m->row_ptr[r + 1] < m->row_ptr[r])
+            return set_err(ctx, SPG_EINVAL, std::string(name) + ": row_ptr is not nondecreasing");
+    return SPG_OK;
+}
+
+// Device pointers of a view, uploading host data into the given buffers.
+spg_status stage(spg_context *ctx, const spg_csr_view *m, DevBuf &p, DevBuf &c, DevBuf &v, const int64_t *&dp,
+                 const int32_t *&dc, const double *&dv) {
+    if (m->location == SPG_DEVICE) {
+        dp = m->row_ptr;
+        dc = m->col_idx;
+        dv = m->values;
+        return SPG_OK;
+    }
+    const size_t rp = ((size_t)m->rows + 1) * sizeof(int64_t);
+    SPG_HIP(p.ensure(rp, ctx->stream), "hipMalloc(row_ptr)");
+    SPG_HIP(c.ensure((size_t)m->nnz * sizeof(int32_t), ctx->stream), "hipMalloc(col)");
+    SPG_HIP(v.ensure((size_t)m->nnz * sizeof(double), ctx->stream), "hipMalloc(val)");
+    SPG_HIP(hipMemcpyAsync(p.p, m->row_ptr, rp, hipMemcpyHostToDevice, ctx->stream), "upload row_ptr");
+    if (m->nnz) {
+        SPG_HIP(hipMemcpyAsync(c.p, m->col_idx, (size_t)m->nnz * 4, hipMemcpyHostToDevice, ctx->stream), "upload col");
+        SPG_HIP(hipMemcpyAsync(v.p, m->values, (size_t)m->nnz * 8, hipMemcpyHostToDevice, ctx->stream), "upload val");
+    }
+    dp = p.as<int64_t>();
+    dc = c.as<int32_t>();
+    dv = v.as<double>();
+    return SPG_OK;
+}
+
+spg_status read_control(spg_context *ctx) {
+    SPG_HIP(hipMemcpyAsync(ctx->h_ctrl, ctx->ctrl.p, sizeof(Control), hipMemcpyDeviceToHost, ctx->stream),
+            "read control block");
+    SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return SPG_OK;
+}
+
+spg_status flag_error(spg_context *ctx, unsigned long long flags) {
+    if (flags & kErrMissing)
+        return set_err(ctx, SPG_EMISSING_ROW, "missing a remote sparse row required by A");
+    if (flags & kErrRange)
+        return set_err(ctx, SPG_ERANGE, "A contains a column index outside B's row range");
+    if (flags & kErrInternal) return set_err(ctx, SPG_EDEVICE, "internal capacity invariant violated");
+    return SPG_OK;
+}
+
+// The whole product on the device.  On success the result is
+// rowptr[cur], ccol[cur], cval[cur] with res_nnz entries.
+spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols) {
+    hipStream_t st = ctx->stream;
+    ctx->have_result = false;
+    spg_stats stats{};
+    const size_t R = rows > 0 ? (size_t)rows : 1;
+    SPG_HIP(ctx->ub.ensure(R * 8, st), "hipMalloc(ub)");
+    SPG_HIP(ctx->bin.ensure(R, st), "hipMalloc(bin)");
+    SPG_HIP(ctx->bin_rows.ensure(R * 4, st), "hipMalloc(bin_rows)");
+    SPG_HIP(ctx->snnz.ensure(R * 8, st), "hipMalloc(snnz)");
+    SPG_HIP(ctx->annz.ensure(R * 8, st), "hipMalloc(annz)");
+    SPG_HIP(ctx->scan_tmp.ensure(scan_tmp_elems(rows) * 8, st), "hipMalloc(scan)");
+    SPG_HIP(ctx->ctrl.ensure(sizeof(Control), st), "hipMalloc(ctrl)");
+    const int out = 0;
+    SPG_HIP(ctx->rowptr[out].ensure(((size_t)rows + 1) * 8, st), "hipMalloc(row_ptr C)");
+
+    Workspace ws{};
+    ws.ub = ctx->ub.as<int64_t>();
+    ws.bin = ctx->bin.as<uint8_t>();
+    ws.bin_rows = ctx->bin_rows.as<int32_t>();
+    ws.snnz = ctx->snnz.as<int64_t>();
+    ws.annz = ctx->annz.as<int64_t>();
+    ws.scan_tmp = ctx->scan_tmp.as<int64_t>();
+    ws.ctrl = ctx->ctrl.as<Control>();
+    int64_t width = ((int64_t)cols + 31) / 32 * 32;
+    const int64_t kMaxWindow = int64_t(1) << 22;
+    if (width > kMaxWindow) width = kMaxWindow;
+    if (width < 32) width = 32;
+    ws.lwidth = width;
+
+    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+    launch_zero_control(ws.ctrl, st);
+    launch_analyze(op, ws, st);
+    spg_status s = read_control(ctx);
+    if (s != SPG_OK) return s;
+    Control h = *ctx->h_ctrl;
+    if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
+    stats.products = (int64_t)h.products;
+    for (int b = 0; b < kNumBins; ++b) stats.rows_per_bin[b] = h.bin_count[b];
+    stats.bins = kNumBins;
+
+    // Large-row path: bitmap (and later accumulator) per resident workgroup.
+    if (h.bin_count[kBinLarge] > 0) {
+        const size_t per_block = (size_t)width * 8 + (size_t)width / 8;
+        size_t budget = ctx->total_mem / 16;
+        if (budget > (size_t(4) << 30)) budget = size_t(4) << 30;
+        size_t nb = budget / per_block;
+        const size_t cap = (size_t)ctx->cus * 2;
+        if (nb > cap) nb = cap;
+        if (nb < 1) nb = 1;
+        ws.lblocks = (int)nb;
+        ctx->lbits.zero_on_grow = true;
+        SPG_HIP(ctx->lbits.ensure(nb * (size_t)(width / 32) * 4, st), "hipMalloc(large bitmaps)");
+        ws.lbits = ctx->lbits.as<uint32_t>();
+    }
+
+    launch_bin_fill(op, ws, st);
+    launch_symbolic(op, ws, h.bin_count, ctx->cus, st);
+    launch_scan(ws.snnz, ctx->rowptr[out].as<int64_t>(), rows, ws.scan_tmp, &ws.ctrl->structural, st);
+    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
+    if ((s = read_control(ctx)) != SPG_OK) return s;
+    h = *ctx->h_ctrl;
+    if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
+    const int64_t nstruct = (int64_t)h.structural;
+    stats.structural_nnz = nstruct;
+
+    SPG_HIP(ctx->ccol[out].ensure((size_t)nstruct * 4, st), "hipMalloc(C col)");
+    SPG_HIP(ctx->cval[out].ensure((size_t)nstruct * 8, st), "hipMalloc(C val)");
+    if (h.bin_count[kBinLarge] > 0) {
+        SPG_HIP(ctx->lacc.ensure((size_t)ws.lblocks * (size_t)width * 8, st), "hipMalloc(large accumulators)");
+        ws.lacc = ctx->lacc.as<double>();
+    }
+    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");
+    launch_numeric(op, ws, h.bin_count, ctx->cus, ctx->rowptr[out].as<int64_t>(), ctx->ccol[out].as<int32_t>(),
+                   ctx->cval[out].as<double>(), st);
+    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[3], st), "hipEventRecord");
+    if ((s = read_control(ctx)) != SPG_OK) return s;
+    h = *ctx->h_ctrl;
+    if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
+    int64_t nnz = (int64_t)h.actual;
+    int result = out;
+    if (nnz != nstruct) {
+        // Exact zeros were dropped: pack rows to their final offsets.
+        const int o2 = 1;
+        SPG_HIP(ctx->rowptr[o2].ensure(((size_t)rows + 1) * 8, st), "hipMalloc(row_ptr C2)");
+        SPG_HIP(ctx->ccol[o2].ensure((size_t)nnz * 4, st), "hipMalloc(C2 col)");
+        SPG_HIP(ctx->cval[o2].ensure((size_t)nnz * 8, st), "hipMalloc(C2 val)");
+        launch_scan(ws.annz, ctx->rowptr[o2].as<int64_t>(), rows, ws.scan_tmp, nullptr, st);
+        launch_compact(rows, ctx->rowptr[out].as<int64_t>(), ctx->ccol[out].as<int32_t>(),
+                       ctx->cval[out].as<double>(), ctx->rowptr[o2].as<int64_t>(), ctx->ccol[o2].as<int32_t>(),
+                       ctx->cval[o2].as<double>(), st);
+        SPG_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+        result = o2;
+        stats.compacted = 1;
+    }
+    SPG_HIP(hipGetLastError(), "kernel launch");
+    stats.nnz = nnz;
+    if (ctx->timing) {
+        float t = 0;
+        SPG_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[3]), "hipEventElapsedTime");
+        stats.ms_total = t;
+        SPG_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
+        stats.ms_symbolic = t;
+        SPG_HIP(hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
+        stats.ms_numeric = t;
+    }
+    ctx->cur = result;
+    ctx->res_rows = rows;
+    ctx->res_cols = cols;
+    ctx->res_nnz = nnz;
+    ctx->have_result = true;
+    ctx->stats = stats;
+    return SPG_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *spg_status_string(spg_status s) {
+    switch (s) {
+        case SPG_OK: return "ok";
+        case SPG_EINVAL: return "invalid argument";
+        case SPG_EOVERFLOW: return "count overflow";
+        case SPG_ENOMEM: return "out of memory";
+        case SPG_EMISSING_ROW: return "missing remote B row";
+        case SPG_ERANGE: return "column index outside B's row range";
+        case SPG_EDEVICE: return "device error";
+        case SPG_ENODEVICE: return "no HIP device";
+        case SPG_ESTATE: return "invalid call sequence";
+        case SPG_EPARSE: return "parse error";
+        case SPG_EBUSY: return "busy";
+    }
+    return "unknown status";
+}
+
+const char *spg_last_error(const spg_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+spg_status spg_create(spg_context **out, int device) {
+    if (!out) return SPG_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return SPG_ENODEVICE;
+    }
+    if (device < 0 || device >= n) return SPG_EINVAL;
+    spg_context *ctx = new (std::nothrow) spg_context();
+    if (!ctx) return SPG_ENOMEM;
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    hipDeviceProp_t prop{};
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_ctrl, sizeof(Control), hipHostMallocDefault);
+    for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&ctx->ev[i]);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        spg_destroy(ctx);
+        return SPG_EDEVICE;
+    }
+    ctx->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    ctx->total_mem = prop.totalGlobalMem;
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return SPG_OK;
+}
+
+spg_status spg_destroy(spg_context *ctx) {
+    if (!ctx) return SPG_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->a_ptr, &ctx->a_col, &ctx->a_val, &ctx->b_ptr, &ctx->b_col, &ctx->b_val,
+                      &ctx->dir_start, &ctx->dir_end, &ctx->ub, &ctx->bin, &ctx->bin_rows, &ctx->snnz,
+                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->rowptr[0],
+                      &ctx->rowptr[1], &ctx->ccol[0], &ctx->ccol[1], &ctx->cval[0], &ctx->cval[1]};
+    for (DevBuf *b : bufs) b->release();
+    for (hipEvent_t &e : ctx->ev)
+        if (e) hipEventDestroy(e);
+    if (ctx->h_ctrl) hipHostFree(ctx->h_ctrl);
+    if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return SPG_OK;
+}
+
+spg_status spg_set_stream(spg_context *ctx, void *stream) {
+    if (!ctx) return SPG_EINVAL;
+    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    return SPG_OK;
+}
+
+spg_status spg_set_timing(spg_context *ctx, int enabled) {
+    if (!ctx) return SPG_EINVAL;
+    ctx->timing = enabled != 0;
+    return SPG_OK;
+}
+
+spg_status spg_last_stats(const spg_context *ctx, spg_stats *out) {
+    if (!ctx || !out) return SPG_EINVAL;
+    *out = ctx->stats;
+    return SPG_OK;
+}
+
+spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int32_t b_first_row,
+                              const spg_halo_view *halo, int64_t *nnz_c) {
+    if (!ctx) return SPG_EINVAL;
+    if (!A || !B || !nnz_c) return set_err(ctx, SPG_EINVAL, "null argument");
+    ctx->have_result = false;
+    SPG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    spg_status s;
+    if ((s = check_host_csr(ctx, A, "A")) != SPG_OK) return s;
+    if ((s = check_host_csr(ctx, B, "B_local")) != SPG_OK) return s;
+    const bool local_only = halo == nullptr && b_first_row == 0;
+    if (local_only && A->cols != B->rows)
+        return set_err(ctx, SPG_EINVAL, "SpGEMM requires A.cols == B.rows");
+    if (b_first_row < 0 || (int64_t)b_first_row + B->rows > (int64_t)A->cols)
+        return set_err(ctx, SPG_EINVAL, "local B block outside A's column range");
+
+    Operands op{};
+    op.arows = A->rows;
+    op.bcols = B->cols;
+    if ((s = stage(ctx, A, ctx->a_ptr, ctx->a_col, ctx->a_val, op.aptr, op.acol, op.aval)) != SPG_OK) return s;
+    const int64_t *bptr = nullptr;
+    const int32_t *bcol = nullptr;
+    const double *bval = nullptr;
+    if (local_only) {
+        if ((s = stage(ctx, B, ctx->b_ptr, ctx->b_col, ctx->b_val, bptr, bcol, bval)) != SPG_OK) return s;
+        op.ndir = B->rows;
+        op.bstart = bptr;
+        op.bend = bptr + 1;
+        op.bcol = bcol;
+        op.bval = bval;
+    } else {
+        // Directory over global B rows: local block first, then halo slots
+        // (bRowForColumn order, spgemm_exchange.cpp:75-87).  Entries are
+        // concatenated on the device: [B_local entries | halo entries].
+        if (halo && (halo->n_rows < 0 || (halo->n_rows > 0 && (!halo->global_rows || !halo->row_ptr))))
+            return set_err(ctx, SPG_EINVAL, "malformed halo view");
+        if (B->location != SPG_HOST || (halo && halo->location != SPG_HOST))
+            return set_err(ctx, SPG_EINVAL, "halo products take host views");
+        const int64_t hn = halo ? halo->n_rows : 0;
+        const int64_t hnnz = hn ? halo->row_ptr[hn] : 0;
+        std::vector<int64_t> ds((size_t)A->cols, -1), de((size_t)A->cols, -1);
+        for (int32_t r = 0; r < B->rows; ++r) {
+            ds[(size_t)b_first_row + r] = B->row_ptr[r];
+            de[(size_t)b_first_row + r] = B->row_ptr[r + 1];
+        }
+        std::unordered_set<int32_t> seen;
+        for (int64_t slot = 0; slot < hn; ++slot) {
+            const int32_t g = halo->global_rows[slot];
+            if (g < 0 || g >= A->cols) return set_err(ctx, SPG_EINVAL, "halo row outside A's column range");
+            if (!seen.insert(g).second) return set_err(ctx, SPG_EINVAL, "duplicate halo row");
+            if (halo->row_ptr[slot + 1] < halo->row_ptr[slot])
+                return set_err(ctx, SPG_EINVAL, "halo row_ptr is not nondecreasing");
+            if (g >= b_first_row && g < b_first_row + B->rows) continue;  // local rows take precedence
+            ds[(size_t)g] = B->nnz + halo->row_ptr[slot];
+            de[(size_t)g] = B->nnz + halo->row_ptr[slot + 1];
+        }
+        const size_t tot = (size_t)(B->nnz + hnnz);
+        SPG_HIP(ctx->b_col.ensure(tot * 4, ctx->stream), "hipMalloc(B col)");
+        SPG_HIP(ctx->b_val.ensure(tot * 8, ctx->stream), "hipMalloc(B val)");
+        SPG_HIP(ctx->dir_start.ensure((size_t)A->cols * 8, ctx->stream), "hipMalloc(dir)");
+        SPG_HIP(ctx->dir_end.ensure((size_t)A->cols * 8, ctx->stream), "hipMalloc(dir)");
+        if (B->nnz) {
+            SPG_HIP(hipMemcpyAsync(ctx->b_col.p, B->col_idx, (size_t)B->nnz * 4, hipMemcpyHostToDevice, ctx->stream), "upload B col");
+            SPG_HIP(hipMemcpyAsync(ctx->b_val.p, B->values, (size_t)B->nnz * 8, hipMemcpyHostToDevice, ctx->stream), "upload B val");
+        }
+        if (hnnz) {
+            SPG_HIP(hipMemcpyAsync(ctx->b_col.as<int32_t>() + B->nnz, halo->col_idx, (size_t)hnnz * 4,
+                                   hipMemcpyHostToDevice, ctx->stream), "upload halo col");
+            SPG_HIP(hipMemcpyAsync(ctx->b_val.as<double>() + B->nnz, halo->values, (size_t)hnnz * 8,
+                                   hipMemcpyHostToDevice, ctx->stream), "upload halo val");
+        }
+        if (A->cols) {
+            SPG_HIP(hipMemcpyAsync(ctx->dir_start.p, ds.data(), ds.size() * 8, hipMemcpyHostToDevice, ctx->stream), "upload dir");
+            SPG_HIP(hipMemcpyAsync(ctx->dir_end.p, de.data(), de.size() * 8, hipMemcpyHostToDevice, ctx->stream), "upload dir");
+        }
+        SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");  // host vectors go out of scope
+        op.ndir = A->cols;
+        op.bstart = ctx->dir_start.as<int64_t>();
+        op.bend = ctx->dir_end.as<int64_t>();
+        op.bcol = ctx->b_col.as<int32_t>();
+        op.bval = ctx->b_val.as<double>();
+    }
+    if ((s = run_product(ctx, op, A->rows, B->cols)) != SPG_OK) return s;
+    *nnz_c = ctx->res_nnz;
+    return SPG_OK;
+}
+
+spg_status spg_multiply_finish(spg_context *ctx, int64_t *row_ptr, int32_t *col_idx, double *values) {
+    if (!ctx) return SPG_EINVAL;
+    if (!ctx->have_result) return set_err(ctx, SPG_ESTATE, "no product to finish");
+    if (!row_ptr || (ctx->res_nnz > 0 && (!col_idx || !values))) return set_err(ctx, SPG_EINVAL, "null output buffer");
+    const int c = ctx->cur;
+    SPG_HIP(hipMemcpyAsync(row_ptr, ctx->rowptr[c].p, ((size_t)ctx->res_rows + 1) * 8, hipMemcpyDeviceToHost, ctx->stream),
+            "download row_ptr");
+    if (ctx->res_nnz) {
+        SPG_HIP(hipMemcpyAsync(col_idx, ctx->ccol[c].p, (size_t)ctx->res_nnz * 4, hipMemcpyDeviceToHost, ctx->stream),
+                "download col");
+        SPG_HIP(hipMemcpyAsync(values, ctx->cval[c].p, (size_t)ctx->res_nnz * 8, hipMemcpyDeviceToHost, ctx->stream),
+                "download val");
+    }
+    SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return SPG_OK;
+}
+
+spg_status spg_result_device(spg_context *ctx, spg_csr_view *out) {
+    if (!ctx || !out) return SPG_EINVAL;
+    if (!ctx->have_result) return set_err(ctx, SPG_ESTATE, "no product available");
+    const int c = ctx->cur;
+    out->rows = ctx->res_rows;
+    out->cols = ctx->res_cols;
+    out->nnz = ctx->res_nnz;
+    out->row_ptr = ctx->rowptr[c].as<int64_t>();
+    out->col_idx = ctx->ccol[c].as<int32_t>();
+    out->values = ctx->cval[c].as<double>();
+    out->location = SPG_DEVICE;
+    return SPG_OK;
+}
+
+// Staged accumulation is not available yet on the device.
+spg_status spg_accumulator_create(spg_context *ctx, int32_t, int32_t, spg_accumulator **acc) {
+    if (acc) *acc = nullptr;
+    return set_err(ctx, SPG_ESTATE, "staged accumulation is not implemented");
+}
+spg_status spg_accumulator_add(spg_accumulator *, const spg_csr_view *, const spg_csr_view *) { return SPG_ESTATE; }
+spg_status spg_accumulator_finish(spg_accumulator *, int64_t *) { return SPG_ESTATE; }
+spg_status spg_accumulator_copy(spg_accumulator *, int64_t *, int32_t *, double *) { return SPG_ESTATE; }
+spg_status spg_accumulator_destroy(spg_accumulator *acc) {
+    delete acc;
+    return SPG_OK;
+}
+
+}  // extern "C"

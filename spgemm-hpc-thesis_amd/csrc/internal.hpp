@@ -1,0 +1,94 @@
+// internal.hpp -- shared declarations between the HIP kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace spg {
+
+// Row bins (by the row's upper-bound work P_row = sum of referenced B row
+// lengths, and by its A-entry count).  Empty rows need no kernel.
+enum Bin : int {
+    kBinEmpty = 0,   // P_row == 0
+    kBinWave1 = 1,   // one 64-lane workgroup per row, <= kCapWave1 products
+    kBinWave2 = 2,   // one 64-lane workgroup per row, <= kCapWave2 products
+    kBinBlock = 3,   // one 256-lane workgroup per row, <= kCapBlock products
+    kBinLarge = 4,   // chunked fold onto a windowed dense accumulator in HBM
+    kNumBins = 5
+};
+
+constexpr int kCapWave1 = 128;
+constexpr int kCapWave2 = 512;
+constexpr int kCapBlock = 2048;
+constexpr int kCapLarge = 1024;   // products per chunk on the large-row path
+constexpr int kTpbLarge = 256;
+
+// Error flags raised by kernels (OR-ed into Control::flags).
+enum : unsigned {
+    kErrRange = 1u,     // A column outside the B row directory
+    kErrMissing = 2u,   // A column names a B row that is neither local nor in the halo
+    kErrInternal = 4u   // capacity/invariant violation (never expected)
+};
+
+// Small control block in device memory, mirrored to pinned host memory.
+struct Control {
+    unsigned long long products;      // sum of P_row
+    unsigned long long flags;         // kErr* bits
+    unsigned long long actual;        // stored entries after the numeric phase
+    unsigned long long structural;    // distinct coordinates (scan total)
+    int bin_count[8];
+    int bin_cursor[8];
+    int bin_offset[9];
+    int pad[3];
+};
+
+// Operands as the kernels see them.  B rows are reached through a directory
+// indexed by the global B row id: entries [bstart[k], bend[k]); bstart[k] < 0
+// marks a row that is neither local nor in the halo.
+struct Operands {
+    const int64_t *aptr;
+    const int32_t *acol;
+    const double *aval;
+    int32_t arows;
+    int32_t ndir;
+    const int64_t *bstart;
+    const int64_t *bend;
+    const int32_t *bcol;
+    const double *bval;
+    int32_t bcols;
+};
+
+struct Workspace {
+    int64_t *ub;          // [arows] per-row product count
+    uint8_t *bin;         // [arows]
+    int32_t *bin_rows;    // [arows] rows grouped by bin
+    int64_t *snnz;        // [arows] structural count per row
+    int64_t *annz;        // [arows] stored count per row
+    int64_t *scan_tmp;    // block partials of the scan
+    Control *ctrl;
+    uint32_t *lbits;      // [lblocks * lwords] large-row presence bitmaps (kept zero between rows)
+    double *lacc;         // [lblocks * lwidth] large-row dense accumulators
+    int64_t lwidth;       // column window of the large-row path (multiple of 32)
+    int lblocks;
+};
+
+// Launch helpers (kernels.hip).  All are asynchronous on `stream`.
+void launch_zero_bytes(void *p, size_t bytes, hipStream_t stream);  // bytes: multiple of 4
+void launch_zero_control(Control *ctrl, hipStream_t stream);
+void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream);
+void launch_bin_fill(const Operands &op, const Workspace &ws, hipStream_t stream);
+// Symbolic phase for every non-empty bin whose host-side count is nonzero.
+void launch_symbolic(const Operands &op, const Workspace &ws, const int *bin_counts, int cus, hipStream_t stream);
+// Exclusive scan of n int64 counts into out[0..n]; out[n] = total; total also
+// stored to *total_dst when non-null.
+size_t scan_tmp_elems(int64_t n);
+void launch_scan(const int64_t *in, int64_t *out, int64_t n, int64_t *tmp, unsigned long long *total_dst,
+                 hipStream_t stream);
+void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, int cus,
+                    const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream);
+// Copy each row's first annz[r] entries from (rowptr_in) to (rowptr_out).
+void launch_compact(int32_t rows, const int64_t *rowptr_in, const int32_t *col_in, const double *val_in,
+                    const int64_t *rowptr_out, int32_t *col_out, double *val_out, hipStream_t stream);
+
+}  // namespace spg

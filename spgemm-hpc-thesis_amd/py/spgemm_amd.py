@@ -1,0 +1,270 @@
+"""Python bindings (ctypes) for libspgemm_amd.so, the MI355X SpGEMM engine.
+
+This is a thin layer over the C ABI in include/spgemm_amd.h, used by the tests
+and the benchmark.  Importing it never falls back to anything: if the HIP
+library is missing the import of a context raises.
+
+    eng = Engine()                      # spg_create on device 0
+    C = eng.multiply(A, B)              # drop-in product, host CSR in/out
+    eng.multiply_device(dA, dB)         # operands resident in HBM (torch tensors)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libspgemm_amd.so")
+
+SPG_HOST, SPG_DEVICE = 0, 1
+
+STATUS = {0: "SPG_OK", 1: "SPG_EINVAL", 2: "SPG_EOVERFLOW", 3: "SPG_ENOMEM", 4: "SPG_EMISSING_ROW",
+          5: "SPG_ERANGE", 6: "SPG_EDEVICE", 7: "SPG_ENODEVICE", 8: "SPG_ESTATE", 9: "SPG_EPARSE",
+          10: "SPG_EBUSY"}
+
+EXPORTED = [
+    "spg_create", "spg_destroy", "spg_status_string", "spg_last_error", "spg_set_stream", "spg_set_timing",
+    "spg_last_stats", "spg_multiply_begin", "spg_multiply_finish", "spg_result_device",
+    "spg_accumulator_create", "spg_accumulator_add", "spg_accumulator_finish", "spg_accumulator_copy",
+    "spg_accumulator_destroy", "spg_matrix_read_mm", "spg_matrix_view", "spg_matrix_free",
+    "spg_matrix_last_error",
+]
+
+
+@dataclass
+class Csr:
+    rows: int
+    cols: int
+    row_ptr: np.ndarray  # int64[rows+1]
+    col: np.ndarray      # int32[nnz]
+    val: np.ndarray      # float64[nnz]
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[-1]) if len(self.row_ptr) else 0
+
+    @staticmethod
+    def make(rows, cols, row_ptr, col, val) -> "Csr":
+        return Csr(int(rows), int(cols), np.ascontiguousarray(row_ptr, dtype=np.int64),
+                   np.ascontiguousarray(col, dtype=np.int32), np.ascontiguousarray(val, dtype=np.float64))
+
+
+class CsrView(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("nnz", C.c_int64),
+                ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("values", C.c_void_p),
+                ("location", C.c_int32)]
+
+
+class HaloView(C.Structure):
+    _fields_ = [("n_rows", C.c_int32), ("global_rows", C.c_void_p), ("row_ptr", C.c_void_p),
+                ("col_idx", C.c_void_p), ("values", C.c_void_p), ("location", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("products", C.c_int64), ("structural_nnz", C.c_int64), ("nnz", C.c_int64),
+                ("rows_per_bin", C.c_int64 * 8), ("compacted", C.c_int32), ("bins", C.c_int32),
+                ("ms_total", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"products": self.products, "structural_nnz": self.structural_nnz, "nnz": self.nnz,
+                "rows_per_bin": list(self.rows_per_bin)[: self.bins], "compacted": bool(self.compacted),
+                "ms_total": self.ms_total, "ms_symbolic": self.ms_symbolic, "ms_numeric": self.ms_numeric}
+
+
+class SpgError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+        self.name = STATUS.get(status, str(status))
+
+
+_lib = None
+
+
+def lib():
+    """Load libspgemm_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libspgemm_amd.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, P = C.c_void_p, C.POINTER
+        L.spg_create.argtypes = [P(vp), C.c_int]
+        L.spg_destroy.argtypes = [vp]
+        L.spg_status_string.argtypes = [C.c_int]
+        L.spg_status_string.restype = C.c_char_p
+        L.spg_last_error.argtypes = [vp]
+        L.spg_last_error.restype = C.c_char_p
+        L.spg_set_stream.argtypes = [vp, vp]
+        L.spg_set_timing.argtypes = [vp, C.c_int]
+        L.spg_last_stats.argtypes = [vp, P(Stats)]
+        L.spg_multiply_begin.argtypes = [vp, P(CsrView), P(CsrView), C.c_int32, P(HaloView), P(C.c_int64)]
+        L.spg_multiply_finish.argtypes = [vp, vp, vp, vp]
+        L.spg_result_device.argtypes = [vp, P(CsrView)]
+        L.spg_accumulator_create.argtypes = [vp, C.c_int32, C.c_int32, P(vp)]
+        L.spg_accumulator_add.argtypes = [vp, P(CsrView), P(CsrView)]
+        L.spg_accumulator_finish.argtypes = [vp, P(C.c_int64)]
+        L.spg_accumulator_copy.argtypes = [vp, vp, vp, vp]
+        L.spg_accumulator_destroy.argtypes = [vp]
+        L.spg_matrix_read_mm.argtypes = [C.c_char_p, P(vp)]
+        L.spg_matrix_view.argtypes = [vp, P(CsrView)]
+        L.spg_matrix_free.argtypes = [vp]
+        L.spg_matrix_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def host_view(m: Csr) -> CsrView:
+    v = CsrView()
+    v.rows, v.cols, v.nnz = m.rows, m.cols, m.nnz
+    v.row_ptr = m.row_ptr.ctypes.data
+    v.col_idx = m.col.ctypes.data if m.nnz else None
+    v.values = m.val.ctypes.data if m.nnz else None
+    v.location = SPG_HOST
+    return v
+
+
+def read_mm(path: str) -> Csr:
+    """readMatrixMarket through the engine's host reader."""
+    L = lib()
+    h = C.c_void_p()
+    s = L.spg_matrix_read_mm(path.encode(), C.byref(h))
+    if s != 0:
+        raise SpgError(s, L.spg_matrix_last_error().decode())
+    try:
+        v = CsrView()
+        L.spg_matrix_view(h, C.byref(v))
+        n = int(v.nnz)
+        rp = np.ctypeslib.as_array(C.cast(v.row_ptr, C.POINTER(C.c_int64)), shape=(v.rows + 1,)).copy()
+        col = (np.ctypeslib.as_array(C.cast(v.col_idx, C.POINTER(C.c_int32)), shape=(n,)).copy()
+               if n else np.zeros(0, np.int32))
+        val = (np.ctypeslib.as_array(C.cast(v.values, C.POINTER(C.c_double)), shape=(n,)).copy()
+               if n else np.zeros(0, np.float64))
+        return Csr(int(v.rows), int(v.cols), rp, col, val)
+    finally:
+        L.spg_matrix_free(h)
+
+
+class DeviceCsr:
+    """A CSR matrix resident in HBM (torch tensors on the GPU hold the memory)."""
+
+    def __init__(self, rows, cols, row_ptr, col, val):
+        self.rows, self.cols = int(rows), int(cols)
+        self.row_ptr, self.col, self.val = row_ptr, col, val
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.numel())
+
+    @staticmethod
+    def upload(m: Csr, device="cuda"):
+        import torch
+        return DeviceCsr(m.rows, m.cols, torch.from_numpy(m.row_ptr).to(device),
+                         torch.from_numpy(m.col).to(device), torch.from_numpy(m.val).to(device))
+
+    def view(self) -> CsrView:
+        v = CsrView()
+        v.rows, v.cols, v.nnz = self.rows, self.cols, self.nnz
+        v.row_ptr = self.row_ptr.data_ptr()
+        v.col_idx = self.col.data_ptr() if self.nnz else None
+        v.values = self.val.data_ptr() if self.nnz else None
+        v.location = SPG_DEVICE
+        return v
+
+
+class Engine:
+    """One spg_context (one device, one stream)."""
+
+    def __init__(self, device: int = 0, timing: bool = False):
+        L = lib()
+        h = C.c_void_p()
+        s = L.spg_create(C.byref(h), device)
+        if s != 0:
+            raise SpgError(s, "spg_create failed")
+        self._h = h
+        self._L = L
+        if timing:
+            L.spg_set_timing(h, 1)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.spg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, s: int):
+        if s != 0:
+            raise SpgError(s, self._L.spg_last_error(self._h).decode(errors="replace"))
+
+    def set_stream(self, stream_handle: int | None):
+        self._check(self._L.spg_set_stream(self._h, stream_handle))
+
+    def set_timing(self, on: bool):
+        self._check(self._L.spg_set_timing(self._h, 1 if on else 0))
+
+    def stats(self) -> dict:
+        st = Stats()
+        self._check(self._L.spg_last_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def begin(self, a_view: CsrView, b_view: CsrView, b_first_row: int = 0, halo: HaloView | None = None) -> int:
+        n = C.c_int64()
+        self._check(self._L.spg_multiply_begin(self._h, C.byref(a_view), C.byref(b_view), b_first_row,
+                                               C.byref(halo) if halo is not None else None, C.byref(n)))
+        return int(n.value)
+
+    def finish(self, rows: int, cols: int, nnz: int) -> Csr:
+        rp = np.zeros(rows + 1, dtype=np.int64)
+        col = np.zeros(max(nnz, 1), dtype=np.int32)
+        val = np.zeros(max(nnz, 1), dtype=np.float64)
+        self._check(self._L.spg_multiply_finish(self._h, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return Csr(rows, cols, rp, col[:nnz], val[:nnz])
+
+    def multiply(self, a: Csr, b: Csr, b_first_row: int = 0, halo=None) -> Csr:
+        """Drop-in local multiply on host CSR matrices (spgemm_exchange.cpp:258-325).
+
+        halo: None or (global_rows int32[n], row_ptr int64[n+1], col int32, val float64).
+        """
+        hv = None
+        keep = []
+        if halo is not None:
+            g, rp, col, val = (np.ascontiguousarray(halo[0], np.int32), np.ascontiguousarray(halo[1], np.int64),
+                               np.ascontiguousarray(halo[2], np.int32), np.ascontiguousarray(halo[3], np.float64))
+            keep = [g, rp, col, val]
+            hv = HaloView()
+            hv.n_rows = len(g)
+            hv.global_rows = g.ctypes.data if len(g) else None
+            hv.row_ptr = rp.ctypes.data
+            hv.col_idx = col.ctypes.data if len(col) else None
+            hv.values = val.ctypes.data if len(val) else None
+            hv.location = SPG_HOST
+        nnz = self.begin(host_view(a), host_view(b), b_first_row, hv)
+        del keep
+        return self.finish(a.rows, b.cols, nnz)
+
+    def multiply_device(self, a: DeviceCsr, b: DeviceCsr) -> int:
+        """Product of HBM-resident operands; the result stays on the device."""
+        return self.begin(a.view(), b.view())
+
+    def result_view(self) -> CsrView:
+        v = CsrView()
+        self._check(self._L.spg_result_device(self._h, C.byref(v)))
+        return v
+
+    def result_host(self) -> Csr:
+        v = self.result_view()
+        return self.finish(v.rows, v.cols, v.nnz)

@@ -1,0 +1,260 @@
+"""Parity of the HIP engine with the reference's serial order, through the C ABI.
+
+Bar: bit-exact.  Every product's (row_ptr, col, value bits) digest must equal
+the digest minted from the reference binary (golden.json), and products the
+oracle computes in seconds are compared with the oracle on the same seeded
+inputs.  At full workload sizes the checks are size-independent properties
+plus oracle comparison on sampled rows.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import oracle as O
+import spgemm_amd as S
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def to_s(m) -> S.Csr:
+    return S.Csr.make(m.rows, m.cols, m.row_ptr, m.col, m.val)
+
+
+def to_o(m) -> O.Csr:
+    return O.Csr.make(m.rows, m.cols, m.row_ptr, m.col, m.val)
+
+
+def digest(m) -> str:
+    return O.csr_digest(to_o(m))
+
+
+def all_cases():
+    out = list(cases.mm_cases()) + cases.synthetic_cases()
+    out += [(f"smoke_{s}",) + cases.smoke_case(s) for s in range(100)]
+    out += [(f"dyadic_{i}", a, b) for i, (a, b) in enumerate(cases.dyadic_cases())]
+    return out
+
+
+def test_golden_products_bit_exact(engine, golden):
+    for name, a, b in all_cases():
+        print("case", name, flush=True)
+        c = engine.multiply(to_s(a), to_s(b))
+        g = golden["products"][name]
+        st = engine.stats()
+        assert st["products"] == g["products"], name
+        assert c.nnz == g["c_nnz"], name
+        assert digest(c) == g["digest"], name
+
+
+def test_validators_accept_engine_output(engine):
+    for name, a, b in list(cases.mm_cases()) + cases.synthetic_cases():
+        if name == "overflow_product":
+            continue
+        print("case", name, flush=True)
+        c = to_o(engine.multiply(to_s(a), to_s(b)))
+        ok, mism = O.partial_validate(a, b, c)
+        assert ok and mism == 0, name
+        rep = O.full_validate(a, b, c)
+        assert rep["passed"] and rep["rejected"] == 0 and rep["max_abs_error"] == 0.0, name
+        assert rep["max_entry_rel_error"] == 0.0 and rep["frobenius_rel_error"] == 0.0
+
+
+def test_overflow_product_keeps_infinities(engine, golden):
+    # tests/check_partial_validation.cmake: the overflow case fails validation
+    a = O.mm_read(cases.data_path("overflow_product.mtx"))
+    c = to_o(engine.multiply(to_s(a), to_s(a)))
+    assert np.all(np.isinf(c.val)) and c.nnz == 2
+    assert not O.partial_validate(a, a, c)[0]
+    assert not O.full_validate(a, a, c)["passed"]
+
+
+def csr(rows, cols, rp, col, val):
+    return S.Csr.make(rows, cols, rp, col, val)
+
+
+def test_special_values(engine):
+    unit = csr(1, 1, [0, 1], [0], [1.0])
+    # 1e-200^2 underflows to zero and is dropped (spgemm_validation_smoke.cpp:74-75)
+    t = csr(1, 1, [0, 1], [0], [1e-200])
+    c = engine.multiply(t, t)
+    assert c.nnz == 0 and engine.stats()["compacted"]
+    # subnormal operands survive (spgemm_full_streaming.cpp:108-110)
+    sub = csr(1, 1, [0, 1], [0], [5e-324])
+    c = engine.multiply(sub, unit)
+    assert c.nnz == 1 and c.val.view(np.uint64)[0] == 1
+    # NaN and Inf propagate; the validators reject them, the multiply does not
+    for v in (math.nan, math.inf, -math.inf):
+        c = engine.multiply(csr(1, 1, [0, 1], [0], [v]), unit)
+        assert c.nnz == 1 and not math.isfinite(c.val[0])
+    # order-dependent cancellation: {1e16,-1e16,1} keeps 1, {1e16,1,-1e16} drops all
+    oa = csr(1, 1, [0, 1], [0], [1.0])
+    assert engine.multiply(oa, csr(1, 1, [0, 3], [0, 0, 0], [1e16, -1e16, 1])).val.tolist() == [1.0]
+    assert engine.multiply(oa, csr(1, 1, [0, 3], [0, 0, 0], [1e16, 1, -1e16])).nnz == 0
+    # 1 + 2^-55 - 1 == 0 in serial order (spgemm_full_streaming.cpp:58-61)
+    a = csr(1, 3, [0, 3], [0, 1, 2], [1, 1, 1])
+    b = csr(3, 1, [0, 1, 2, 3], [0, 0, 0], [1, math.ldexp(1, -55), -1])
+    assert engine.multiply(a, b).nnz == 0
+
+
+def test_empty_and_degenerate_shapes(engine):
+    z = csr(0, 5, [0], [], [])
+    b = csr(5, 4, [0, 1, 1, 2, 2, 3], [0, 3, 1], [1.0, 2.0, 3.0])
+    c = engine.multiply(z, b)
+    assert c.rows == 0 and c.cols == 4 and c.nnz == 0 and c.row_ptr.tolist() == [0]
+    ea = csr(3, 5, [0, 0, 0, 0], [], [])
+    c = engine.multiply(ea, b)
+    assert c.nnz == 0 and c.row_ptr.tolist() == [0, 0, 0, 0]
+    nb = csr(5, 0, [0] * 6, [], [])
+    a = csr(2, 5, [0, 2, 3], [0, 4, 2], [1.0, 1.0, 1.0])
+    c = engine.multiply(a, nb)
+    assert c.cols == 0 and c.nnz == 0
+    # A entries that reference empty B rows only
+    c = engine.multiply(csr(2, 5, [0, 2, 3], [1, 3, 3], [1.0, 2.0, 3.0]), b)
+    assert c.nnz == 0 and engine.stats()["rows_per_bin"][0] == 2
+
+
+def test_errors(engine):
+    a = csr(2, 3, [0, 1, 2], [0, 2], [1.0, 1.0])
+    b = csr(4, 2, [0, 1, 1, 1, 2], [0, 1], [1.0, 1.0])
+    with pytest.raises(S.SpgError) as e:
+        engine.multiply(a, b)
+    assert e.value.name == "SPG_EINVAL"
+    bad = csr(1, 3, [0, 1], [7], [1.0])  # column outside B's rows
+    with pytest.raises(S.SpgError) as e:
+        engine.multiply(bad, csr(3, 2, [0, 1, 1, 2], [0, 1], [1.0, 1.0]))
+    assert e.value.name == "SPG_ERANGE"
+    with pytest.raises(S.SpgError) as e:
+        engine.multiply(csr(1, 1, [0, 2], [0], [1.0]), csr(1, 1, [0, 1], [0], [1.0]))
+    assert e.value.name == "SPG_EINVAL"
+    # the engine stays usable after errors
+    c = engine.multiply(csr(1, 1, [0, 1], [0], [2.0]), csr(1, 1, [0, 1], [0], [3.0]))
+    assert c.val.tolist() == [6.0]
+
+
+def split_halo(b: O.Csr, first: int, count: int, seed: int, drop=None):
+    """Local block rows [first, first+count) and the other rows as a shuffled halo."""
+    s0, s1 = b.row_ptr[first], b.row_ptr[first + count]
+    local = S.Csr.make(count, b.cols, b.row_ptr[first:first + count + 1] - s0, b.col[s0:s1], b.val[s0:s1])
+    rows = [r for r in range(b.rows) if not (first <= r < first + count) and r != drop]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rng.shuffle(rows)
+    rp, col, val = [0], [], []
+    for r in rows:
+        col.append(b.col[b.row_ptr[r]:b.row_ptr[r + 1]])
+        val.append(b.val[b.row_ptr[r]:b.row_ptr[r + 1]])
+        rp.append(rp[-1] + int(b.row_ptr[r + 1] - b.row_ptr[r]))
+    halo = (np.array(rows, np.int32), np.array(rp, np.int64),
+            np.concatenate(col) if col else np.zeros(0, np.int32), np.concatenate(val) if val else np.zeros(0))
+    return local, halo
+
+
+def test_halo_rows_resolve_like_the_full_product(engine, golden):
+    # bRowForColumn (spgemm_exchange.cpp:75-87): local block first, then halo slots
+    for name in ("uniform_rect", "ragged_dups_unsorted", "heavy_rows"):
+        _, a, b = next(x for x in cases.synthetic_cases() if x[0] == name)
+        first, count = b.rows // 3, b.rows // 2
+        local, halo = split_halo(b, first, count, seed=1)
+        c = engine.multiply(to_s(a), local, b_first_row=first, halo=halo)
+        assert digest(c) == golden["products"][name]["digest"], name
+    _, a, b = next(x for x in cases.synthetic_cases() if x[0] == "uniform_rect")
+    referenced = int(a.col[0])
+    local, halo = split_halo(b, 0, 10, seed=2, drop=referenced if referenced >= 10 else None)
+    if referenced >= 10:
+        with pytest.raises(S.SpgError) as e:
+            engine.multiply(to_s(a), local, b_first_row=0, halo=halo)
+        assert e.value.name == "SPG_EMISSING_ROW"
+
+
+def test_device_resident_operands(engine, golden):
+    import torch
+    for name in ("cage8_AA", "heavy_rows", "int_cancel_zeros"):
+        _, a, b = next(x for x in list(cases.mm_cases()) + cases.synthetic_cases() if x[0] == name)
+        da, db = S.DeviceCsr.upload(to_s(a)), S.DeviceCsr.upload(to_s(b))
+        n = engine.multiply_device(da, db)
+        torch.cuda.synchronize()
+        c = engine.result_host()
+        assert c.nnz == n and digest(c) == golden["products"][name]["digest"], name
+
+
+def test_buffer_reuse_across_sizes(engine):
+    big = W.uniform_random(20000, 20000, 12, seed=5)
+    small = W.uniform_random(300, 20000, 3, seed=6)
+    want_big = O.csr_digest(O.product(to_o(big), to_o(big)))
+    want_small = O.csr_digest(O.product(to_o(small), to_o(big)))
+    for _ in range(2):
+        assert digest(engine.multiply(big, big)) == want_big
+        assert digest(engine.multiply(small, big)) == want_small
+
+
+def test_uniform_16384_x8_full(engine):
+    # 16384^2, exactly 8 uniform columns per row, values U[-1,1): full size
+    a = W.uniform_random(16384, 16384, 8, seed=42)
+    b = W.uniform_random(16384, 16384, 8, seed=43)
+    c = to_o(engine.multiply(a, b))
+    assert engine.stats()["products"] == 16384 * 64
+    ref = O.product(to_o(a), to_o(b))
+    assert O.csr_digest(c) == O.csr_digest(ref)
+    assert O.partial_validate(to_o(a), to_o(b), c) == (True, 0)
+    rep = O.full_validate(to_o(a), to_o(b), c)
+    assert rep["passed"] and rep["max_abs_error"] == 0.0
+
+
+def sorted_rows(c) -> bool:
+    d = np.diff(c.col.astype(np.int64))
+    starts = c.row_ptr[1:-1]
+    ok = d > 0
+    ok[starts[(starts > 0) & (starts < c.nnz)] - 1] = True
+    return bool(np.all(ok))
+
+
+def sample_rows(a, rows):
+    rp = [0]
+    cols, vals = [], []
+    for r in rows:
+        s, e = a.row_ptr[r], a.row_ptr[r + 1]
+        cols.append(a.col[s:e])
+        vals.append(a.val[s:e])
+        rp.append(rp[-1] + int(e - s))
+    return O.Csr.make(len(rows), a.cols, rp, np.concatenate(cols), np.concatenate(vals))
+
+
+def check_sampled_rows(a, b, c, nsample, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rows = np.sort(rng.choice(a.rows, size=min(nsample, a.rows), replace=False))
+    sub = O.product(sample_rows(to_o(a), rows), to_o(b))
+    for i, r in enumerate(rows):
+        s, e = c.row_ptr[r], c.row_ptr[r + 1]
+        ss, se = sub.row_ptr[i], sub.row_ptr[i + 1]
+        assert np.array_equal(c.col[s:e], sub.col[ss:se]), r
+        assert np.array_equal(c.val[s:e].view(np.uint64), sub.val[ss:se].view(np.uint64)), r
+
+
+def test_uniform_2p20_x16_properties(engine):
+    # 2^20 x 2^20, 16 uniform columns per row: properties at full size plus
+    # bit-exact comparison with the oracle on 4096 sampled rows.
+    n = 1 << 20
+    a = W.uniform_random(n, n, 16, seed=42)
+    b = W.uniform_random(n, n, 16, seed=43)
+    c = engine.multiply(a, b)
+    st = engine.stats()
+    assert st["products"] == n * 256
+    assert c.nnz == st["structural_nnz"] == st["nnz"]
+    assert c.row_ptr[0] == 0 and c.row_ptr[-1] == c.nnz and np.all(np.diff(c.row_ptr) >= 0)
+    assert sorted_rows(c)
+    assert np.all(c.val != 0) and np.all(np.isfinite(c.val))
+    assert c.col.min() >= 0 and c.col.max() < n
+    check_sampled_rows(a, b, c, 4096, seed=7)
+
+
+def test_rmat_s13_AA(engine):
+    # R-MAT (0.57, 0.19, 0.19, 0.05) scale 13, edge factor 32: heavy rows take
+    # the large-row path; whole product against the oracle.
+    a = W.rmat(13, 32, seed=42)
+    c = engine.multiply(a, a)
+    st = engine.stats()
+    assert st["rows_per_bin"][4] > 0
+    assert digest(c) == O.csr_digest(O.product(to_o(a), to_o(a)))
