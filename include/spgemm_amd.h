@@ -83,7 +83,8 @@ typedef struct spg_stats {
                                   (scalarMultiplicationCount, spgemm_common.cpp:474-487) */
     int64_t structural_nnz;    /* distinct output coordinates */
     int64_t nnz;               /* stored output entries after exact-zero drop */
-    int64_t rows_per_bin[8];   /* rows routed to each kernel bin */
+    int64_t rows_per_bin[8];   /* rows per kernel bin: empty, wave<=128, wave<=512,
+                                  block<=2048 products, large (chunked) */
     int32_t compacted;         /* 1 if exact zeros forced a compaction pass */
     int32_t bins;              /* number of bins used */
     double ms_total;           /* device time of the whole pipeline (events), if timing on */
@@ -127,6 +128,18 @@ spg_status spg_multiply_finish(spg_context *ctx, int64_t *row_ptr, int32_t *col_
 /* Device view of the last result (pointers owned by the context, valid until
  * the next multiply on it or spg_destroy). */
 spg_status spg_result_device(spg_context *ctx, spg_csr_view *out);
+
+/* ---- HBM-resident operands ----------------------------------------------
+ * Upload a host CSR matrix once and keep it in device memory owned by the
+ * library (for repeated, batched or served products).  spg_device_matrix_view
+ * gives a device view (location SPG_DEVICE) usable as A or B_local in
+ * spg_multiply_begin.  The handle must be freed before its context. */
+typedef struct spg_device_matrix spg_device_matrix;
+spg_status spg_matrix_upload(spg_context *ctx, const spg_csr_view *host, spg_device_matrix **out);
+spg_status spg_device_matrix_view(const spg_device_matrix *m, spg_csr_view *out);
+spg_status spg_device_matrix_free(spg_device_matrix *m);
+/* Wait for all work queued on the context's stream. */
+spg_status spg_synchronize(spg_context *ctx);
 
 /* ---- staged accumulation (trident::LocalAccumulator) -------------------
  * Replaces trident::LocalAccumulator{LocalAccumulator(rows, cols);

@@ -483,6 +483,70 @@ spg_status spg_result_device(spg_context *ctx, spg_csr_view *out) {
     return SPG_OK;
 }
 
+struct spg_device_matrix {
+    int device = 0;
+    int32_t rows = 0, cols = 0;
+    int64_t nnz = 0;
+    void *row_ptr = nullptr, *col = nullptr, *val = nullptr;
+};
+
+spg_status spg_matrix_upload(spg_context *ctx, const spg_csr_view *host, spg_device_matrix **out) {
+    if (!ctx) return SPG_EINVAL;
+    if (!host || !out) return set_err(ctx, SPG_EINVAL, "null argument");
+    *out = nullptr;
+    if (host->location != SPG_HOST) return set_err(ctx, SPG_EINVAL, "upload takes a host view");
+    spg_status s = check_host_csr(ctx, host, "matrix");
+    if (s != SPG_OK) return s;
+    SPG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    spg_device_matrix *m = new (std::nothrow) spg_device_matrix();
+    if (!m) return set_err(ctx, SPG_ENOMEM, "out of memory");
+    m->device = ctx->device;
+    m->rows = host->rows;
+    m->cols = host->cols;
+    m->nnz = host->nnz;
+    const size_t rp = ((size_t)host->rows + 1) * 8;
+    hipError_t e = hipMalloc(&m->row_ptr, rp);
+    if (e == hipSuccess) e = hipMalloc(&m->col, host->nnz ? (size_t)host->nnz * 4 : 4);
+    if (e == hipSuccess) e = hipMalloc(&m->val, host->nnz ? (size_t)host->nnz * 8 : 8);
+    if (e == hipSuccess) e = hipMemcpy(m->row_ptr, host->row_ptr, rp, hipMemcpyHostToDevice);
+    if (e == hipSuccess && host->nnz) e = hipMemcpy(m->col, host->col_idx, (size_t)host->nnz * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && host->nnz) e = hipMemcpy(m->val, host->values, (size_t)host->nnz * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        spg_device_matrix_free(m);
+        return hip_err(ctx, e, "upload matrix");
+    }
+    *out = m;
+    return SPG_OK;
+}
+
+spg_status spg_device_matrix_view(const spg_device_matrix *m, spg_csr_view *out) {
+    if (!m || !out) return SPG_EINVAL;
+    out->rows = m->rows;
+    out->cols = m->cols;
+    out->nnz = m->nnz;
+    out->row_ptr = static_cast<const int64_t *>(m->row_ptr);
+    out->col_idx = static_cast<const int32_t *>(m->col);
+    out->values = static_cast<const double *>(m->val);
+    out->location = SPG_DEVICE;
+    return SPG_OK;
+}
+
+spg_status spg_device_matrix_free(spg_device_matrix *m) {
+    if (!m) return SPG_OK;
+    (void)hipSetDevice(m->device);
+    if (m->row_ptr) hipFree(m->row_ptr);
+    if (m->col) hipFree(m->col);
+    if (m->val) hipFree(m->val);
+    delete m;
+    return SPG_OK;
+}
+
+spg_status spg_synchronize(spg_context *ctx) {
+    if (!ctx) return SPG_EINVAL;
+    SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return SPG_OK;
+}
+
 // Staged accumulation is not available yet on the device.
 spg_status spg_accumulator_create(spg_context *ctx, int32_t, int32_t, spg_accumulator **acc) {
     if (acc) *acc = nullptr;

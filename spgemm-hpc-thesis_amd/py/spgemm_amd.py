@@ -6,7 +6,8 @@ library is missing the import of a context raises.
 
     eng = Engine()                      # spg_create on device 0
     C = eng.multiply(A, B)              # drop-in product, host CSR in/out
-    eng.multiply_device(dA, dB)         # operands resident in HBM (torch tensors)
+    dA = eng.upload(A)                  # operand resident in HBM
+    eng.multiply_device(dA, dB)         # product of resident operands
 """
 from __future__ import annotations
 
@@ -30,7 +31,8 @@ EXPORTED = [
     "spg_last_stats", "spg_multiply_begin", "spg_multiply_finish", "spg_result_device",
     "spg_accumulator_create", "spg_accumulator_add", "spg_accumulator_finish", "spg_accumulator_copy",
     "spg_accumulator_destroy", "spg_matrix_read_mm", "spg_matrix_view", "spg_matrix_free",
-    "spg_matrix_last_error",
+    "spg_matrix_last_error", "spg_matrix_upload", "spg_device_matrix_view", "spg_device_matrix_free",
+    "spg_synchronize",
 ]
 
 
@@ -113,16 +115,24 @@ def lib():
         L.spg_matrix_view.argtypes = [vp, P(CsrView)]
         L.spg_matrix_free.argtypes = [vp]
         L.spg_matrix_last_error.restype = C.c_char_p
+        L.spg_matrix_upload.argtypes = [vp, P(CsrView), P(vp)]
+        L.spg_device_matrix_view.argtypes = [vp, P(CsrView)]
+        L.spg_device_matrix_free.argtypes = [vp]
+        L.spg_synchronize.argtypes = [vp]
         _lib = L
     return _lib
 
 
 def host_view(m: Csr) -> CsrView:
+    """Host view; nnz is the length of the entry arrays (the engine checks it
+    against row_ptr[rows])."""
+    if len(m.col) != len(m.val):
+        raise ValueError("column and value arrays differ in length")
     v = CsrView()
-    v.rows, v.cols, v.nnz = m.rows, m.cols, m.nnz
+    v.rows, v.cols, v.nnz = m.rows, m.cols, len(m.col)
     v.row_ptr = m.row_ptr.ctypes.data
-    v.col_idx = m.col.ctypes.data if m.nnz else None
-    v.values = m.val.ctypes.data if m.nnz else None
+    v.col_idx = m.col.ctypes.data if len(m.col) else None
+    v.values = m.val.ctypes.data if len(m.val) else None
     v.location = SPG_HOST
     return v
 
@@ -148,31 +158,31 @@ def read_mm(path: str) -> Csr:
         L.spg_matrix_free(h)
 
 
-class DeviceCsr:
-    """A CSR matrix resident in HBM (torch tensors on the GPU hold the memory)."""
+class DeviceMatrix:
+    """A CSR matrix resident in HBM, owned by the library (spg_matrix_upload)."""
 
-    def __init__(self, rows, cols, row_ptr, col, val):
-        self.rows, self.cols = int(rows), int(cols)
-        self.row_ptr, self.col, self.val = row_ptr, col, val
-
-    @property
-    def nnz(self) -> int:
-        return int(self.col.numel())
-
-    @staticmethod
-    def upload(m: Csr, device="cuda"):
-        import torch
-        return DeviceCsr(m.rows, m.cols, torch.from_numpy(m.row_ptr).to(device),
-                         torch.from_numpy(m.col).to(device), torch.from_numpy(m.val).to(device))
+    def __init__(self, engine: "Engine", m: Csr):
+        L = lib()
+        h = C.c_void_p()
+        engine._check(L.spg_matrix_upload(engine._h, C.byref(host_view(m)), C.byref(h)))
+        self._h, self._L = h, L
+        self.rows, self.cols, self.nnz = m.rows, m.cols, len(m.col)
 
     def view(self) -> CsrView:
         v = CsrView()
-        v.rows, v.cols, v.nnz = self.rows, self.cols, self.nnz
-        v.row_ptr = self.row_ptr.data_ptr()
-        v.col_idx = self.col.data_ptr() if self.nnz else None
-        v.values = self.val.data_ptr() if self.nnz else None
-        v.location = SPG_DEVICE
+        self._L.spg_device_matrix_view(self._h, C.byref(v))
         return v
+
+    def free(self):
+        if getattr(self, "_h", None):
+            self._L.spg_device_matrix_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Engine:
@@ -256,7 +266,13 @@ class Engine:
         del keep
         return self.finish(a.rows, b.cols, nnz)
 
-    def multiply_device(self, a: DeviceCsr, b: DeviceCsr) -> int:
+    def upload(self, m: Csr) -> DeviceMatrix:
+        return DeviceMatrix(self, m)
+
+    def synchronize(self):
+        self._check(self._L.spg_synchronize(self._h))
+
+    def multiply_device(self, a: DeviceMatrix, b: DeviceMatrix) -> int:
         """Product of HBM-resident operands; the result stays on the device."""
         return self.begin(a.view(), b.view())
 

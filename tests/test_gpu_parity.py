@@ -170,13 +170,13 @@ def test_halo_rows_resolve_like_the_full_product(engine, golden):
 
 
 def test_device_resident_operands(engine, golden):
-    import torch
     for name in ("cage8_AA", "heavy_rows", "int_cancel_zeros"):
         _, a, b = next(x for x in list(cases.mm_cases()) + cases.synthetic_cases() if x[0] == name)
-        da, db = S.DeviceCsr.upload(to_s(a)), S.DeviceCsr.upload(to_s(b))
+        da, db = engine.upload(to_s(a)), engine.upload(to_s(b))
         n = engine.multiply_device(da, db)
-        torch.cuda.synchronize()
         c = engine.result_host()
+        da.free()
+        db.free()
         assert c.nnz == n and digest(c) == golden["products"][name]["digest"], name
 
 
