@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}"]
 
-HIP_SOURCES = ["kernels.hip", "engine.hip"]
+HIP_SOURCES = ["analyze.hip", "symbolic.hip", "numeric.hip", "scan.hip", "engine.hip"]
 CXX_SOURCES = ["host_matrix.cpp"]
 
 

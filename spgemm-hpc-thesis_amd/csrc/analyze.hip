@@ -1,0 +1,139 @@
+// analyze.hip -- per-row work bound, bins and bin row lists.
+//
+// P_row = sum over the row's A entries of nnz(B row) is the row's upper bound
+// on distinct output columns and its share of scalarMultiplicationCount
+// (src/common/spgemm_common.cpp:474-487).  One wave takes a tile of 64
+// consecutive rows and streams the tile's A entries with all 64 lanes
+// (coalesced, balanced whatever the row lengths), finds each entry's row by a
+// 6-step search over the lanes' row starts, and reduces lengths per row with a
+// segmented shuffle scan.  Rows are appended to their bin's list with one
+// atomic per bin per wave.  The (start, length) of the B row every A entry
+// references is written out as a coalesced stream, so the later phases never
+// gather B's row offsets again.
+#include "device_common.hpp"
+
+namespace spg {
+namespace {
+
+using namespace dev;
+
+__global__ void k_zero_control(Control *c) {
+    int *p = reinterpret_cast<int *>(c);
+    for (int i = threadIdx.x; i < (int)(sizeof(Control) / sizeof(int)); i += blockDim.x) p[i] = 0;
+}
+
+__global__ void k_zero_words(uint32_t *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+constexpr int kAnalyzeTpb = 256;
+
+__global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace ws) {
+    __shared__ unsigned long long s_ub[kAnalyzeTpb / 64][64];
+    const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int64_t tile = (int64_t)blockIdx.x * (kAnalyzeTpb / 64) + w;
+    const int64_t r = tile * 64 + lane;
+    if (tile * 64 >= op.arows) return;  // whole wave past the end (uniform per wave)
+    const bool valid = r < op.arows;
+    const int last = (int)min((int64_t)63, (int64_t)op.arows - 1 - tile * 64);
+    const int64_t a0 = valid ? op.aptr[r] : 0;
+    const int64_t a1 = valid ? op.aptr[r + 1] : 0;
+    const int64_t E0 = __shfl(a0, 0, 64);
+    const int64_t E1 = __shfl(a1, last, 64);
+    s_ub[w][lane] = 0ull;
+    wave_sync();
+    unsigned flags = 0;
+    for (int64_t base = E0; base < E1; base += 64) {
+        const int64_t e = base + lane;
+        bool in = e < E1;
+        if (in && e >= op.nnz_a) {  // row_ptr inconsistent with nnz(A)
+            flags |= kErrInternal;
+            in = false;
+        }
+        int64_t len = 0;
+        if (in) {
+            int64_t s, en;
+            brow(op, __builtin_nontemporal_load(&op.acol[e]), s, en, flags);
+            len = en - s;
+            if (len > INT_MAX) {
+                flags |= kErrLongRow;
+                len = 0;
+            }
+            __builtin_nontemporal_store(s, &ws.ebst[e]);
+            __builtin_nontemporal_store((int32_t)len, &ws.elen[e]);
+        }
+        // row (lane) holding entry e: largest l <= last with a0_l <= e
+        int lo = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const int mid = lo + step;
+            const int64_t am = __shfl(a0, mid > 63 ? 63 : mid, 64);
+            if (mid <= last && am <= e) lo = mid;
+        }
+        const int key = in ? lo : 64 + lane;  // inactive lanes never join a segment
+        int64_t v = len;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t vv = __shfl_up(v, d, 64);
+            const int kk = __shfl_up(key, d, 64);
+            if (lane >= d && kk == key) v += vv;
+        }
+        const int next = __shfl_down(key, 1, 64);
+        if (in && (lane == 63 || next != key)) atomicAdd(&s_ub[w][key], (unsigned long long)v);
+    }
+    wave_sync();
+    const int64_t ub = (int64_t)s_ub[w][lane];
+    int b = -1;
+    if (valid) {
+        b = classify(ub, a1 - a0);
+        ws.ub[r] = ub;
+        if (b == kBinEmpty) {
+            ws.snnz[r] = 0;
+            ws.annz[r] = 0;
+        }
+    }
+    const unsigned long long tot = (unsigned long long)wave_sum(valid ? ub : (int64_t)0);
+    unsigned long long fl = flags;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) fl |= __shfl_xor(fl, d, 64);
+    if (lane == 0) {
+        if (tot) atomicAdd(&ws.ctrl->products, tot);
+        if (fl) atomicOr(&ws.ctrl->flags, fl);
+    }
+#pragma unroll
+    for (int bb = 0; bb < kNumBins; ++bb) {
+        const unsigned long long m = __ballot(b == bb);
+        if (!m) continue;
+        int basei = 0;
+        if (lane == 0) basei = atomicAdd(&ws.ctrl->bin_count[bb], (int)__popcll(m));
+        basei = __shfl(basei, 0, 64);
+        if (bb != kBinEmpty && b == bb)
+            ws.bin_rows[(size_t)bb * (size_t)ws.list_stride + (size_t)(basei + (int)__popcll(m & lanemask_lt()))] =
+                (int32_t)r;
+    }
+}
+
+}  // namespace
+
+void launch_zero_bytes(void *p, size_t bytes, hipStream_t stream) {
+    const size_t n = bytes / 4;
+    if (!n) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)blocks), dim3(256), 0, stream, static_cast<uint32_t *>(p), n);
+}
+
+void launch_zero_control(Control *ctrl, hipStream_t stream) {
+    hipLaunchKernelGGL(k_zero_control, dim3(1), dim3(64), 0, stream, ctrl);
+}
+
+void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream) {
+    if (op.arows <= 0) return;
+    const int64_t tiles = ((int64_t)op.arows + 63) / 64;
+    const int64_t blocks = (tiles + kAnalyzeTpb / 64 - 1) / (kAnalyzeTpb / 64);
+    hipLaunchKernelGGL(k_analyze, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
+}
+
+}  // namespace spg

@@ -3,9 +3,9 @@
 //
 // The context owns growth-only device buffers (no allocation in steady state),
 // a stream, and a pinned mirror of the kernels' control block.  A multiply is:
-//   analyze (per-row work bound and bin)            -> sync 1: errors, bin counts
-//   bin lists, symbolic phase, scan -> C row_ptr   -> sync 2: nnz(C) for allocation
-//   numeric phase                                   -> sync 3: stored entries
+//   analyze (per-row work bound, bins, bin row lists), symbolic phase (exact
+//   distinct-column counts), scan -> C row_ptr      -> sync 1: errors, nnz(C)
+//   numeric phase                                   -> sync 2: stored entries
 //   compaction, only when exact zeros were dropped.
 #include "internal.hpp"
 #include "spgemm_amd.h"
@@ -71,7 +71,8 @@ struct spg_context {
     // uploaded operands (host inputs) and the halo directory
     DevBuf a_ptr, a_col, a_val, b_ptr, b_col, b_val, dir_start, dir_end;
     // workspace
-    DevBuf ub, bin, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc;
+    DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc;
+    int64_t lbits_width = 0;
     // results (ping-pong for compaction)
     DevBuf rowptr[2], ccol[2], cval[2];
     int cur = 0;
@@ -152,6 +153,7 @@ spg_status read_control(spg_context *ctx) {
 }
 
 spg_status flag_error(spg_context *ctx, unsigned long long flags) {
+    if (flags & kErrLongRow) return set_err(ctx, SPG_EOVERFLOW, "a B row has more than INT32_MAX entries");
     if (flags & kErrMissing)
         return set_err(ctx, SPG_EMISSING_ROW, "missing a remote sparse row required by A");
     if (flags & kErrRange)
@@ -168,8 +170,10 @@ spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols
     spg_stats stats{};
     const size_t R = rows > 0 ? (size_t)rows : 1;
     SPG_HIP(ctx->ub.ensure(R * 8, st), "hipMalloc(ub)");
-    SPG_HIP(ctx->bin.ensure(R, st), "hipMalloc(bin)");
-    SPG_HIP(ctx->bin_rows.ensure(R * 4, st), "hipMalloc(bin_rows)");
+    const size_t E = op.nnz_a > 0 ? (size_t)op.nnz_a : 1;
+    SPG_HIP(ctx->ebst.ensure(E * 8, st), "hipMalloc(entry starts)");
+    SPG_HIP(ctx->elen.ensure(E * 4, st), "hipMalloc(entry lengths)");
+    SPG_HIP(ctx->bin_rows.ensure(R * 4 * kNumBins, st), "hipMalloc(bin lists)");
     SPG_HIP(ctx->snnz.ensure(R * 8, st), "hipMalloc(snnz)");
     SPG_HIP(ctx->annz.ensure(R * 8, st), "hipMalloc(annz)");
     SPG_HIP(ctx->scan_tmp.ensure(scan_tmp_elems(rows) * 8, st), "hipMalloc(scan)");
@@ -178,32 +182,24 @@ spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols
     SPG_HIP(ctx->rowptr[out].ensure(((size_t)rows + 1) * 8, st), "hipMalloc(row_ptr C)");
 
     Workspace ws{};
+    ws.ebst = ctx->ebst.as<int64_t>();
+    ws.elen = ctx->elen.as<int32_t>();
     ws.ub = ctx->ub.as<int64_t>();
-    ws.bin = ctx->bin.as<uint8_t>();
     ws.bin_rows = ctx->bin_rows.as<int32_t>();
+    ws.list_stride = (int64_t)R;
     ws.snnz = ctx->snnz.as<int64_t>();
     ws.annz = ctx->annz.as<int64_t>();
     ws.scan_tmp = ctx->scan_tmp.as<int64_t>();
     ws.ctrl = ctx->ctrl.as<Control>();
+    // Large-row path: one column window and one presence bitmap per resident
+    // workgroup (kept all-zero between rows); the accumulators are allocated
+    // only when large rows exist.
     int64_t width = ((int64_t)cols + 31) / 32 * 32;
     const int64_t kMaxWindow = int64_t(1) << 22;
     if (width > kMaxWindow) width = kMaxWindow;
     if (width < 32) width = 32;
     ws.lwidth = width;
-
-    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
-    launch_zero_control(ws.ctrl, st);
-    launch_analyze(op, ws, st);
-    spg_status s = read_control(ctx);
-    if (s != SPG_OK) return s;
-    Control h = *ctx->h_ctrl;
-    if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
-    stats.products = (int64_t)h.products;
-    for (int b = 0; b < kNumBins; ++b) stats.rows_per_bin[b] = h.bin_count[b];
-    stats.bins = kNumBins;
-
-    // Large-row path: bitmap (and later accumulator) per resident workgroup.
-    if (h.bin_count[kBinLarge] > 0) {
+    {
         const size_t per_block = (size_t)width * 8 + (size_t)width / 8;
         size_t budget = ctx->total_mem / 16;
         if (budget > (size_t(4) << 30)) budget = size_t(4) << 30;
@@ -213,17 +209,27 @@ spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols
         if (nb < 1) nb = 1;
         ws.lblocks = (int)nb;
         ctx->lbits.zero_on_grow = true;
+        if (ctx->lbits_width != width) {  // bitmaps are laid out per window width
+            ctx->lbits.release();
+            ctx->lbits_width = width;
+        }
         SPG_HIP(ctx->lbits.ensure(nb * (size_t)(width / 32) * 4, st), "hipMalloc(large bitmaps)");
         ws.lbits = ctx->lbits.as<uint32_t>();
     }
 
-    launch_bin_fill(op, ws, st);
-    launch_symbolic(op, ws, h.bin_count, ctx->cus, st);
+    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+    launch_zero_control(ws.ctrl, st);
+    launch_analyze(op, ws, st);
+    launch_symbolic(op, ws, ctx->cus, st);
     launch_scan(ws.snnz, ctx->rowptr[out].as<int64_t>(), rows, ws.scan_tmp, &ws.ctrl->structural, st);
     if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
-    if ((s = read_control(ctx)) != SPG_OK) return s;
-    h = *ctx->h_ctrl;
+    spg_status s = read_control(ctx);
+    if (s != SPG_OK) return s;
+    Control h = *ctx->h_ctrl;
     if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
+    stats.products = (int64_t)h.products;
+    for (int b = 0; b < kNumBins; ++b) stats.rows_per_bin[b] = h.bin_count[b];
+    stats.bins = kNumBins;
     const int64_t nstruct = (int64_t)h.structural;
     stats.structural_nnz = nstruct;
 
@@ -336,7 +342,7 @@ spg_status spg_destroy(spg_context *ctx) {
     if (!ctx) return SPG_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->a_ptr, &ctx->a_col, &ctx->a_val, &ctx->b_ptr, &ctx->b_col, &ctx->b_val,
-                      &ctx->dir_start, &ctx->dir_end, &ctx->ub, &ctx->bin, &ctx->bin_rows, &ctx->snnz,
+                      &ctx->dir_start, &ctx->dir_end, &ctx->ebst, &ctx->elen, &ctx->ub, &ctx->bin_rows, &ctx->snnz,
                       &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->rowptr[0],
                       &ctx->rowptr[1], &ctx->ccol[0], &ctx->ccol[1], &ctx->cval[0], &ctx->cval[1]};
     for (DevBuf *b : bufs) b->release();
@@ -383,6 +389,7 @@ spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg
 
     Operands op{};
     op.arows = A->rows;
+    op.nnz_a = A->nnz;
     op.bcols = B->cols;
     if ((s = stage(ctx, A, ctx->a_ptr, ctx->a_col, ctx->a_val, op.aptr, op.acol, op.aval)) != SPG_OK) return s;
     const int64_t *bptr = nullptr;

@@ -7,40 +7,41 @@
 
 namespace spg {
 
-// Row bins (by the row's upper-bound work P_row = sum of referenced B row
-// lengths, and by its A-entry count).  Empty rows need no kernel.
+// Row bins, by the row's upper-bound work P_row (sum of the lengths of the B
+// rows its A entries reference) and its A-entry count.
 enum Bin : int {
-    kBinEmpty = 0,   // P_row == 0
-    kBinWave1 = 1,   // one 64-lane workgroup per row, <= kCapWave1 products
-    kBinWave2 = 2,   // one 64-lane workgroup per row, <= kCapWave2 products
-    kBinBlock = 3,   // one 256-lane workgroup per row, <= kCapBlock products
-    kBinLarge = 4,   // chunked fold onto a windowed dense accumulator in HBM
-    kNumBins = 5
+    kBinEmpty = 0,  // P_row == 0: no kernel
+    kBinW1 = 1,     // one 64-lane workgroup per row, <= kCapW1 products
+    kBinW2 = 2,     // one 64-lane workgroup per row, <= kCapW2 products
+    kBinW3 = 3,     // one 64-lane workgroup per row, <= kCapW3 products
+    kBinBlock = 4,  // one 256-lane workgroup per row, <= kCapBlock products
+    kBinLarge = 5,  // chunked fold onto a windowed dense accumulator in HBM
+    kNumBins = 6
 };
 
-constexpr int kCapWave1 = 128;
-constexpr int kCapWave2 = 512;
+constexpr int kCapW1 = 128;
+constexpr int kCapW2 = 256;
+constexpr int kCapW3 = 512;
 constexpr int kCapBlock = 2048;
-constexpr int kCapLarge = 1024;   // products per chunk on the large-row path
+constexpr int kTpbBlock = 256;
+constexpr int kCapLarge = 1024;  // products per chunk on the large-row path
 constexpr int kTpbLarge = 256;
 
 // Error flags raised by kernels (OR-ed into Control::flags).
 enum : unsigned {
     kErrRange = 1u,     // A column outside the B row directory
     kErrMissing = 2u,   // A column names a B row that is neither local nor in the halo
-    kErrInternal = 4u   // capacity/invariant violation (never expected)
+    kErrInternal = 4u,  // capacity/invariant violation (never expected)
+    kErrLongRow = 8u    // a referenced B row has more than INT32_MAX entries
 };
 
 // Small control block in device memory, mirrored to pinned host memory.
 struct Control {
-    unsigned long long products;      // sum of P_row
-    unsigned long long flags;         // kErr* bits
-    unsigned long long actual;        // stored entries after the numeric phase
-    unsigned long long structural;    // distinct coordinates (scan total)
-    int bin_count[8];
-    int bin_cursor[8];
-    int bin_offset[9];
-    int pad[3];
+    unsigned long long products;    // sum of P_row
+    unsigned long long flags;       // kErr* bits
+    unsigned long long actual;      // stored entries after the numeric phase
+    unsigned long long structural;  // distinct coordinates (scan total)
+    int bin_count[8];               // rows listed per bin
 };
 
 // Operands as the kernels see them.  B rows are reached through a directory
@@ -50,6 +51,7 @@ struct Operands {
     const int64_t *aptr;
     const int32_t *acol;
     const double *aval;
+    int64_t nnz_a;
     int32_t arows;
     int32_t ndir;
     const int64_t *bstart;
@@ -60,34 +62,39 @@ struct Operands {
 };
 
 struct Workspace {
+    int64_t *ebst;        // [nnz(A)] start of the B row each A entry references
+    int32_t *elen;        // [nnz(A)] length of that B row (0 for a bad/missing row)
     int64_t *ub;          // [arows] per-row product count
-    uint8_t *bin;         // [arows]
-    int32_t *bin_rows;    // [arows] rows grouped by bin
+    int32_t *bin_rows;    // [kNumBins * list_stride] row lists per bin
+    int64_t list_stride;  // >= arows
     int64_t *snnz;        // [arows] structural count per row
     int64_t *annz;        // [arows] stored count per row
     int64_t *scan_tmp;    // block partials of the scan
     Control *ctrl;
-    uint32_t *lbits;      // [lblocks * lwords] large-row presence bitmaps (kept zero between rows)
+    uint32_t *lbits;      // [lblocks * lwidth/32] large-row presence bitmaps (zero between rows)
     double *lacc;         // [lblocks * lwidth] large-row dense accumulators
     int64_t lwidth;       // column window of the large-row path (multiple of 32)
     int lblocks;
 };
 
-// Launch helpers (kernels.hip).  All are asynchronous on `stream`.
+// Launch helpers.  All are asynchronous on `stream`.
 void launch_zero_bytes(void *p, size_t bytes, hipStream_t stream);  // bytes: multiple of 4
 void launch_zero_control(Control *ctrl, hipStream_t stream);
+// Per-row work bound, bins and bin row lists (analyze.hip).
 void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream);
-void launch_bin_fill(const Operands &op, const Workspace &ws, hipStream_t stream);
-// Symbolic phase for every non-empty bin whose host-side count is nonzero.
-void launch_symbolic(const Operands &op, const Workspace &ws, const int *bin_counts, int cus, hipStream_t stream);
+// Symbolic phase for every bin (symbolic.hip); kernels read their row counts
+// from the control block, so no host synchronisation is needed before it.
+void launch_symbolic(const Operands &op, const Workspace &ws, int cus, hipStream_t stream);
 // Exclusive scan of n int64 counts into out[0..n]; out[n] = total; total also
-// stored to *total_dst when non-null.
+// stored to *total_dst when non-null (scan.hip).
 size_t scan_tmp_elems(int64_t n);
 void launch_scan(const int64_t *in, int64_t *out, int64_t n, int64_t *tmp, unsigned long long *total_dst,
                  hipStream_t stream);
+// Numeric phase for the bins whose host-side counts are nonzero (numeric.hip).
 void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, int cus,
                     const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream);
-// Copy each row's first annz[r] entries from (rowptr_in) to (rowptr_out).
+// Copy each row's entries from (rowptr_in) to (rowptr_out); the count of row r
+// is rowptr_out[r+1] - rowptr_out[r] (scan.hip).
 void launch_compact(int32_t rows, const int64_t *rowptr_in, const int32_t *col_in, const double *val_in,
                     const int64_t *rowptr_out, int32_t *col_out, double *val_out, hipStream_t stream);
 

@@ -1,0 +1,471 @@
+// numeric.hip -- values of C in the reference's serial summation order.
+//
+// Semantics (src/baselines/spgemm_exchange.cpp:274-309): for each output row,
+// products a*b are formed for A entries in stored order and, within each, B
+// entries in stored order; a column's first product initialises its sum and
+// later ones are added left to right; columns come out ascending and sums equal
+// to 0.0 are dropped.  Bit-exactness needs every coordinate summed sequentially
+// in that order with separately rounded multiply and add (built with
+// -ffp-contract=off), so there are no value atomics and no tree reductions.
+//
+// Rows that fit LDS ("ordered ESC"): products are expanded with their serial
+// index s, bucketed by column range with LDS counters and scattered.  Each item
+// then computes its rank inside its (small) bucket by the key (column, s) --
+// its final sorted position -- so the order the LDS atomics placed items in
+// never matters and no thread loops over more than one bucket.  After the
+// items move to their positions, the first position of each column folds the
+// column's products left to right (serial order), and a scan of the nonzero
+// flags in position order gives coalesced output offsets.
+//
+// Larger rows are taken in chunks of the serial order; each chunk is sorted the
+// same way and folded onto a dense per-workgroup accumulator in HBM (columns in
+// windows), so a column's running sum continues where the previous chunk left it.
+#include "device_common.hpp"
+
+namespace spg {
+namespace {
+
+using namespace dev;
+
+template <int TPB, int CAP>
+__global__ __launch_bounds__(TPB) void k_numeric_esc(Operands op, Workspace ws, int binid, const int64_t *crowptr,
+                                                     int32_t *ccol, double *cval) {
+    constexpr int ITEMS = CAP / TPB;
+    static_assert(CAP % TPB == 0 && CAP <= 65536, "CAP must be a multiple of TPB and fit 16-bit indices");
+    __shared__ double s_a[CAP];     // meta: A value of entry e | items: product value
+    __shared__ int64_t s_b64[CAP];  // meta: B start of entry e | items: column, serial index, bucket
+    __shared__ int s_c[CAP + 1];    // meta: product offset of entry e | bucket counts, then bucket starts
+    __shared__ int s_wsi[2 * (TPB / 64) + 2];
+    int32_t *s_col = reinterpret_cast<int32_t *>(s_b64);                // bytes [0, 4 CAP)
+    uint16_t *s_sx = reinterpret_cast<uint16_t *>(s_b64) + 2 * CAP;     // bytes [4 CAP, 6 CAP)
+    uint16_t *s_bk = reinterpret_cast<uint16_t *>(s_b64) + 3 * CAP;     // bytes [6 CAP, 8 CAP)
+    const int tid = threadIdx.x;
+    const int32_t *list = bin_list(ws, binid);
+    const int count = ws.ctrl->bin_count[binid];
+    unsigned long long stored = 0;
+    unsigned flags = 0;
+
+    for (int i = (int)blockIdx.x; i < count; i += (int)gridDim.x) {
+        const int row = list[i];
+        const int64_t a0 = op.aptr[row];
+        const int na = (int)(op.aptr[row + 1] - a0);  // <= CAP by binning
+
+        // --- metadata: B start, A value, exclusive offsets of B row lengths
+        int P = 0, L = 0;
+        for (int e0 = 0; e0 < na; e0 += TPB) {
+            const int e = e0 + tid;
+            int len = 0;
+            if (e < na) {
+                len = __builtin_nontemporal_load(&ws.elen[a0 + e]);
+                s_b64[e] = __builtin_nontemporal_load(&ws.ebst[a0 + e]);
+                s_a[e] = __builtin_nontemporal_load(&op.aval[a0 + e]);
+                L = max(L, len);
+            }
+            int tot;
+            const int ex = block_excl_scan<TPB>(len, s_wsi, tot);
+            if (e < na) s_c[e] = P + ex;
+            P += tot;
+        }
+        L = block_max<TPB>(L, s_wsi);
+        if (P > CAP) {  // cannot happen: binning used the same lengths
+            flags |= kErrInternal;
+            P = CAP;
+        }
+        if (tid == 0) s_c[na] = P;
+        __syncthreads();
+
+        // --- expansion into registers: column, product, serial index s
+        int icol[ITEMS];
+        double ival[ITEMS];
+        int isx[ITEMS];
+        int cmn = INT_MAX, cmx = INT_MIN;
+        int Lp = 1;
+        while (Lp < L) Lp <<= 1;
+        const int logL = 31 - __clz(Lp);
+        const int G = Lp <= TPB ? (TPB >> logL) : 1;
+        const bool entry_major = Lp <= TPB && (na + G - 1) / G <= ITEMS;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            isx[j] = -1;
+            icol[j] = 0;
+            ival[j] = 0.0;
+            int e = -1, t = 0;
+            if (entry_major) {
+                const int ee = j * G + (tid >> logL);
+                const int tt = tid & (Lp - 1);
+                if (ee < na && tt < s_c[ee + 1] - s_c[ee]) {
+                    e = ee;
+                    t = tt;
+                }
+            } else {
+                const int s = tid + j * TPB;
+                if (s < P) {
+                    e = upper_entry(s_c, na, s);
+                    t = s - s_c[e];
+                }
+            }
+            if (e >= 0) {
+                const int64_t idx = s_b64[e] + t;
+                const int c = op.bcol[idx];
+                icol[j] = c;
+                ival[j] = s_a[e] * op.bval[idx];
+                isx[j] = s_c[e] + t;
+                cmn = min(cmn, c);
+                cmx = max(cmx, c);
+            }
+        }
+        block_minmax<TPB>(cmn, cmx, s_wsi);
+        __syncthreads();  // metadata no longer needed
+
+        // --- buckets over the row's column range (power of two, nb in [TPB, CAP])
+        int nb = TPB;
+        while (nb < P && nb < CAP) nb <<= 1;
+        const int lognb = 31 - __clz(nb);
+        const uint64_t range = (uint64_t)((int64_t)cmx - (int64_t)cmn) + 1ull;
+        const int lr = range <= 1 ? 0 : 64 - __clzll((long long)(range - 1));
+        const int sh = lr > lognb ? lr - lognb : 0;
+        for (int t = tid; t < nb; t += TPB) s_c[t] = 0;
+        __syncthreads();
+        int bk[ITEMS], pos[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            bk[j] = 0;
+            pos[j] = 0;
+            if (isx[j] >= 0) {
+                bk[j] = (int)((uint32_t)((int64_t)icol[j] - (int64_t)cmn) >> sh);
+                pos[j] = atomicAdd(&s_c[bk[j]], 1);
+            }
+        }
+        __syncthreads();
+        {
+            int run = 0;
+            for (int q = 0; q < nb; q += TPB) {
+                const int b = q + tid;
+                const int c = s_c[b];
+                int tot;
+                const int ex = block_excl_scan<TPB>(c, s_wsi, tot);
+                s_c[b] = run + ex;
+                run += tot;
+            }
+        }
+        if (tid == 0) s_c[nb] = P;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            if (isx[j] >= 0) {
+                const int d = s_c[bk[j]] + pos[j];
+                s_col[d] = icol[j];
+                s_sx[d] = (uint16_t)isx[j];
+                s_bk[d] = (uint16_t)bk[j];
+                s_a[d] = ival[j];
+            }
+        }
+        __syncthreads();
+
+        // --- rank of each item inside its bucket by (column, s): its sorted position
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int x = tid + j * TPB;
+            isx[j] = -1;
+            if (x < P) {
+                const int32_t c = s_col[x];
+                const int sx = s_sx[x];
+                const int b = s_bk[x];
+                const int st = s_c[b], en = s_c[b + 1];
+                int rank = 0;
+                for (int y = st; y < en; ++y) {
+                    const int32_t cy = s_col[y];
+                    rank += (cy < c) | ((cy == c) & (s_sx[y] < sx));
+                }
+                icol[j] = c;
+                ival[j] = s_a[x];
+                isx[j] = st + rank;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            if (isx[j] >= 0) {
+                s_col[isx[j]] = icol[j];
+                s_a[isx[j]] = ival[j];
+            }
+        }
+        __syncthreads();
+
+        // --- ordered fold at each column's first position, scan, coalesced write
+        const int64_t cbase = crowptr[row];
+        const int64_t lim = crowptr[row + 1] - cbase;
+        int run = 0;
+        for (int x0 = 0; x0 < P; x0 += TPB) {
+            const int x = x0 + tid;
+            int nz = 0;
+            int32_t c = 0;
+            double v = 0.0;
+            if (x < P) {
+                c = s_col[x];
+                if (x == 0 || s_col[x - 1] != c) {
+                    v = s_a[x];
+                    for (int y = x + 1; y < P && s_col[y] == c; ++y) v = v + s_a[y];
+                    nz = v != 0.0;
+                }
+            }
+            int tot;
+            const int ex = block_excl_scan<TPB>(nz, s_wsi, tot);
+            if (nz) {
+                const int64_t o = (int64_t)run + ex;
+                if (o < lim) {
+                    __builtin_nontemporal_store(c, &ccol[cbase + o]);
+                    __builtin_nontemporal_store(v, &cval[cbase + o]);
+                }
+            }
+            run += tot;
+        }
+        if (tid == 0) {
+            ws.annz[row] = run;
+            stored += (unsigned long long)run;
+            if (run > lim) flags |= kErrInternal;
+        }
+        __syncthreads();  // LDS is reused by the next row
+    }
+    if (tid == 0 && stored) atomicAdd(&ws.ctrl->actual, stored);
+    if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
+}
+
+template <int TPB, int CAP>
+__device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Workspace &ws, int64_t e0, int nbt,
+                                                    int64_t *s_bst, double *s_av, int64_t *s_off, int64_t *s_ws64) {
+    const int tid = threadIdx.x;
+    const int ipt = (nbt + TPB - 1) / TPB;
+    const int f = tid * ipt;
+    int64_t mylen = 0;
+    for (int j = 0; j < ipt; ++j) {
+        const int e = f + j;
+        if (e < nbt) {
+            const int64_t len = ws.elen[e0 + e];
+            s_bst[e] = ws.ebst[e0 + e];
+            s_av[e] = op.aval[e0 + e];
+            s_off[e] = len;
+            mylen += len;
+        }
+    }
+    int64_t Pb;
+    int64_t base = block_excl_scan<TPB>(mylen, s_ws64, Pb);
+    for (int j = 0; j < ipt; ++j) {
+        const int e = f + j;
+        if (e < nbt) {
+            const int64_t l = s_off[e];
+            s_off[e] = base;
+            base += l;
+        }
+    }
+    __syncthreads();
+    return Pb;
+}
+
+template <int TPB, int CAP>
+__global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws, const int64_t *crowptr,
+                                                       int32_t *ccol, double *cval) {
+    constexpr int ITEMS = CAP / TPB;
+    __shared__ int64_t s_bst[CAP];
+    __shared__ double s_av[CAP];
+    __shared__ int64_t s_off[CAP + 1];
+    __shared__ uint64_t s_key[CAP];
+    __shared__ double s_val[CAP];
+    __shared__ int s_cnt[CAP + 1];
+    __shared__ int s_wsi[2 * (TPB / 64) + 2];
+    __shared__ int64_t s_ws64[TPB / 64 + 1];
+    const int tid = threadIdx.x;
+    const int32_t *list = bin_list(ws, kBinLarge);
+    const int count = ws.ctrl->bin_count[kBinLarge];
+    const int64_t W = ws.lwidth;
+    uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
+    double *acc = ws.lacc + (size_t)blockIdx.x * (size_t)W;
+    unsigned flags = 0;
+    unsigned long long stored = 0;
+    for (int i = (int)blockIdx.x; i < count; i += (int)gridDim.x) {
+        const int row = list[i];
+        const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
+        const int64_t cbase = crowptr[row];
+        const int64_t lim = crowptr[row + 1] - cbase;
+        int64_t outn = 0;
+        for (int64_t w0 = 0; w0 < op.bcols; w0 += W) {
+            const int64_t wend = min(w0 + W, (int64_t)op.bcols);
+            for (int64_t e0 = a0; e0 < a1; e0 += CAP) {
+                const int nbt = (int)min((int64_t)CAP, a1 - e0);
+                const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, s_av, s_off, s_ws64);
+                for (int64_t c0 = 0; c0 < Pb; c0 += CAP) {
+                    const int Pc = (int)min((int64_t)CAP, Pb - c0);
+                    int icol[ITEMS];
+                    double ival[ITEMS];
+                    bool ok[ITEMS];
+                    int cmn = INT_MAX, cmx = INT_MIN, nvalid = 0;
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        const int s = tid + j * TPB;
+                        icol[j] = 0;
+                        ival[j] = 0.0;
+                        ok[j] = false;
+                        if (s < Pc) {
+                            const int64_t g = c0 + s;
+                            const int e = upper_entry(s_off, nbt, g);
+                            const int64_t idx = s_bst[e] + (g - s_off[e]);
+                            const int c = op.bcol[idx];
+                            if (c >= w0 && c < wend) {
+                                icol[j] = c;
+                                ival[j] = s_av[e] * op.bval[idx];
+                                ok[j] = true;
+                                cmn = min(cmn, c);
+                                cmx = max(cmx, c);
+                                ++nvalid;
+                            }
+                        }
+                    }
+                    const int V = block_sum<TPB>(nvalid, s_wsi);
+                    if (V == 0) continue;  // uniform across the block
+                    block_minmax<TPB>(cmn, cmx, s_wsi);
+                    int nb = TPB;
+                    while (nb < V && nb < CAP) nb <<= 1;
+                    const int lognb = 31 - __clz(nb);
+                    const uint64_t range = (uint64_t)((int64_t)cmx - (int64_t)cmn) + 1ull;
+                    const int lr = range <= 1 ? 0 : 64 - __clzll((long long)(range - 1));
+                    const int sh = lr > lognb ? lr - lognb : 0;
+                    for (int t = tid; t < nb; t += TPB) s_cnt[t] = 0;
+                    __syncthreads();
+                    int bk[ITEMS], pos[ITEMS];
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        bk[j] = 0;
+                        pos[j] = 0;
+                        if (ok[j]) {
+                            bk[j] = (int)((uint32_t)((int64_t)icol[j] - (int64_t)cmn) >> sh);
+                            pos[j] = atomicAdd(&s_cnt[bk[j]], 1);
+                        }
+                    }
+                    __syncthreads();
+                    {
+                        int run = 0;
+                        for (int q = 0; q < nb; q += TPB) {
+                            const int b = q + tid;
+                            const int c = s_cnt[b];
+                            int tot;
+                            const int ex = block_excl_scan<TPB>(c, s_wsi, tot);
+                            s_cnt[b] = run + ex;
+                            run += tot;
+                        }
+                    }
+                    if (tid == 0) s_cnt[nb] = V;
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        if (ok[j]) {
+                            const int s = tid + j * TPB;
+                            const int d = s_cnt[bk[j]] + pos[j];
+                            s_key[d] = ((uint64_t)(uint32_t)icol[j] << 32) | (uint32_t)s;
+                            s_val[d] = ival[j];
+                        }
+                    }
+                    __syncthreads();
+                    for (int q = 0; q < nb; q += TPB) {
+                        const int b = q + tid;
+                        const int lo = s_cnt[b];
+                        const int hi = s_cnt[b + 1];
+                        for (int x = lo + 1; x < hi; ++x) {
+                            const uint64_t k = s_key[x];
+                            const double v = s_val[x];
+                            int y = x - 1;
+                            while (y >= lo && s_key[y] > k) {
+                                s_key[y + 1] = s_key[y];
+                                s_val[y + 1] = s_val[y];
+                                --y;
+                            }
+                            s_key[y + 1] = k;
+                            s_val[y + 1] = v;
+                        }
+                        for (int x = lo; x < hi;) {
+                            const uint32_t c = (uint32_t)(s_key[x] >> 32);
+                            const uint32_t rel = (uint32_t)((int64_t)(int32_t)c - w0);
+                            const uint32_t bit = 1u << (rel & 31u);
+                            const bool present = (ld_bits(&bits[rel >> 5]) & bit) != 0u;
+                            double v = present ? acc[rel] : 0.0;
+                            while (x < hi && (uint32_t)(s_key[x] >> 32) == c) {
+                                v = v + s_val[x];
+                                ++x;
+                            }
+                            acc[rel] = v;
+                            if (!present) atomicOr(&bits[rel >> 5], bit);
+                        }
+                    }
+                    __syncthreads();  // chunk folded; LDS and accumulator state consistent
+                }
+            }
+            // Extraction of the window in ascending column order, zeros dropped.
+            const int64_t nwords = (wend - w0 + 31) / 32;
+            const int64_t wpt = (nwords + TPB - 1) / TPB;
+            const int64_t q0 = (int64_t)tid * wpt;
+            const int64_t q1 = min(q0 + wpt, nwords);
+            int64_t mine = 0;
+            for (int64_t q = q0; q < q1; ++q) {
+                uint32_t m = ld_bits(&bits[q]);
+                while (m) {
+                    const int b = __ffs((int)m) - 1;
+                    m &= m - 1u;
+                    if (acc[q * 32 + b] != 0.0) ++mine;
+                }
+            }
+            int64_t wtotal;
+            int64_t o = outn + block_excl_scan<TPB>(mine, s_ws64, wtotal);
+            for (int64_t q = q0; q < q1; ++q) {
+                uint32_t m = ld_bits(&bits[q]);
+                while (m) {
+                    const int b = __ffs((int)m) - 1;
+                    m &= m - 1u;
+                    const double v = acc[q * 32 + b];
+                    if (v != 0.0) {
+                        if (o < lim) {
+                            ccol[cbase + o] = (int32_t)(w0 + q * 32 + b);
+                            cval[cbase + o] = v;
+                        }
+                        ++o;
+                    }
+                }
+                st_bits(&bits[q], 0u);
+            }
+            outn += wtotal;
+            __threadfence();
+            __syncthreads();
+        }
+        if (tid == 0) {
+            ws.annz[row] = outn;
+            stored += (unsigned long long)outn;
+            if (outn > lim) flags |= kErrInternal;
+        }
+    }
+    if (tid == 0 && stored) atomicAdd(&ws.ctrl->actual, stored);
+    if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
+}
+
+template <int TPB, int CAP>
+void launch_esc(const Operands &op, const Workspace &ws, int bin, int rows, int cus, const int64_t *crowptr,
+                int32_t *ccol, double *cval, hipStream_t stream) {
+    if (rows <= 0) return;
+    auto k = k_numeric_esc<TPB, CAP>;
+    const int g = occupancy_grid(k, TPB, cus, rows);
+    hipLaunchKernelGGL(k, dim3(g), dim3(TPB), 0, stream, op, ws, bin, crowptr, ccol, cval);
+}
+
+}  // namespace
+
+void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, int cus,
+                    const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream) {
+    launch_esc<64, kCapW1>(op, ws, kBinW1, bin_counts[kBinW1], cus, crowptr, ccol, cval, stream);
+    launch_esc<64, kCapW2>(op, ws, kBinW2, bin_counts[kBinW2], cus, crowptr, ccol, cval, stream);
+    launch_esc<64, kCapW3>(op, ws, kBinW3, bin_counts[kBinW3], cus, crowptr, ccol, cval, stream);
+    launch_esc<kTpbBlock, kCapBlock>(op, ws, kBinBlock, bin_counts[kBinBlock], cus, crowptr, ccol, cval, stream);
+    if (bin_counts[kBinLarge] > 0) {
+        const int g = bin_counts[kBinLarge] < ws.lblocks ? bin_counts[kBinLarge] : ws.lblocks;
+        hipLaunchKernelGGL((k_large_numeric<kTpbLarge, kCapLarge>), dim3(g), dim3(kTpbLarge), 0, stream, op, ws,
+                           crowptr, ccol, cval);
+    }
+}
+
+}  // namespace spg
