@@ -109,9 +109,13 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
         int basei = 0;
         if (lane == 0) basei = atomicAdd(&ws.ctrl->bin_count[bb], (int)__popcll(m));
         basei = __shfl(basei, 0, 64);
-        if (bb != kBinEmpty && b == bb)
-            ws.bin_rows[(size_t)bb * (size_t)ws.list_stride + (size_t)(basei + (int)__popcll(m & lanemask_lt()))] =
-                (int32_t)r;
+        if (bb != kBinEmpty && b == bb) {
+            RowRec rec;
+            rec.row = (int32_t)r;
+            rec.na = (int32_t)min(a1 - a0, (int64_t)INT_MAX);
+            rec.a0 = a0;
+            ws.bin_recs[(size_t)bb * (size_t)ws.list_stride + (size_t)(basei + (int)__popcll(m & lanemask_lt()))] = rec;
+        }
     }
 }
 

@@ -123,6 +123,10 @@ __device__ __forceinline__ void block_minmax(int &mn, int &mx, int *ws) {
     }
 }
 
+// Block-uniform value held in a scalar register (the value must already be
+// equal in every lane; this lets the compiler branch on it with scalar code).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // B row k through the directory; flags record a bad column or a missing row.
 __device__ __forceinline__ void brow(const Operands &op, int32_t k, int64_t &s, int64_t &e, unsigned &flags) {
     if (k < 0 || k >= op.ndir) {
@@ -152,6 +156,46 @@ __device__ __forceinline__ int upper_entry(const T *off, int n, T s) {
     return lo;
 }
 
+// Where the products of a row go, slot by slot: slot j of thread tid holds the
+// product (A entry e[j], B offset t[j]) when ok[j].  Entry-major when every
+// referenced B row fits a power-of-two lane group (L lanes per A entry, no
+// search, contiguous B reads); otherwise products are dealt out in serial
+// order (s = tid + j*TPB) and located by a binary search over the offsets.
+// off[e] is the exclusive offset of entry e's products, off[na] = P.
+template <int TPB, int ITEMS>
+__device__ __forceinline__ void slot_products(int na, int P, int L, const int *off, int (&e)[ITEMS], int (&t)[ITEMS],
+                                              bool (&ok)[ITEMS]) {
+    const int tid = threadIdx.x;
+    int Lp = 1;
+    while (Lp < L) Lp <<= 1;
+    Lp = uni(Lp);
+    const int logL = 31 - __clz(Lp);
+    const int G = Lp <= TPB ? (TPB >> logL) : 1;
+    const bool entry_major = Lp <= TPB && (na + G - 1) / G <= ITEMS;
+    if (entry_major) {
+        const int sub = tid & (Lp - 1);
+        const int grp = tid >> logL;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int ee = j * G + grp;
+            const bool in = ee < na;
+            const int len = in ? off[ee + 1] - off[ee] : 0;
+            ok[j] = in && sub < len;
+            e[j] = in ? ee : 0;
+            t[j] = sub;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int s = tid + j * TPB;
+            ok[j] = s < P;
+            const int ee = ok[j] ? upper_entry(off, na, s) : 0;
+            e[j] = ee;
+            t[j] = ok[j] ? s - off[ee] : 0;
+        }
+    }
+}
+
 __device__ __forceinline__ int classify(int64_t ub, int64_t na) {
     if (ub == 0) return kBinEmpty;
     if (ub <= kCapW1 && na <= kCapW1) return kBinW1;
@@ -171,8 +215,83 @@ __device__ __forceinline__ void st_bits(uint32_t *p, uint32_t v) {
 }
 
 // Rows of one bin: list and count (filled by the analyze pass).
-__device__ __forceinline__ const int32_t *bin_list(const Workspace &ws, int bin) {
-    return ws.bin_rows + (size_t)bin * (size_t)ws.list_stride;
+__device__ __forceinline__ const RowRec *bin_list(const Workspace &ws, int bin) {
+    return ws.bin_recs + (size_t)bin * (size_t)ws.list_stride;
+}
+
+// Record i (block-uniform i) is fetched as one dword per lane (lanes 0..3 of
+// each wave), i.e. with a vector load whose latency is tracked apart from LDS
+// traffic, and decoded with shuffles only when it is needed.
+__device__ __forceinline__ int rec_word(const RowRec *list, int i) {
+    return reinterpret_cast<const int *>(list + i)[threadIdx.x & 3];
+}
+__device__ __forceinline__ RowRec rec_decode(int w) {
+    RowRec r;
+    r.row = uni(__shfl(w, 0, 64));
+    r.na = uni(__shfl(w, 1, 64));
+    const uint32_t lo = (uint32_t)uni(__shfl(w, 2, 64));
+    const uint32_t hi = (uint32_t)uni(__shfl(w, 3, 64));
+    r.a0 = (int64_t)(((uint64_t)hi << 32) | lo);
+    return r;
+}
+
+// A row's per-entry metadata held in registers (thread e holds entry e), so
+// the next row's metadata can be in flight while the current row computes.
+struct MetaPre {
+    int64_t bst;
+    int len;
+    double av;
+};
+
+// Loads are unconditional (lanes past the last entry re-read the row's last
+// entry) so the destination registers never need a prior initialisation,
+// which would force a wait on loads still in flight.
+template <bool VALUES>
+__device__ __forceinline__ MetaPre meta_prefetch(const Operands &op, const Workspace &ws, const RowRec &r) {
+    MetaPre m;
+    const int64_t e = r.a0 + min((int)threadIdx.x, max(r.na - 1, 0));
+    m.bst = __builtin_nontemporal_load(&ws.ebst[e]);
+    m.len = __builtin_nontemporal_load(&ws.elen[e]);
+    m.av = VALUES ? __builtin_nontemporal_load(&op.aval[e]) : 0.0;
+    return m;
+}
+
+// Row metadata into LDS: B starts, (A values), exclusive product offsets
+// off[0..na] and the longest referenced B row.  Entries < TPB come from the
+// prefetched registers when `pre_ok`.
+template <int TPB, bool VALUES>
+__device__ __forceinline__ void meta_to_lds(const Operands &op, const Workspace &ws, const RowRec &r,
+                                            const MetaPre &pre, bool pre_ok, int64_t *s_bst, double *s_av,
+                                            int *s_off, int *s_wsi, int &P, int &L) {
+    const int tid = threadIdx.x;
+    P = 0;
+    L = 0;
+    for (int e0 = 0; e0 < r.na; e0 += TPB) {
+        const int e = e0 + tid;
+        int len = 0;
+        if (e < r.na) {
+            int64_t bst;
+            double av = 0.0;
+            if (pre_ok && e0 == 0) {
+                bst = pre.bst;
+                len = pre.len;
+                av = pre.av;
+            } else {
+                bst = __builtin_nontemporal_load(&ws.ebst[r.a0 + e]);
+                len = __builtin_nontemporal_load(&ws.elen[r.a0 + e]);
+                if (VALUES) av = __builtin_nontemporal_load(&op.aval[r.a0 + e]);
+            }
+            s_bst[e] = bst;
+            if (VALUES) s_av[e] = av;
+            L = max(L, len);
+        }
+        int tot;
+        const int ex = block_excl_scan<TPB>(len, s_wsi, tot);
+        if (e < r.na) s_off[e] = P + ex;
+        P += tot;
+    }
+    P = uni(P);
+    L = uni(block_max<TPB>(L, s_wsi));
 }
 
 template <typename K>

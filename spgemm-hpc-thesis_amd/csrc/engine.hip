@@ -173,7 +173,7 @@ spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols
     const size_t E = op.nnz_a > 0 ? (size_t)op.nnz_a : 1;
     SPG_HIP(ctx->ebst.ensure(E * 8, st), "hipMalloc(entry starts)");
     SPG_HIP(ctx->elen.ensure(E * 4, st), "hipMalloc(entry lengths)");
-    SPG_HIP(ctx->bin_rows.ensure(R * 4 * kNumBins, st), "hipMalloc(bin lists)");
+    SPG_HIP(ctx->bin_rows.ensure(R * sizeof(RowRec) * kNumBins, st), "hipMalloc(bin lists)");
     SPG_HIP(ctx->snnz.ensure(R * 8, st), "hipMalloc(snnz)");
     SPG_HIP(ctx->annz.ensure(R * 8, st), "hipMalloc(annz)");
     SPG_HIP(ctx->scan_tmp.ensure(scan_tmp_elems(rows) * 8, st), "hipMalloc(scan)");
@@ -185,7 +185,7 @@ spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols
     ws.ebst = ctx->ebst.as<int64_t>();
     ws.elen = ctx->elen.as<int32_t>();
     ws.ub = ctx->ub.as<int64_t>();
-    ws.bin_rows = ctx->bin_rows.as<int32_t>();
+    ws.bin_recs = ctx->bin_rows.as<RowRec>();
     ws.list_stride = (int64_t)R;
     ws.snnz = ctx->snnz.as<int64_t>();
     ws.annz = ctx->annz.as<int64_t>();

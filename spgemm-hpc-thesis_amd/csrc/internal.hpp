@@ -61,11 +61,19 @@ struct Operands {
     int32_t bcols;
 };
 
+// One listed row of a bin: its index, its A-entry count and its first A entry
+// (so kernels need not load A's row offsets again).
+struct RowRec {
+    int32_t row;
+    int32_t na;   // A entries (saturated at INT32_MAX; only bins <= kCapBlock rely on it)
+    int64_t a0;
+};
+
 struct Workspace {
     int64_t *ebst;        // [nnz(A)] start of the B row each A entry references
     int32_t *elen;        // [nnz(A)] length of that B row (0 for a bad/missing row)
     int64_t *ub;          // [arows] per-row product count
-    int32_t *bin_rows;    // [kNumBins * list_stride] row lists per bin
+    RowRec *bin_recs;     // [kNumBins * list_stride] row records per bin
     int64_t list_stride;  // >= arows
     int64_t *snnz;        // [arows] structural count per row
     int64_t *annz;        // [arows] stored count per row

@@ -28,204 +28,218 @@ namespace {
 using namespace dev;
 
 template <int TPB, int CAP>
-__global__ __launch_bounds__(TPB) void k_numeric_esc(Operands op, Workspace ws, int binid, const int64_t *crowptr,
-                                                     int32_t *ccol, double *cval) {
+__device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws, const RowRec &cur, int P, int L,
+                                        const int64_t *crowptr, int32_t *ccol, double *cval, double *s_a,
+                                        int64_t *s_b64, int *s_c, int *s_wsi, unsigned long long &stored,
+                                        unsigned &flags) {
     constexpr int ITEMS = CAP / TPB;
-    static_assert(CAP % TPB == 0 && CAP <= 65536, "CAP must be a multiple of TPB and fit 16-bit indices");
-    __shared__ double s_a[CAP];     // meta: A value of entry e | items: product value
-    __shared__ int64_t s_b64[CAP];  // meta: B start of entry e | items: column, serial index, bucket
-    __shared__ int s_c[CAP + 1];    // meta: product offset of entry e | bucket counts, then bucket starts
-    __shared__ int s_wsi[2 * (TPB / 64) + 2];
-    int32_t *s_col = reinterpret_cast<int32_t *>(s_b64);                // bytes [0, 4 CAP)
-    uint16_t *s_sx = reinterpret_cast<uint16_t *>(s_b64) + 2 * CAP;     // bytes [4 CAP, 6 CAP)
-    uint16_t *s_bk = reinterpret_cast<uint16_t *>(s_b64) + 3 * CAP;     // bytes [6 CAP, 8 CAP)
+    int32_t *s_col = reinterpret_cast<int32_t *>(s_b64);             // bytes [0, 4 CAP)
+    uint16_t *s_sx = reinterpret_cast<uint16_t *>(s_b64) + 2 * CAP;  // bytes [4 CAP, 6 CAP)
+    uint16_t *s_bk = reinterpret_cast<uint16_t *>(s_b64) + 3 * CAP;  // bytes [6 CAP, 8 CAP)
     const int tid = threadIdx.x;
-    const int32_t *list = bin_list(ws, binid);
-    const int count = ws.ctrl->bin_count[binid];
-    unsigned long long stored = 0;
-    unsigned flags = 0;
+    const int na = cur.na;
 
-    for (int i = (int)blockIdx.x; i < count; i += (int)gridDim.x) {
-        const int row = list[i];
-        const int64_t a0 = op.aptr[row];
-        const int na = (int)(op.aptr[row + 1] - a0);  // <= CAP by binning
-
-        // --- metadata: B start, A value, exclusive offsets of B row lengths
-        int P = 0, L = 0;
-        for (int e0 = 0; e0 < na; e0 += TPB) {
-            const int e = e0 + tid;
-            int len = 0;
-            if (e < na) {
-                len = __builtin_nontemporal_load(&ws.elen[a0 + e]);
-                s_b64[e] = __builtin_nontemporal_load(&ws.ebst[a0 + e]);
-                s_a[e] = __builtin_nontemporal_load(&op.aval[a0 + e]);
-                L = max(L, len);
-            }
-            int tot;
-            const int ex = block_excl_scan<TPB>(len, s_wsi, tot);
-            if (e < na) s_c[e] = P + ex;
-            P += tot;
-        }
-        L = block_max<TPB>(L, s_wsi);
-        if (P > CAP) {  // cannot happen: binning used the same lengths
-            flags |= kErrInternal;
-            P = CAP;
-        }
-        if (tid == 0) s_c[na] = P;
-        __syncthreads();
-
-        // --- expansion into registers: column, product, serial index s
-        int icol[ITEMS];
-        double ival[ITEMS];
-        int isx[ITEMS];
-        int cmn = INT_MAX, cmx = INT_MIN;
-        int Lp = 1;
-        while (Lp < L) Lp <<= 1;
-        const int logL = 31 - __clz(Lp);
-        const int G = Lp <= TPB ? (TPB >> logL) : 1;
-        const bool entry_major = Lp <= TPB && (na + G - 1) / G <= ITEMS;
+    // --- expansion into registers: column, product, serial index s
+    int icol[ITEMS], isx[ITEMS];
+    double ival[ITEMS];
+    int cmn = INT_MAX, cmx = INT_MIN;
+    {
+        // Every slot's LDS reads and B loads are issued before any product is
+        // formed, so the row's gathers are in flight together.  Empty slots
+        // read B entry 0 (the row's products exist, so B is not empty).
+        int e[ITEMS], t[ITEMS];
+        bool ok[ITEMS];
+        slot_products<TPB, ITEMS>(na, P, L, s_c, e, t, ok);
+        int64_t idx[ITEMS];
+        double av[ITEMS], bv[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
-            isx[j] = -1;
-            icol[j] = 0;
-            ival[j] = 0.0;
-            int e = -1, t = 0;
-            if (entry_major) {
-                const int ee = j * G + (tid >> logL);
-                const int tt = tid & (Lp - 1);
-                if (ee < na && tt < s_c[ee + 1] - s_c[ee]) {
-                    e = ee;
-                    t = tt;
-                }
-            } else {
-                const int s = tid + j * TPB;
-                if (s < P) {
-                    e = upper_entry(s_c, na, s);
-                    t = s - s_c[e];
-                }
-            }
-            if (e >= 0) {
-                const int64_t idx = s_b64[e] + t;
-                const int c = op.bcol[idx];
-                icol[j] = c;
-                ival[j] = s_a[e] * op.bval[idx];
-                isx[j] = s_c[e] + t;
-                cmn = min(cmn, c);
-                cmx = max(cmx, c);
-            }
+            idx[j] = ok[j] ? s_b64[e[j]] + t[j] : 0;
+            av[j] = s_a[e[j]];
+            isx[j] = ok[j] ? s_c[e[j]] + t[j] : -1;
         }
-        block_minmax<TPB>(cmn, cmx, s_wsi);
-        __syncthreads();  // metadata no longer needed
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            icol[j] = op.bcol[idx[j]];
+            bv[j] = op.bval[idx[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            ival[j] = av[j] * bv[j];
+            cmn = ok[j] ? min(cmn, icol[j]) : cmn;
+            cmx = ok[j] ? max(cmx, icol[j]) : cmx;
+        }
+    }
+    block_minmax<TPB>(cmn, cmx, s_wsi);
+    __syncthreads();  // metadata no longer needed
 
-        // --- buckets over the row's column range (power of two, nb in [TPB, CAP])
-        int nb = TPB;
-        while (nb < P && nb < CAP) nb <<= 1;
-        const int lognb = 31 - __clz(nb);
-        const uint64_t range = (uint64_t)((int64_t)cmx - (int64_t)cmn) + 1ull;
-        const int lr = range <= 1 ? 0 : 64 - __clzll((long long)(range - 1));
-        const int sh = lr > lognb ? lr - lognb : 0;
-        for (int t = tid; t < nb; t += TPB) s_c[t] = 0;
-        __syncthreads();
-        int bk[ITEMS], pos[ITEMS];
+    // --- buckets over the row's column range (power of two, nb in [TPB, CAP])
+    int nb = TPB;
+    while (nb < P && nb < CAP) nb <<= 1;
+    nb = uni(nb);
+    cmn = uni(cmn);
+    cmx = uni(cmx);
+    const int lognb = 31 - __clz(nb);
+    const uint64_t range = (uint64_t)((int64_t)cmx - (int64_t)cmn) + 1ull;
+    const int lr = range <= 1 ? 0 : 64 - __clzll((long long)(range - 1));
+    const int sh = uni(lr > lognb ? lr - lognb : 0);
+    for (int t = tid; t < nb; t += TPB) s_c[t] = 0;
+    __syncthreads();
+    int bk[ITEMS], pos[ITEMS];
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            bk[j] = 0;
-            pos[j] = 0;
-            if (isx[j] >= 0) {
-                bk[j] = (int)((uint32_t)((int64_t)icol[j] - (int64_t)cmn) >> sh);
-                pos[j] = atomicAdd(&s_c[bk[j]], 1);
-            }
+    for (int j = 0; j < ITEMS; ++j) {
+        bk[j] = 0;
+        pos[j] = 0;
+        if (isx[j] >= 0) {
+            bk[j] = (int)((uint32_t)((int64_t)icol[j] - (int64_t)cmn) >> sh);
+            pos[j] = atomicAdd(&s_c[bk[j]], 1);
         }
-        __syncthreads();
-        {
-            int run = 0;
-            for (int q = 0; q < nb; q += TPB) {
-                const int b = q + tid;
-                const int c = s_c[b];
-                int tot;
-                const int ex = block_excl_scan<TPB>(c, s_wsi, tot);
-                s_c[b] = run + ex;
-                run += tot;
-            }
-        }
-        if (tid == 0) s_c[nb] = P;
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            if (isx[j] >= 0) {
-                const int d = s_c[bk[j]] + pos[j];
-                s_col[d] = icol[j];
-                s_sx[d] = (uint16_t)isx[j];
-                s_bk[d] = (uint16_t)bk[j];
-                s_a[d] = ival[j];
-            }
-        }
-        __syncthreads();
-
-        // --- rank of each item inside its bucket by (column, s): its sorted position
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const int x = tid + j * TPB;
-            isx[j] = -1;
-            if (x < P) {
-                const int32_t c = s_col[x];
-                const int sx = s_sx[x];
-                const int b = s_bk[x];
-                const int st = s_c[b], en = s_c[b + 1];
-                int rank = 0;
-                for (int y = st; y < en; ++y) {
-                    const int32_t cy = s_col[y];
-                    rank += (cy < c) | ((cy == c) & (s_sx[y] < sx));
-                }
-                icol[j] = c;
-                ival[j] = s_a[x];
-                isx[j] = st + rank;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            if (isx[j] >= 0) {
-                s_col[isx[j]] = icol[j];
-                s_a[isx[j]] = ival[j];
-            }
-        }
-        __syncthreads();
-
-        // --- ordered fold at each column's first position, scan, coalesced write
-        const int64_t cbase = crowptr[row];
-        const int64_t lim = crowptr[row + 1] - cbase;
+    }
+    __syncthreads();
+    {
         int run = 0;
-        for (int x0 = 0; x0 < P; x0 += TPB) {
-            const int x = x0 + tid;
-            int nz = 0;
-            int32_t c = 0;
-            double v = 0.0;
-            if (x < P) {
-                c = s_col[x];
-                if (x == 0 || s_col[x - 1] != c) {
-                    v = s_a[x];
-                    for (int y = x + 1; y < P && s_col[y] == c; ++y) v = v + s_a[y];
-                    nz = v != 0.0;
-                }
-            }
+        for (int q = 0; q < nb; q += TPB) {
+            const int b = q + tid;
+            const int c = s_c[b];
             int tot;
-            const int ex = block_excl_scan<TPB>(nz, s_wsi, tot);
-            if (nz) {
-                const int64_t o = (int64_t)run + ex;
-                if (o < lim) {
-                    __builtin_nontemporal_store(c, &ccol[cbase + o]);
-                    __builtin_nontemporal_store(v, &cval[cbase + o]);
-                }
-            }
+            const int ex = block_excl_scan<TPB>(c, s_wsi, tot);
+            s_c[b] = run + ex;
             run += tot;
         }
-        if (tid == 0) {
-            ws.annz[row] = run;
-            stored += (unsigned long long)run;
-            if (run > lim) flags |= kErrInternal;
+    }
+    if (tid == 0) s_c[nb] = P;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        if (isx[j] >= 0) {
+            const int d = s_c[bk[j]] + pos[j];
+            s_col[d] = icol[j];
+            s_sx[d] = (uint16_t)isx[j];
+            s_bk[d] = (uint16_t)bk[j];
+            s_a[d] = ival[j];
         }
-        __syncthreads();  // LDS is reused by the next row
+    }
+    __syncthreads();
+
+    // --- rank of each item inside its bucket by (column, s): its sorted position
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const int x = tid + j * TPB;
+        isx[j] = -1;
+        if (x < P) {
+            const int32_t c = s_col[x];
+            const int sx = s_sx[x];
+            const int b = s_bk[x];
+            const int st = s_c[b], en = s_c[b + 1];
+            int rank = 0;
+            for (int y = st; y < en; ++y) {
+                const int32_t cy = s_col[y];
+                rank += (cy < c) | ((cy == c) & (s_sx[y] < sx));
+            }
+            icol[j] = c;
+            ival[j] = s_a[x];
+            isx[j] = st + rank;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        if (isx[j] >= 0) {
+            s_col[isx[j]] = icol[j];
+            s_a[isx[j]] = ival[j];
+        }
+    }
+    __syncthreads();
+
+    // --- ordered fold at each column's first position, scan, coalesced write
+    const int64_t cbase = crowptr[cur.row];
+    const int64_t lim = crowptr[cur.row + 1] - cbase;
+    int run = 0;
+    for (int x0 = 0; x0 < P; x0 += TPB) {
+        const int x = x0 + tid;
+        int nz = 0;
+        int32_t c = 0;
+        double v = 0.0;
+        if (x < P) {
+            c = s_col[x];
+            if (x == 0 || s_col[x - 1] != c) {
+                v = s_a[x];
+                for (int y = x + 1; y < P && s_col[y] == c; ++y) v = v + s_a[y];
+                nz = v != 0.0;
+            }
+        }
+        int tot;
+        const int ex = block_excl_scan<TPB>(nz, s_wsi, tot);
+        if (nz) {
+            const int64_t o = (int64_t)run + ex;
+            if (o < lim) {
+                __builtin_nontemporal_store(c, &ccol[cbase + o]);
+                __builtin_nontemporal_store(v, &cval[cbase + o]);
+            }
+        }
+        run += tot;
+    }
+    if (tid == 0) {
+        ws.annz[cur.row] = run;
+        stored += (unsigned long long)run;
+        if (run > lim) flags |= kErrInternal;
+    }
+}
+
+template <int TPB, int CAP>
+struct EscLds {
+    double a[CAP];     // meta: A value of entry e | items: product value
+    int64_t b64[CAP];  // meta: B start of entry e | items: column, serial index, bucket
+    int c[CAP + 1];    // meta: product offset of entry e | bucket counts, then bucket starts
+    int wsi[2 * (TPB / 64) + 2];
+};
+
+// One row of the software pipeline (see hash_step in symbolic.hip).
+template <int TPB, int CAP>
+__device__ __forceinline__ void esc_step(const Operands &op, const Workspace &ws, const RowRec *list, int count,
+                                         int G, int &i, RowRec &cur, const MetaPre &pcur, MetaPre &pnext,
+                                         int &wnext, const int64_t *crowptr, int32_t *ccol, double *cval,
+                                         EscLds<TPB, CAP> &sh, unsigned long long &stored, unsigned &flags) {
+    const int tid = threadIdx.x;
+    int P, L;
+    meta_to_lds<TPB, true>(op, ws, cur, pcur, cur.na <= TPB, sh.b64, sh.a, sh.c, sh.wsi, P, L);
+    const RowRec nxt = i + G < count ? rec_decode(wnext) : cur;
+    wnext = rec_word(list, min(i + 2 * G, count - 1));
+    pnext = meta_prefetch<true>(op, ws, nxt);
+    if (P > CAP) flags |= kErrInternal;  // cannot happen: binning used the same lengths
+    if (P <= CAP) {
+        if (tid == 0) sh.c[cur.na] = P;
+        __syncthreads();
+        esc_row<TPB, CAP>(op, ws, cur, P, L, crowptr, ccol, cval, sh.a, sh.b64, sh.c, sh.wsi, stored, flags);
+    }
+    __syncthreads();  // LDS is reused by the next row
+    cur = nxt;
+    i += G;
+}
+
+template <int TPB, int CAP>
+__global__ __launch_bounds__(TPB) void k_numeric_esc(Operands op, Workspace ws, int binid, const int64_t *crowptr,
+                                                     int32_t *ccol, double *cval) {
+    static_assert(CAP % TPB == 0 && CAP <= 65536, "CAP must be a multiple of TPB and fit 16-bit indices");
+    __shared__ EscLds<TPB, CAP> sh;
+    const int tid = threadIdx.x;
+    const RowRec *list = bin_list(ws, binid);
+    const int count = ws.ctrl->bin_count[binid];
+    const int G = (int)gridDim.x;
+    unsigned long long stored = 0;
+    unsigned flags = 0;
+    int i = (int)blockIdx.x;
+    if (i < count) {
+        // Two-step unrolled loop with ping-pong prefetch registers (pa, pb).
+        RowRec cur = rec_decode(rec_word(list, i));
+        MetaPre pa = meta_prefetch<true>(op, ws, cur), pb;
+        int wnext = rec_word(list, min(i + G, count - 1));
+        for (;;) {
+            if (i >= count) break;
+            esc_step<TPB, CAP>(op, ws, list, count, G, i, cur, pa, pb, wnext, crowptr, ccol, cval, sh, stored, flags);
+            if (i >= count) break;
+            esc_step<TPB, CAP>(op, ws, list, count, G, i, cur, pb, pa, wnext, crowptr, ccol, cval, sh, stored, flags);
+        }
     }
     if (tid == 0 && stored) atomicAdd(&ws.ctrl->actual, stored);
     if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
@@ -275,7 +289,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
     __shared__ int s_wsi[2 * (TPB / 64) + 2];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
     const int tid = threadIdx.x;
-    const int32_t *list = bin_list(ws, kBinLarge);
+    const RowRec *list = bin_list(ws, kBinLarge);
     const int count = ws.ctrl->bin_count[kBinLarge];
     const int64_t W = ws.lwidth;
     uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
@@ -283,7 +297,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
     unsigned flags = 0;
     unsigned long long stored = 0;
     for (int i = (int)blockIdx.x; i < count; i += (int)gridDim.x) {
-        const int row = list[i];
+        const int row = list[i].row;
         const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
         const int64_t cbase = crowptr[row];
         const int64_t lim = crowptr[row + 1] - cbase;

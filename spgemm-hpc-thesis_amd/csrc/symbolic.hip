@@ -1,11 +1,13 @@
 // symbolic.hip -- exact structural count (distinct output columns) per row.
 //
 // The count does not depend on summation order, so small and medium rows use
-// an open-addressing hash set of column indices in LDS (one compare-and-swap
-// per product, table at least twice the row's product bound, so probing always
-// ends).  A entries are expanded "entry-major": with L the power of two >= the
-// longest referenced B row, each group of L lanes streams one B row, so there
-// is no per-product search and reads of a B row are contiguous.
+// an open-addressing hash set of column indices in LDS (table two or four times
+// the row's product bound, so probing always ends).  A entries are expanded
+// "entry-major": with L the power of two >= the longest referenced B row, each
+// group of L lanes streams one B row, so there is no per-product search, reads
+// of a B row are contiguous, and all of a row's column loads are in flight
+// before the first insertion.  Insertions then run in lockstep probe rounds
+// with every register slot's compare-and-swap issued back to back.
 // Large rows set bits in a per-workgroup column bitmap in HBM (window by window)
 // and count the bits they newly set.
 #include "device_common.hpp"
@@ -20,80 +22,107 @@ constexpr int32_t kEmptyKey = INT_MAX;  // never a valid column (col < cols <= I
 __device__ __forceinline__ uint32_t hash_col(int32_t c) { return (uint32_t)c * 2654435761u; }
 
 template <int TPB, int CAP>
-__global__ __launch_bounds__(TPB) void k_symbolic_hash(Operands op, Workspace ws, int binid) {
-    constexpr int TMAX = 2 * CAP;
-    __shared__ int32_t s_tab[TMAX];
-    __shared__ int64_t s_bst[CAP];
-    __shared__ int32_t s_len[CAP];
-    __shared__ int s_wsi[TPB / 64 + 1];
+struct HashLds {
+    static constexpr int FILL = CAP <= 256 ? 4 : 2;  // table size / product bound
+    static constexpr int TMAX = FILL * CAP;
+    int32_t tab[TMAX + TPB];  // table, then one dummy word per thread
+    int64_t bst[CAP];
+    int off[CAP + 1];
+    int wsi[2 * (TPB / 64) + 2];
+};
+
+// One row of the software pipeline: the row's metadata comes from registers
+// loaded one step earlier (pcur), and the next row's record and metadata are
+// issued into the other register set (pnext) before this row's gathers.
+template <int TPB, int CAP>
+__device__ __forceinline__ void hash_step(const Operands &op, const Workspace &ws, const RowRec *list, int count,
+                                          int G, int &i, RowRec &cur, const MetaPre &pcur, MetaPre &pnext,
+                                          int &wnext, HashLds<TPB, CAP> &sh, unsigned &flags) {
+    constexpr int ITEMS = CAP / TPB;
+    constexpr int FILL = HashLds<TPB, CAP>::FILL;
+    constexpr int TMAX = HashLds<TPB, CAP>::TMAX;
     const int tid = threadIdx.x;
-    for (int i = tid; i < TMAX; i += TPB) s_tab[i] = kEmptyKey;
-    __syncthreads();
-    const int32_t *list = bin_list(ws, binid);
-    const int count = ws.ctrl->bin_count[binid];
-    unsigned flags = 0;
-    for (int i = (int)blockIdx.x; i < count; i += (int)gridDim.x) {
-        const int row = list[i];
-        const int64_t a0 = op.aptr[row];
-        const int na = (int)(op.aptr[row + 1] - a0);  // <= CAP by binning
-        int mylen = 0, mymax = 0;
-        for (int e = tid; e < na; e += TPB) {
-            const int64_t s = __builtin_nontemporal_load(&ws.ebst[a0 + e]);
-            const int len = __builtin_nontemporal_load(&ws.elen[a0 + e]);
-            s_bst[e] = s;
-            s_len[e] = len;
-            mylen += len;
-            mymax = max(mymax, len);
-        }
-        const int P = block_sum<TPB>(mylen, s_wsi);
-        const int L = block_max<TPB>(mymax, s_wsi);
-        if (P > CAP) {  // cannot happen: binning used the same lengths (uniform across the block)
-            flags |= kErrInternal;
-            __syncthreads();
-            continue;
-        }
+    int P, L;
+    meta_to_lds<TPB, false>(op, ws, cur, pcur, cur.na <= TPB, sh.bst, nullptr, sh.off, sh.wsi, P, L);
+    const RowRec nxt = i + G < count ? rec_decode(wnext) : cur;
+    wnext = rec_word(list, min(i + 2 * G, count - 1));
+    pnext = meta_prefetch<false>(op, ws, nxt);
+    if (P > CAP) flags |= kErrInternal;  // cannot happen: binning used the same lengths
+    if (P <= CAP) {
+        if (tid == 0) sh.off[cur.na] = P;
+        __syncthreads();
+        // All of the row's column loads are issued before any insertion.
+        int e[ITEMS], t[ITEMS], icol[ITEMS];
+        bool pend[ITEMS];
+        slot_products<TPB, ITEMS>(cur.na, P, L, sh.off, e, t, pend);
+        int64_t idx[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) idx[j] = pend[j] ? sh.bst[e[j]] + t[j] : 0;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) icol[j] = op.bcol[idx[j]];
         int T = 64;
-        while (T < 2 * P && T < TMAX) T <<= 1;
+        while (T < FILL * P && T < TMAX) T <<= 1;
+        T = uni(T);
         const int logT = 31 - __clz(T);
-        __syncthreads();
+        uint32_t h[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) h[j] = hash_col(icol[j]) >> (32 - logT);
+        // Linear probing in lockstep rounds, branch-free: a settled slot aims
+        // its compare-and-swap at this thread's private dummy word with a
+        // compare value that never matches.  T >= 2P > distinct columns, so
+        // every item settles within T rounds.
         int fresh = 0;
-        // T >= 2P > distinct columns, so an empty slot is always found; the
-        // probe bound only makes termination unconditional.
-        auto insert = [&](int32_t c) {
-            uint32_t h = hash_col(c) >> (32 - logT);
-            for (int probe = 0; probe < T; ++probe) {
-                const int32_t old = atomicCAS(&s_tab[h], kEmptyKey, c);
-                if (old == kEmptyKey) {
-                    ++fresh;
-                    return;
-                }
-                if (old == c) return;
-                h = (h + 1u) & (uint32_t)(T - 1);
+        for (int round = 0; round < T; ++round) {
+            int32_t old[ITEMS];
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j)
+                old[j] = atomicCAS(pend[j] ? &sh.tab[h[j]] : &sh.tab[TMAX + tid], pend[j] ? kEmptyKey : INT_MIN,
+                                   icol[j]);
+            int left = 0;
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                const bool ins = pend[j] & (old[j] == kEmptyKey);
+                const bool dup = pend[j] & (old[j] == icol[j]);
+                fresh += ins;
+                pend[j] = pend[j] & !ins & !dup;
+                h[j] = pend[j] ? ((h[j] + 1u) & (uint32_t)(T - 1)) : h[j];
+                left += pend[j];
             }
-            flags |= kErrInternal;
-        };
-        if (L <= TPB) {
-            int Lp = 1;
-            while (Lp < L) Lp <<= 1;
-            const int logL = 31 - __clz(Lp);
-            const int G = TPB >> logL;
-            const int sub = tid & (Lp - 1);
-            for (int e0 = 0; e0 < na; e0 += G) {
-                const int e = e0 + (tid >> logL);
-                if (e < na && sub < s_len[e]) insert(op.bcol[s_bst[e] + sub]);
-            }
-        } else {
-            for (int e = 0; e < na; ++e) {
-                const int len = s_len[e];
-                const int64_t s = s_bst[e];
-                for (int t = tid; t < len; t += TPB) insert(op.bcol[s + t]);
-            }
+            if (!__syncthreads_or(left)) break;
         }
-        const int total = block_sum<TPB>(fresh, s_wsi);
-        if (tid == 0) ws.snnz[row] = total;
+        const int total = block_sum<TPB>(fresh, sh.wsi);
+        if (tid == 0) ws.snnz[cur.row] = total;
         __syncthreads();
-        for (int q = tid; q < T; q += TPB) s_tab[q] = kEmptyKey;
-        __syncthreads();
+        for (int q = tid; q < T; q += TPB) sh.tab[q] = kEmptyKey;
+    }
+    __syncthreads();
+    cur = nxt;
+    i += G;
+}
+
+template <int TPB, int CAP>
+__global__ __launch_bounds__(TPB) void k_symbolic_hash(Operands op, Workspace ws, int binid) {
+    __shared__ HashLds<TPB, CAP> sh;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < HashLds<TPB, CAP>::TMAX + TPB; q += TPB) sh.tab[q] = kEmptyKey;
+    __syncthreads();
+    const RowRec *list = bin_list(ws, binid);
+    const int count = ws.ctrl->bin_count[binid];
+    const int G = (int)gridDim.x;
+    unsigned flags = 0;
+    int i = (int)blockIdx.x;
+    if (i < count) {
+        // Two-step unrolled loop with ping-pong prefetch registers (pa, pb), so
+        // no register that an outstanding load writes is ever copied.
+        RowRec cur = rec_decode(rec_word(list, i));
+        MetaPre pa = meta_prefetch<false>(op, ws, cur), pb;
+        int wnext = rec_word(list, min(i + G, count - 1));
+        for (;;) {
+            if (i >= count) break;
+            hash_step<TPB, CAP>(op, ws, list, count, G, i, cur, pa, pb, wnext, sh, flags);
+            if (i >= count) break;
+            hash_step<TPB, CAP>(op, ws, list, count, G, i, cur, pb, pa, wnext, sh, flags);
+        }
     }
     if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
 }
@@ -136,13 +165,13 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
     __shared__ int64_t s_off[CAP + 1];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
     const int tid = threadIdx.x;
-    const int32_t *list = bin_list(ws, kBinLarge);
+    const RowRec *list = bin_list(ws, kBinLarge);
     const int count = ws.ctrl->bin_count[kBinLarge];
     const int64_t W = ws.lwidth;
     uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
     unsigned flags = 0;
     for (int i = (int)blockIdx.x; i < count; i += (int)gridDim.x) {
-        const int row = list[i];
+        const int row = list[i].row;
         const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
         int64_t cnt = 0;
         for (int64_t w0 = 0; w0 < op.bcols; w0 += W) {
