@@ -87,6 +87,8 @@ typedef struct spg_stats {
                                   block<=2048 products, large (chunked) */
     int32_t compacted;         /* 1 if exact zeros forced a compaction pass */
     int32_t bins;              /* number of bins used */
+    int32_t single_pass;       /* 1 if the small rows took the single-pass kernel */
+    int32_t reserved_;
     double ms_total;           /* device time of the whole pipeline (events), if timing on */
     double ms_symbolic;        /* device time of the symbolic phase, if timing on */
     double ms_numeric;         /* device time of the numeric phase, if timing on */
@@ -100,6 +102,16 @@ const char *spg_last_error(const spg_context *ctx);
 /* Kernels of this context run on `stream` (a hipStream_t; NULL = the context's
  * own stream). */
 spg_status spg_set_stream(spg_context *ctx, void *stream);
+/* Which pipeline computes the small rows (<= 256 products and A entries):
+ *   SPG_PATH_AUTO         single pass when its C capacity (small rows' product
+ *                         bounds + big rows' exact counts) fits half of the
+ *                         device memory, else two-phase
+ *   SPG_PATH_SINGLE_PASS  always single pass
+ *   SPG_PATH_TWO_PHASE    count every row first, then compute
+ * Results are identical on every path.  No counterpart in the reference (its
+ * spgemm() has one CPU pipeline); exposed so callers and tests can pin a path. */
+enum { SPG_PATH_AUTO = 0, SPG_PATH_SINGLE_PASS = 1, SPG_PATH_TWO_PHASE = 2 };
+spg_status spg_set_path(spg_context *ctx, int path);
 /* Record HIP events around the phases of each multiply (spg_stats.ms_*). */
 spg_status spg_set_timing(spg_context *ctx, int enabled);
 spg_status spg_last_stats(const spg_context *ctx, spg_stats *out);

@@ -1,7 +1,6 @@
 """Build recipe for libspgemm_amd.so (gfx950) and its C++ test programs.
 
-    python spgemm-hpc-thesis_amd/build.py          # build the library
-    python spgemm-hpc-thesis_amd/build.py --check  # also compile the C++ shim test
+    python spgemm-hpc-thesis_amd/build.py   # library + the C++ shim test program
 
 Device code is compiled with -ffp-contract=off: the engine's bit-exactness
 rests on every product and every sum being rounded separately.  Packed FP32
@@ -35,7 +34,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}"]
 
-HIP_SOURCES = ["analyze.hip", "symbolic.hip", "numeric.hip", "scan.hip", "engine.hip"]
+HIP_SOURCES = ["analyze.hip", "symbolic.hip", "numeric.hip", "fused.hip", "scan.hip", "engine.hip"]
 CXX_SOURCES = ["host_matrix.cpp"]
 
 
@@ -71,22 +70,20 @@ def build_library(verbose: bool = False) -> str:
 def build_shim_test(verbose: bool = False) -> str:
     """Compile the C++17 shim test program against the library."""
     src = os.path.join(PKG, "host", "shim_test.cpp")
-    exe = os.path.join(BUILD, "shim_test")
+    exe = os.path.join(LIBDIR, "shim_test")
     _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{os.path.join(PKG, 'host')}", src, "-o", exe,
-          f"-L{LIBDIR}", "-lspgemm_amd", f"-Wl,-rpath,{LIBDIR}"], verbose)
+          f"-L{LIBDIR}", "-lspgemm_amd", "-Wl,-rpath,$ORIGIN"], verbose)
     return exe
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--check", action="store_true")
     ap.add_argument("-v", "--verbose", action="store_true")
     a = ap.parse_args()
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         sys.exit("hipcc not found")
     print(build_library(a.verbose))
-    if a.check:
-        print(build_shim_test(a.verbose))
+    print(build_shim_test(a.verbose))
 
 
 if __name__ == "__main__":

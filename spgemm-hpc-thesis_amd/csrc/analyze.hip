@@ -29,20 +29,18 @@ __global__ void k_zero_words(uint32_t *p, size_t n) {
 
 constexpr int kAnalyzeTpb = 256;
 
-__global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace ws) {
-    __shared__ unsigned long long s_ub[kAnalyzeTpb / 64][64];
-    const int w = threadIdx.x >> 6;
+// One wave: rows tile*64 .. tile*64+63.
+__device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace &ws, int64_t tile,
+                                             unsigned long long *s_ub, int64_t *s_part) {
     const int lane = lane_id();
-    const int64_t tile = (int64_t)blockIdx.x * (kAnalyzeTpb / 64) + w;
     const int64_t r = tile * 64 + lane;
-    if (tile * 64 >= op.arows) return;  // whole wave past the end (uniform per wave)
     const bool valid = r < op.arows;
     const int last = (int)min((int64_t)63, (int64_t)op.arows - 1 - tile * 64);
     const int64_t a0 = valid ? op.aptr[r] : 0;
     const int64_t a1 = valid ? op.aptr[r + 1] : 0;
     const int64_t E0 = __shfl(a0, 0, 64);
     const int64_t E1 = __shfl(a1, last, 64);
-    s_ub[w][lane] = 0ull;
+    s_ub[lane] = 0ull;
     wave_sync();
     unsigned flags = 0;
     for (int64_t base = E0; base < E1; base += 64) {
@@ -81,10 +79,10 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
             if (lane >= d && kk == key) v += vv;
         }
         const int next = __shfl_down(key, 1, 64);
-        if (in && (lane == 63 || next != key)) atomicAdd(&s_ub[w][key], (unsigned long long)v);
+        if (in && (lane == 63 || next != key)) atomicAdd(&s_ub[key], (unsigned long long)v);
     }
     wave_sync();
-    const int64_t ub = (int64_t)s_ub[w][lane];
+    const int64_t ub = (int64_t)s_ub[lane];
     int b = -1;
     if (valid) {
         b = classify(ub, a1 - a0);
@@ -94,27 +92,64 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
             ws.annz[r] = 0;
         }
     }
-    const unsigned long long tot = (unsigned long long)wave_sum(valid ? ub : (int64_t)0);
+    const int64_t tot = wave_sum(valid ? ub : (int64_t)0);
+    const int64_t small = wave_sum((b == kBinW1 || b == kBinW2) ? ub : (int64_t)0);
     unsigned long long fl = flags;
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) fl |= __shfl_xor(fl, d, 64);
     if (lane == 0) {
-        if (tot) atomicAdd(&ws.ctrl->products, tot);
+        s_part[0] = tot;
+        s_part[1] = small;
         if (fl) atomicOr(&ws.ctrl->flags, fl);
     }
 #pragma unroll
     for (int bb = 0; bb < kNumBins; ++bb) {
         const unsigned long long m = __ballot(b == bb);
         if (!m) continue;
+        if (lane == 0) s_part[2 + bb] = (int64_t)__popcll(m);
+        const bool listed = bb != kBinEmpty && (ws.list_small || (bb != kBinW1 && bb != kBinW2));
+        if (!listed) continue;
         int basei = 0;
         if (lane == 0) basei = atomicAdd(&ws.ctrl->bin_count[bb], (int)__popcll(m));
         basei = __shfl(basei, 0, 64);
-        if (bb != kBinEmpty && b == bb) {
+        if (b == bb) {
             RowRec rec;
             rec.row = (int32_t)r;
             rec.na = (int32_t)min(a1 - a0, (int64_t)INT_MAX);
             rec.a0 = a0;
             ws.bin_recs[(size_t)bb * (size_t)ws.list_stride + (size_t)(basei + (int)__popcll(m & lanemask_lt()))] = rec;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace ws) {
+    constexpr int NW = kAnalyzeTpb / 64;
+    __shared__ unsigned long long s_ub[NW][64];
+    __shared__ int64_t s_part[NW][kAnalyzeFields];
+    const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int64_t tile = (int64_t)blockIdx.x * NW + w;
+    if (lane < kAnalyzeFields) s_part[w][lane] = 0;
+    if (tile * 64 < op.arows) analyze_tile(op, ws, tile, s_ub[w], s_part[w]);  // uniform per wave
+    __syncthreads();
+    if (threadIdx.x < kAnalyzeFields) {
+        int64_t v = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) v += s_part[i][threadIdx.x];
+        ws.part[(size_t)blockIdx.x * kAnalyzeFields + threadIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_analyze_reduce(const int64_t *part, int64_t nblocks, Control *c) {
+    __shared__ int64_t s_ws[4];
+    for (int f = 0; f < kAnalyzeFields; ++f) {
+        int64_t v = 0;
+        for (int64_t i = threadIdx.x; i < nblocks; i += 256) v += part[(size_t)i * kAnalyzeFields + f];
+        v = block_sum<256>(v, s_ws);
+        if (threadIdx.x == 0) {
+            if (f == 0) c->products = (unsigned long long)v;
+            else if (f == 1) c->ub_small = (unsigned long long)v;
+            else c->bin_rows[f - 2] = v;
         }
     }
 }
@@ -133,11 +168,16 @@ void launch_zero_control(Control *ctrl, hipStream_t stream) {
     hipLaunchKernelGGL(k_zero_control, dim3(1), dim3(64), 0, stream, ctrl);
 }
 
+int64_t analyze_blocks(int32_t rows) {
+    const int64_t tiles = ((int64_t)rows + 63) / 64;
+    return (tiles + kAnalyzeTpb / 64 - 1) / (kAnalyzeTpb / 64);
+}
+
 void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream) {
     if (op.arows <= 0) return;
-    const int64_t tiles = ((int64_t)op.arows + 63) / 64;
-    const int64_t blocks = (tiles + kAnalyzeTpb / 64 - 1) / (kAnalyzeTpb / 64);
+    const int64_t blocks = analyze_blocks(op.arows);
     hipLaunchKernelGGL(k_analyze, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
+    hipLaunchKernelGGL(k_analyze_reduce, dim3(1), dim3(256), 0, stream, ws.part, blocks, ws.ctrl);
 }
 
 }  // namespace spg

@@ -123,9 +123,56 @@ __device__ __forceinline__ void block_minmax(int &mn, int &mx, int *ws) {
     }
 }
 
+// DPP forms of the wave scans and reductions (no LDS traffic): row_shr 1, 2,
+// 4, 8 inside each 16-lane row, then row_bcast 15 and 31 carry the row totals
+// upward (row masks 0xa and 0xc).  Lanes whose source is outside the row read
+// the identity.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i(int identity, int v) {
+    return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWMASK, 0xf, false);
+}
+
+__device__ __forceinline__ int wave_incl_sum_dpp(int v) {
+    v += dpp_i<0x111, 0xf>(0, v);
+    v += dpp_i<0x112, 0xf>(0, v);
+    v += dpp_i<0x114, 0xf>(0, v);
+    v += dpp_i<0x118, 0xf>(0, v);
+    v += dpp_i<0x142, 0xa>(0, v);
+    v += dpp_i<0x143, 0xc>(0, v);
+    return v;
+}
+
+// Lane 63 of an inclusive scan holds the wave's result (a scalar register).
+__device__ __forceinline__ int wave_last(int v) { return __builtin_amdgcn_readlane(v, 63); }
+
+__device__ __forceinline__ int wave_max_dpp(int v) {
+    v = max(v, dpp_i<0x111, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x112, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x114, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x118, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x142, 0xa>(INT_MIN, v));
+    v = max(v, dpp_i<0x143, 0xc>(INT_MIN, v));
+    return wave_last(v);
+}
+
+__device__ __forceinline__ int wave_min_dpp(int v) {
+    v = min(v, dpp_i<0x111, 0xf>(INT_MAX, v));
+    v = min(v, dpp_i<0x112, 0xf>(INT_MAX, v));
+    v = min(v, dpp_i<0x114, 0xf>(INT_MAX, v));
+    v = min(v, dpp_i<0x118, 0xf>(INT_MAX, v));
+    v = min(v, dpp_i<0x142, 0xa>(INT_MAX, v));
+    v = min(v, dpp_i<0x143, 0xc>(INT_MAX, v));
+    return wave_last(v);
+}
+
 // Block-uniform value held in a scalar register (the value must already be
 // equal in every lane; this lets the compiler branch on it with scalar code).
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 
 // B row k through the directory; flags record a bad column or a missing row.
 __device__ __forceinline__ void brow(const Operands &op, int32_t k, int64_t &s, int64_t &e, unsigned &flags) {

@@ -71,7 +71,8 @@ struct spg_context {
     // uploaded operands (host inputs) and the halo directory
     DevBuf a_ptr, a_col, a_val, b_ptr, b_col, b_val, dir_start, dir_end;
     // workspace
-    DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc;
+    DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc, tile_status, tile_distinct, part;
+    int path = SPG_PATH_AUTO;
     int64_t lbits_width = 0;
     // results (ping-pong for compaction)
     DevBuf rowptr[2], ccol[2], cval[2];
@@ -164,7 +165,7 @@ spg_status flag_error(spg_context *ctx, unsigned long long flags) {
 
 // The whole product on the device.  On success the result is
 // rowptr[cur], ccol[cur], cval[cur] with res_nnz entries.
-spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols) {
+spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols, bool small_b) {
     hipStream_t st = ctx->stream;
     ctx->have_result = false;
     spg_stats stats{};
@@ -178,6 +179,7 @@ spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols
     SPG_HIP(ctx->annz.ensure(R * 8, st), "hipMalloc(annz)");
     SPG_HIP(ctx->scan_tmp.ensure(scan_tmp_elems(rows) * 8, st), "hipMalloc(scan)");
     SPG_HIP(ctx->ctrl.ensure(sizeof(Control), st), "hipMalloc(ctrl)");
+    SPG_HIP(ctx->part.ensure((size_t)(analyze_blocks(rows) + 1) * kAnalyzeFields * 8, st), "hipMalloc(partials)");
     const int out = 0;
     SPG_HIP(ctx->rowptr[out].ensure(((size_t)rows + 1) * 8, st), "hipMalloc(row_ptr C)");
 
@@ -191,6 +193,7 @@ spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols
     ws.annz = ctx->annz.as<int64_t>();
     ws.scan_tmp = ctx->scan_tmp.as<int64_t>();
     ws.ctrl = ctx->ctrl.as<Control>();
+    ws.part = ctx->part.as<int64_t>();
     // Large-row path: one column window and one presence bitmap per resident
     // workgroup (kept all-zero between rows); the accumulators are allocated
     // only when large rows exist.
@@ -218,37 +221,82 @@ spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols
     }
 
     if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+    // Single-pass path: only the rows of the big bins are listed and counted
+    // ahead; the rows of bins W1, W2 are computed in one pass by the fused kernel.
+    const bool try_fused = ctx->path != SPG_PATH_TWO_PHASE;
+    ws.list_small = try_fused ? 0 : 1;
     launch_zero_control(ws.ctrl, st);
     launch_analyze(op, ws, st);
-    launch_symbolic(op, ws, ctx->cus, st);
-    launch_scan(ws.snnz, ctx->rowptr[out].as<int64_t>(), rows, ws.scan_tmp, &ws.ctrl->structural, st);
+    launch_symbolic(op, ws, try_fused ? kBigBins : kAllBins, ctx->cus, st);
+    if (!try_fused) launch_scan(ws.snnz, ctx->rowptr[out].as<int64_t>(), rows, ws.scan_tmp, &ws.ctrl->structural, st);
     if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
     spg_status s = read_control(ctx);
     if (s != SPG_OK) return s;
     Control h = *ctx->h_ctrl;
     if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
     stats.products = (int64_t)h.products;
-    for (int b = 0; b < kNumBins; ++b) stats.rows_per_bin[b] = h.bin_count[b];
+    for (int b = 0; b < kNumBins; ++b) stats.rows_per_bin[b] = h.bin_rows[b];
     stats.bins = kNumBins;
-    const int64_t nstruct = (int64_t)h.structural;
-    stats.structural_nnz = nstruct;
 
-    SPG_HIP(ctx->ccol[out].ensure((size_t)nstruct * 4, st), "hipMalloc(C col)");
-    SPG_HIP(ctx->cval[out].ensure((size_t)nstruct * 8, st), "hipMalloc(C val)");
+    // C capacity of the single-pass path: the small rows' work bounds plus the
+    // big rows' exact counts.  Taken when it fits half of the device memory
+    // (always when the caller asked for it and it can be allocated).
+    const uint64_t fused_cap = h.ub_small + h.sym_total;
+    bool fused = false;
+    if (try_fused) {
+        fused = ctx->path == SPG_PATH_SINGLE_PASS || fused_cap * 12ull <= (uint64_t)(ctx->total_mem / 2);
+        if (!fused) {
+            // Two-phase after all: list the small rows too (analyze again) and
+            // count them.
+            ws.list_small = 1;
+            launch_zero_control(ws.ctrl, st);
+            launch_analyze(op, ws, st);
+            launch_symbolic(op, ws, kAllBins, ctx->cus, st);
+            launch_scan(ws.snnz, ctx->rowptr[out].as<int64_t>(), rows, ws.scan_tmp, &ws.ctrl->structural, st);
+            if ((s = read_control(ctx)) != SPG_OK) return s;
+            h = *ctx->h_ctrl;
+            if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
+        }
+    }
+    const int64_t nrows_cap = fused ? (int64_t)fused_cap : (int64_t)h.structural;
+    SPG_HIP(ctx->ccol[out].ensure((size_t)nrows_cap * 4, st), "hipMalloc(C col)");
+    SPG_HIP(ctx->cval[out].ensure((size_t)nrows_cap * 8, st), "hipMalloc(C val)");
     if (h.bin_count[kBinLarge] > 0) {
         SPG_HIP(ctx->lacc.ensure((size_t)ws.lblocks * (size_t)width * 8, st), "hipMalloc(large accumulators)");
         ws.lacc = ctx->lacc.as<double>();
     }
+    FusedOut fo{};
+    if (fused) {
+        fo.ntiles = fused_tiles(rows);
+        SPG_HIP(ctx->tile_status.ensure((size_t)(fo.ntiles > 0 ? fo.ntiles : 1) * 8, st), "hipMalloc(tile status)");
+        SPG_HIP(ctx->tile_distinct.ensure((size_t)(fo.ntiles > 0 ? fo.ntiles : 1) * 8, st), "hipMalloc(tile counts)");
+        launch_zero_bytes(ctx->tile_status.p, (size_t)(fo.ntiles > 0 ? fo.ntiles : 1) * 8, st);
+        if (rows == 0) launch_zero_bytes(ctx->rowptr[out].p, 8, st);
+        fo.rowptr = ctx->rowptr[out].as<int64_t>();
+        fo.ccol = ctx->ccol[out].as<int32_t>();
+        fo.cval = ctx->cval[out].as<double>();
+        fo.status = ctx->tile_status.as<uint64_t>();
+        fo.distinct = ctx->tile_distinct.as<int64_t>();
+    }
     if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");
-    launch_numeric(op, ws, h.bin_count, ctx->cus, ctx->rowptr[out].as<int64_t>(), ctx->ccol[out].as<int32_t>(),
-                   ctx->cval[out].as<double>(), st);
+    if (fused) launch_fused(op, ws, fo, small_b, ctx->cus, st);
+    launch_numeric(op, ws, h.bin_count, fused ? kBigBins : kAllBins, ctx->cus, ctx->rowptr[out].as<int64_t>(),
+                   ctx->ccol[out].as<int32_t>(), ctx->cval[out].as<double>(), st);
+    if (fused && fo.ntiles > 0) launch_sum(fo.distinct, fo.ntiles, &ws.ctrl->distinct, st);
     if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[3], st), "hipEventRecord");
     if ((s = read_control(ctx)) != SPG_OK) return s;
     h = *ctx->h_ctrl;
     if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
-    int64_t nnz = (int64_t)h.actual;
+    // Entries of the row layout: structural counts (two-phase path), or the
+    // single-pass rows' stored counts plus the big rows' structural counts.
+    // The numeric kernels report the entries they store (ctrl.actual); the
+    // single-pass rows' stored total is the layout minus the big rows' share.
+    const int64_t layout = fused ? (int64_t)h.reserved : (int64_t)h.structural;
+    stats.structural_nnz = fused ? (int64_t)(h.distinct + h.sym_total) : (int64_t)h.structural;
+    stats.single_pass = fused ? 1 : 0;
+    int64_t nnz = fused ? (int64_t)(h.reserved - h.sym_total + h.actual) : (int64_t)h.actual;
     int result = out;
-    if (nnz != nstruct) {
+    if (nnz != layout) {
         // Exact zeros were dropped: pack rows to their final offsets.
         const int o2 = 1;
         SPG_HIP(ctx->rowptr[o2].ensure(((size_t)rows + 1) * 8, st), "hipMalloc(row_ptr C2)");
@@ -343,7 +391,7 @@ spg_status spg_destroy(spg_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->a_ptr, &ctx->a_col, &ctx->a_val, &ctx->b_ptr, &ctx->b_col, &ctx->b_val,
                       &ctx->dir_start, &ctx->dir_end, &ctx->ebst, &ctx->elen, &ctx->ub, &ctx->bin_rows, &ctx->snnz,
-                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->rowptr[0],
+                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->tile_status, &ctx->tile_distinct, &ctx->part, &ctx->rowptr[0],
                       &ctx->rowptr[1], &ctx->ccol[0], &ctx->ccol[1], &ctx->cval[0], &ctx->cval[1]};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t &e : ctx->ev)
@@ -357,6 +405,13 @@ spg_status spg_destroy(spg_context *ctx) {
 spg_status spg_set_stream(spg_context *ctx, void *stream) {
     if (!ctx) return SPG_EINVAL;
     ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    return SPG_OK;
+}
+
+spg_status spg_set_path(spg_context *ctx, int path) {
+    if (!ctx) return SPG_EINVAL;
+    if (path < SPG_PATH_AUTO || path > SPG_PATH_TWO_PHASE) return set_err(ctx, SPG_EINVAL, "unknown path");
+    ctx->path = path;
     return SPG_OK;
 }
 
@@ -395,6 +450,7 @@ spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg
     const int64_t *bptr = nullptr;
     const int32_t *bcol = nullptr;
     const double *bval = nullptr;
+    int64_t b_span = B->nnz;
     if (local_only) {
         if ((s = stage(ctx, B, ctx->b_ptr, ctx->b_col, ctx->b_val, bptr, bcol, bval)) != SPG_OK) return s;
         op.ndir = B->rows;
@@ -412,6 +468,7 @@ spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg
             return set_err(ctx, SPG_EINVAL, "halo products take host views");
         const int64_t hn = halo ? halo->n_rows : 0;
         const int64_t hnnz = hn ? halo->row_ptr[hn] : 0;
+        b_span = B->nnz + hnnz;
         std::vector<int64_t> ds((size_t)A->cols, -1), de((size_t)A->cols, -1);
         for (int32_t r = 0; r < B->rows; ++r) {
             ds[(size_t)b_first_row + r] = B->row_ptr[r];
@@ -454,7 +511,10 @@ spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg
         op.bcol = ctx->b_col.as<int32_t>();
         op.bval = ctx->b_val.as<double>();
     }
-    if ((s = run_product(ctx, op, A->rows, B->cols)) != SPG_OK) return s;
+    // Every B entry index i reachable through the directory has 8*i < 2^32:
+    // the single-pass kernel then addresses B with 32-bit offsets.
+    const bool small_b = b_span < (int64_t(1) << 29);
+    if ((s = run_product(ctx, op, A->rows, B->cols, small_b)) != SPG_OK) return s;
     *nnz_c = ctx->res_nnz;
     return SPG_OK;
 }

@@ -1,0 +1,520 @@
+// fused.hip -- single-pass product of the rows whose products fit one wave.
+//
+// Semantics are those of the reference's local multiply
+// (src/baselines/spgemm_exchange.cpp:274-309): per output row, products a*b in
+// serial order (A entries in stored order, then B entries in stored order), a
+// column's first product starts its sum and later ones are added left to
+// right, columns ascending, sums equal to 0.0 dropped.
+//
+// A row with at most 256 products is computed entirely by one wave (four
+// products per lane): its products are expanded with their serial index s,
+// counted into 256 buckets over the row's column range with LDS atomics,
+// scattered by bucket, ranked inside their bucket by (column, s) -- the rank is
+// the final sorted position -- and folded left to right at each column's first
+// position.  Sorted position x lives in slot x / 64 of lane x % 64, so the
+// stores of a row are 64 consecutive entries per instruction.  The row's
+// stored entries stay in registers until its output offset is known.
+//
+// Rows are taken in tiles of consecutive rows, one tile per workgroup in
+// workgroup order (a global ticket counter would serialise: device-scope
+// atomics on one address cost tens of nanoseconds each).  A tile publishes
+// its entry count, finds its output offset by looking back over the published
+// counts of earlier tiles (decoupled look-back), and writes its rows at their
+// final place: no separate counting pass and no scan over the rows.  Rows with
+// more products are "reserved": their exact distinct-column count was computed
+// before this kernel by the symbolic kernels, and the numeric kernels fill them
+// in afterwards.
+#include "device_common.hpp"
+
+#include <type_traits>
+
+namespace spg {
+namespace {
+
+using namespace dev;
+
+constexpr int kFItems = kFusedCap / 64;   // products per lane
+constexpr int kFNb = 256;                 // buckets per row
+constexpr int kFKeys = 2 * kFusedCap;     // sorted keys, then padding read by the rank loop
+constexpr uint64_t kStAgg = 1ull << 62;   // tile status: count of the tile only
+constexpr uint64_t kStIncl = 1ull << 63;  // tile status: count of tiles 0..t
+constexpr uint64_t kStVal = kStAgg - 1;
+constexpr int kSpinLimit = 1 << 22;       // never reached unless a tile fails to publish
+
+// A wave's LDS.  The row's entry metadata is dead once its gathers are
+// issued, so it shares storage with the sorted products.
+template <bool SMALLB>
+struct FusedWave {
+    using bst_t = typename std::conditional<SMALLB, uint32_t, int64_t>::type;
+    uint32_t key[kFKeys];  // bucket-ordered keys (+ padding), then sorted columns
+    union {
+        struct {
+            double av[kFusedCap];    // A value of each of the row's entries
+            bst_t bst[kFusedCap];    // first B entry of each entry's B row
+            int off[kFusedCap + 1];  // serial index of each entry's first product
+        };
+        double val[kFusedCap + 64];  // products at their sorted positions (+ one dummy per lane)
+    };
+    alignas(16) int start[kFNb + 4];  // bucket counts, then bucket starts; [kFNb+1] is a dummy counter
+    uint8_t sx[kFKeys];               // serial index by bucket position (wide rows)
+};
+
+template <bool SMALLB>
+struct MetaReg {
+    typename FusedWave<SMALLB>::bst_t bst;
+    int len;
+    double av;
+};
+
+// Row results held in registers until the tile's output offset is known:
+// slot j of a lane is sorted position lane + 64*j of the row.
+struct RowOut {
+    int col[kFItems];
+    double val[kFItems];
+    int pos[kFItems];  // output position inside the row, -1 when nothing is stored
+};
+
+template <typename T>
+__device__ __forceinline__ T ld_byte(const T *base, uint32_t byte_off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+// Entry lane of a row: lanes past the row's last entry re-read it, so the
+// load is unconditional (no register initialisation that would wait on it).
+template <bool SMALLB>
+__device__ __forceinline__ MetaReg<SMALLB> load_meta(const Operands &op, const Workspace &ws, int64_t a0, int na,
+                                                     int c0) {
+    using bst_t = typename FusedWave<SMALLB>::bst_t;
+    const int64_t e = a0 + min(c0 + lane_id(), max(na - 1, 0));
+    MetaReg<SMALLB> m;
+    m.bst = (bst_t)__builtin_nontemporal_load(&ws.ebst[e]);
+    m.len = __builtin_nontemporal_load(&ws.elen[e]);
+    m.av = __builtin_nontemporal_load(&op.aval[e]);
+    return m;
+}
+
+// One row (1 <= P <= kFusedCap products, 1 <= na <= kFusedCap entries) by one wave.
+// Returns the stored entry count; `distinct` receives the distinct columns.
+template <bool SMALLB>
+__device__ __forceinline__ int fused_row(const Operands &op, const Workspace &ws, FusedWave<SMALLB> &w, int64_t a0,
+                                         int na, int P, const MetaReg<SMALLB> &m0, RowOut &out, int &distinct,
+                                         unsigned &flags) {
+    using bst_t = typename FusedWave<SMALLB>::bst_t;
+    const int lane = lane_id();
+
+    // --- the row's entries into LDS: B row starts, A values, product offsets.
+    // Lanes past the row's entries write harmless values below kFusedCap;
+    // off[na] is written last.
+    int L = 0, carry = 0;
+    for (int c0 = 0; c0 < na; c0 += 64) {
+        const MetaReg<SMALLB> m = c0 == 0 ? m0 : load_meta<SMALLB>(op, ws, a0, na, c0);
+        const int e = c0 + lane;
+        const int len = e < na ? m.len : 0;
+        const int incl = wave_incl_sum_dpp(len);
+        w.bst[e] = m.bst;
+        w.av[e] = m.av;
+        w.off[e] = carry + incl - len;
+        carry += wave_last(incl);
+        L = max(L, len);
+    }
+    L = wave_max_dpp(L);
+    wave_sync();
+    if (lane == 0) w.off[na] = carry;
+    if (carry != P) flags |= kErrInternal;  // P comes from the same lengths
+    wave_sync();
+
+    // --- expansion: slot j of each lane gets one product (column, value, s)
+    int col[kFItems], sidx[kFItems];
+    double pv[kFItems];
+    {
+        int Lp = 1;
+        while (Lp < L) Lp <<= 1;
+        Lp = uni(Lp);
+        const int logL = 31 - __clz(Lp);
+        const int G = Lp <= 64 ? (64 >> logL) : 1;
+        const bool entry_major = Lp <= 64 && na <= kFItems * G;
+        bst_t bs[kFItems];
+        int t[kFItems];
+        double av[kFItems];
+        bool ok[kFItems];
+        if (entry_major) {
+            // Lp lanes per A entry: contiguous reads of each B row, no search.
+            const int sub = lane & (Lp - 1);
+            const int grp = lane >> logL;
+#pragma unroll
+            for (int j = 0; j < kFItems; ++j) {
+                const int e = j * G + grp;
+                const bool in = e < na;
+                const int ec = in ? e : 0;
+                const int o = w.off[ec];
+                ok[j] = in && sub < w.off[ec + 1] - o;
+                bs[j] = w.bst[ec];
+                av[j] = w.av[ec];
+                t[j] = sub;
+                sidx[j] = ok[j] ? o + sub : -1;
+            }
+        } else {
+            // Serial order dealt out lane by lane; entry found by binary search.
+#pragma unroll
+            for (int j = 0; j < kFItems; ++j) {
+                const int s = lane + 64 * j;
+                ok[j] = s < P;
+                const int e = ok[j] ? upper_entry(w.off, na, s) : 0;
+                bs[j] = w.bst[e];
+                av[j] = w.av[e];
+                t[j] = ok[j] ? s - w.off[e] : 0;
+                sidx[j] = ok[j] ? s : -1;
+            }
+        }
+        // Every slot's loads are issued before any product is formed; empty
+        // slots read B entry 0 (the row has products, so B is not empty).
+        double bv[kFItems];
+        if constexpr (SMALLB) {
+            // Byte offsets fit 32 bits: scalar base + vector offset addressing.
+#pragma unroll
+            for (int j = 0; j < kFItems; ++j) {
+                const uint32_t ix = ok[j] ? bs[j] + (uint32_t)t[j] : 0u;
+                col[j] = ld_byte(op.bcol, ix * 4u);
+                bv[j] = ld_byte(op.bval, ix * 8u);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kFItems; ++j) {
+                const int64_t ix = ok[j] ? bs[j] + t[j] : 0;
+                col[j] = op.bcol[ix];
+                bv[j] = op.bval[ix];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kFItems; ++j) pv[j] = av[j] * bv[j];
+    }
+
+    // --- buckets over the row's column range.  Keys are offsets from the
+    // row's smallest column (unsigned, order preserving).
+    int mn = INT_MAX, mx = INT_MIN;
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) {
+        mn = sidx[j] >= 0 ? min(mn, col[j]) : mn;
+        mx = sidx[j] >= 0 ? max(mx, col[j]) : mx;
+    }
+    mn = wave_min_dpp(mn);
+    mx = wave_max_dpp(mx);
+    const uint32_t range = (uint32_t)mx - (uint32_t)mn;
+    const int lr = range == 0 ? 0 : 32 - __clz(range);
+    const int sh = uni(lr > 8 ? lr - 8 : 0);
+    const bool packed = lr <= 24;  // offset << 8 | s fits 32 bits
+    reinterpret_cast<int4 *>(w.start)[lane] = make_int4(0, 0, 0, 0);
+    wave_sync();
+    uint32_t rel[kFItems];
+    int bk[kFItems], pos[kFItems];
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) {
+        const bool ok = sidx[j] >= 0;
+        rel[j] = ok ? (uint32_t)col[j] - (uint32_t)mn : 0u;
+        bk[j] = ok ? (int)(rel[j] >> sh) : kFNb + 1;  // empty slots count into a dummy
+        pos[j] = atomicAdd(&w.start[bk[j]], 1);
+    }
+    wave_sync();
+    {
+        const int4 c = reinterpret_cast<const int4 *>(w.start)[lane];
+        const int s4 = c.x + c.y + c.z + c.w;
+        const int incl = wave_incl_sum_dpp(s4);
+        const int ex = incl - s4;
+        reinterpret_cast<int4 *>(w.start)[lane] = make_int4(ex, ex + c.x, ex + c.x + c.y, ex + c.x + c.y + c.z);
+        if (lane == 63) w.start[kFNb] = incl;
+    }
+    wave_sync();
+
+    // --- scatter keys by bucket; empty slots write at P + lane, which the
+    // padding below then overwrites
+    uint32_t key[kFItems];
+    int st[kFItems], len[kFItems];
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) {
+        const bool ok = sidx[j] >= 0;
+        const int b = ok ? bk[j] : 0;
+        st[j] = w.start[b];
+        len[j] = ok ? w.start[b + 1] - st[j] : 0;
+        key[j] = packed ? ((rel[j] << 8) | (uint32_t)sidx[j]) : rel[j];
+        const int d = ok ? st[j] + pos[j] : P + lane;
+        w.key[d] = key[j];
+        if (!packed) w.sx[d] = (uint8_t)sidx[j];
+    }
+    int lmax = 0;
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) lmax = max(lmax, len[j]);
+    lmax = wave_max_dpp(lmax);
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) w.key[P + lane + 64 * j] = 0xffffffffu;  // padding: above every key
+    wave_sync();
+
+    // --- rank inside the bucket by (column, s): the item's sorted position.
+    // Keys after the bucket belong to later buckets or the padding and are all
+    // larger, so the packed loop reads on without bounds.
+    int rank[kFItems];
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) rank[j] = 0;
+    if (packed) {
+        for (int k = 0; k < lmax; ++k) {
+#pragma unroll
+            for (int j = 0; j < kFItems; ++j) rank[j] += w.key[st[j] + k] < key[j] ? 1 : 0;
+        }
+    } else {
+        for (int k = 0; k < lmax; ++k) {
+#pragma unroll
+            for (int j = 0; j < kFItems; ++j) {
+                const int y = st[j] + min(k, max(len[j] - 1, 0));
+                const uint32_t cy = w.key[y];
+                const int sy = w.sx[y];
+                rank[j] += ((k < len[j]) & ((cy < key[j]) | ((cy == key[j]) & (sy < sidx[j])))) ? 1 : 0;
+            }
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) {
+        const int x = sidx[j] >= 0 ? st[j] + rank[j] : P + lane;
+        w.key[x] = (uint32_t)col[j];
+        w.val[x] = pv[j];
+    }
+    wave_sync();
+
+    // --- fold each column's run left to right at its first position; sorted
+    // position x = lane + 64*j
+    int cc[kFItems];
+    double vv[kFItems];
+    bool head[kFItems], cont[kFItems];
+    bool anycont = false;
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) {
+        const int x = lane + 64 * j;
+        cc[j] = (int)w.key[x];
+        vv[j] = w.val[x];
+        const int pc = (int)w.key[x > 0 ? x - 1 : 0];
+        const int nc = (int)w.key[x + 1];
+        const bool valid = x < P;
+        head[j] = valid && (x == 0 || pc != cc[j]);
+        cont[j] = x + 1 < P && nc == cc[j];
+        anycont |= head[j] & cont[j];
+    }
+    if (__ballot(anycont)) {
+#pragma unroll
+        for (int j = 0; j < kFItems; ++j) {
+            if (head[j] && cont[j]) {
+                double v = vv[j];
+                for (int y = lane + 64 * j + 1; y < P && (int)w.key[y] == cc[j]; ++y) v = v + w.val[y];
+                vv[j] = v;
+            }
+        }
+    }
+    // Output positions: one scan of the four slots' flags packed in bytes
+    // (no field exceeds 64, so there is no carry between fields).
+    uint32_t packed_nz = 0, packed_head = 0;
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) {
+        packed_nz |= (head[j] && vv[j] != 0.0 ? 1u : 0u) << (8 * j);
+        packed_head |= (head[j] ? 1u : 0u) << (8 * j);
+    }
+    const uint32_t incl = (uint32_t)wave_incl_sum_dpp((int)packed_nz);
+    const uint32_t tot = (uint32_t)wave_last((int)incl);
+    const uint32_t ex = incl - packed_nz;
+    uint32_t before = 0;
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) {
+        const bool nz = (packed_nz >> (8 * j)) & 1u;
+        out.pos[j] = nz ? (int)(before + ((ex >> (8 * j)) & 0xffu)) : -1;
+        out.col[j] = cc[j];
+        out.val[j] = vv[j];
+        before += (tot >> (8 * j)) & 0xffu;
+    }
+    const uint32_t hs = (uint32_t)wave_last(wave_incl_sum_dpp((int)packed_head));
+    distinct = (int)((hs & 0xffu) + ((hs >> 8) & 0xffu) + ((hs >> 16) & 0xffu) + (hs >> 24));
+    wave_sync();  // LDS of this wave is reused by its next row
+    return (int)before;
+}
+
+__device__ __forceinline__ uint64_t ld_status(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive entry offset of tile t (one wave): publish the tile's count, then
+// sum the counts of earlier tiles back to the nearest published inclusive
+// total, 64 tiles per probe.
+__device__ int64_t tile_lookback(uint64_t *status, int64_t t, int64_t agg, unsigned &flags) {
+    const int lane = lane_id();
+    if (lane == 0) st_status(&status[t], (t == 0 ? kStIncl : kStAgg) | (uint64_t)agg);
+    if (t == 0) return 0;
+    int64_t excl = 0;
+    int64_t p = t - 1;
+    for (int spins = 0;;) {
+        const int64_t q = p - lane;
+        const uint64_t s = q >= 0 ? ld_status(&status[q]) : kStIncl;  // before tile 0: total 0
+        const uint64_t incl_m = __ballot((s & kStIncl) != 0);
+        const uint64_t ready_m = __ballot((s & (kStIncl | kStAgg)) != 0);
+        const int fn = ~ready_m ? __ffsll((long long)~ready_m) - 1 : 64;
+        const int fi = incl_m ? __ffsll((long long)incl_m) - 1 : 64;
+        const int64_t v = (int64_t)(s & kStVal);
+        if (fi < fn) {
+            excl += wave_sum(lane <= fi ? v : (int64_t)0);
+            break;
+        }
+        if (fn > 0) {
+            excl += wave_sum(lane < fn ? v : (int64_t)0);
+            p -= fn;
+        } else {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > kSpinLimit) {
+                flags |= kErrInternal;
+                break;
+            }
+        }
+    }
+    if (lane == 0) st_status(&status[t], kStIncl | (uint64_t)(excl + agg));
+    return excl;
+}
+
+__device__ __forceinline__ int64_t lane64(int64_t v, int k) {
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), k) << 32) |
+                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k));
+}
+
+template <bool SMALLB>
+__global__ __launch_bounds__(kFusedTpb) void k_fused(Operands op, Workspace ws, FusedOut fo) {
+    constexpr int NW = kFusedTpb / 64;
+    constexpr int R = kFusedRows;
+    __shared__ FusedWave<SMALLB> s_w[NW];
+    __shared__ int64_t s_wtot[NW];
+    __shared__ int64_t s_base;
+    const int wv = threadIdx.x >> 6;
+    const int lane = lane_id();
+    FusedWave<SMALLB> &w = s_w[wv];
+    __shared__ int64_t s_dist[NW];
+    unsigned flags = 0;
+    int64_t distinct_sum = 0;
+
+    // One tile per workgroup, in workgroup order: every earlier tile belongs
+    // to a workgroup dispatched before this one, so the look-back below only
+    // waits on workgroups that are running or done.
+    const int64_t t = (int64_t)blockIdx.x;
+    const int64_t r0 = t * (int64_t)(NW * R) + (int64_t)wv * R;
+    // Row offsets and work bounds of this wave's rows (lane k: row r0+k).
+    const int64_t ap = lane <= R ? op.aptr[min(r0 + lane, (int64_t)op.arows)] : 0;
+    const int64_t rq = min(r0 + lane, (int64_t)op.arows - 1);
+    const int64_t ubl = lane < R ? ws.ub[rq] : 0;
+    const int64_t snz = lane < R ? ws.snnz[rq] : 0;
+    RowOut out[R];
+    int cnt[R];
+    bool fused[R];
+    int64_t wave_total = 0;
+    MetaReg<SMALLB> meta[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int64_t ub = lane64(ubl, k);
+        const int64_t na = lane64(ap, k + 1) - lane64(ap, k);
+        fused[k] = r0 + k < op.arows && ub > 0 && ub <= kFusedCap && na <= kFusedCap;
+    }
+    if (fused[0]) meta[0] = load_meta<SMALLB>(op, ws, lane64(ap, 0), (int)(lane64(ap, 1) - lane64(ap, 0)), 0);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        cnt[k] = 0;
+#pragma unroll
+        for (int j = 0; j < kFItems; ++j) out[k].pos[j] = -1;
+        if (k + 1 < R && fused[k + 1])
+            meta[k + 1] =
+                load_meta<SMALLB>(op, ws, lane64(ap, k + 1), (int)(lane64(ap, k + 2) - lane64(ap, k + 1)), 0);
+        if (fused[k]) {
+            int dist = 0;
+            cnt[k] = fused_row<SMALLB>(op, ws, w, lane64(ap, k), (int)(lane64(ap, k + 1) - lane64(ap, k)),
+                                       (int)lane64(ubl, k), meta[k], out[k], dist, flags);
+            distinct_sum += dist;
+            wave_total += cnt[k];
+        } else if (r0 + k < op.arows && lane64(ubl, k) > 0) {
+            wave_total += lane64(snz, k);  // reserved for the numeric kernels
+        }
+    }
+    if (lane == 0) {
+        s_wtot[wv] = wave_total;
+        s_dist[wv] = distinct_sum;
+    }
+    __syncthreads();
+    if (wv == 0) {
+        int64_t agg = 0, dist = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            agg += s_wtot[i];
+            dist += s_dist[i];
+        }
+        if (lane == 0) fo.distinct[t] = dist;
+        const int64_t excl = tile_lookback(fo.status, t, agg, flags);
+        if (lane == 0) {
+            s_base = excl;
+            if (t == fo.ntiles - 1) {
+                fo.rowptr[op.arows] = excl + agg;
+                ws.ctrl->reserved = (unsigned long long)(excl + agg);
+            }
+        }
+    }
+    __syncthreads();
+    int64_t base = s_base;
+    for (int i = 0; i < wv; ++i) base += s_wtot[i];
+    base = uni64(base);
+    // Row offsets and stored counts of the wave's rows (lane k: row r0+k).
+    int64_t rp = base;
+    int64_t my_rp = 0, my_cnt = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (lane == k) {
+            my_rp = rp;
+            my_cnt = cnt[k];
+        }
+        if (fused[k]) {
+            int32_t *cc = fo.ccol + rp;
+            double *cv = fo.cval + rp;
+#pragma unroll
+            for (int j = 0; j < kFItems; ++j) {
+                const int o = out[k].pos[j];
+                if (o >= 0) {
+                    __builtin_nontemporal_store(
+                        out[k].col[j], reinterpret_cast<int32_t *>(reinterpret_cast<char *>(cc) + (uint32_t)o * 4u));
+                    __builtin_nontemporal_store(
+                        out[k].val[j], reinterpret_cast<double *>(reinterpret_cast<char *>(cv) + (uint32_t)o * 8u));
+                }
+            }
+            rp += cnt[k];
+        } else if (r0 + k < op.arows && lane64(ubl, k) > 0) {
+            rp += lane64(snz, k);
+        }
+    }
+    if (lane < R && r0 + lane < op.arows) {
+        fo.rowptr[r0 + lane] = my_rp;
+        bool fl = false;
+#pragma unroll
+        for (int k = 0; k < R; ++k) fl |= (lane == k) & fused[k];
+        if (fl) ws.annz[r0 + lane] = my_cnt;
+    }
+    if (lane == 0 && flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
+}
+
+}  // namespace
+
+int64_t fused_tiles(int32_t rows) {
+    constexpr int64_t tile = (int64_t)(kFusedTpb / 64) * kFusedRows;
+    return ((int64_t)rows + tile - 1) / tile;
+}
+
+void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b, int cus,
+                  hipStream_t stream) {
+    if (op.arows <= 0) return;
+    (void)cus;
+    const dim3 grid((unsigned)fo.ntiles);
+    if (small_b)
+        hipLaunchKernelGGL(k_fused<true>, grid, dim3(kFusedTpb), 0, stream, op, ws, fo);
+    else
+        hipLaunchKernelGGL(k_fused<false>, grid, dim3(kFusedTpb), 0, stream, op, ws, fo);
+}
+
+}  // namespace spg

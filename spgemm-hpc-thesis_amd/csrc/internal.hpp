@@ -26,6 +26,12 @@ constexpr int kCapBlock = 2048;
 constexpr int kTpbBlock = 256;
 constexpr int kCapLarge = 1024;  // products per chunk on the large-row path
 constexpr int kTpbLarge = 256;
+// Single-pass path: rows with at most kFusedCap products and A entries are
+// computed by one wave each, kFusedRows consecutive rows per wave, in tiles of
+// (kFusedTpb / 64) * kFusedRows rows (these are exactly the rows of bins W1, W2).
+constexpr int kFusedCap = kCapW2;
+constexpr int kFusedTpb = 256;
+constexpr int kFusedRows = 2;
 
 // Error flags raised by kernels (OR-ed into Control::flags).
 enum : unsigned {
@@ -40,8 +46,28 @@ struct Control {
     unsigned long long products;    // sum of P_row
     unsigned long long flags;       // kErr* bits
     unsigned long long actual;      // stored entries after the numeric phase
-    unsigned long long structural;  // distinct coordinates (scan total)
-    int bin_count[8];               // rows listed per bin
+    unsigned long long structural;  // distinct coordinates (scan total, two-phase path)
+    unsigned long long sym_total;   // distinct coordinates counted by the symbolic kernels
+    unsigned long long ub_small;    // sum of P_row over rows of bins W1, W2
+    unsigned long long reserved;    // entries of the C row layout (single-pass path)
+    unsigned long long distinct;    // distinct coordinates of the single-pass rows
+    int bin_count[8];               // rows listed per bin (list lengths)
+    long long bin_rows[8];          // rows per bin (statistics; listed or not)
+};
+
+// Per-block partial sums of the analyze pass, reduced by one small kernel
+// (a device-scope atomic on one address costs tens of nanoseconds, so the
+// hot kernels never add into the control block per wave).
+constexpr int kAnalyzeFields = 2 + kNumBins;  // products, ub of bins W1/W2, rows per bin
+
+// Outputs of the single-pass kernel.
+struct FusedOut {
+    int64_t *rowptr;    // [rows+1] C row offsets (all rows)
+    int32_t *ccol;
+    double *cval;
+    uint64_t *status;   // [ntiles] look-back words, zero before the launch
+    int64_t *distinct;  // [ntiles] distinct columns of the tile's single-pass rows
+    int64_t ntiles;
 };
 
 // Operands as the kernels see them.  B rows are reached through a directory
@@ -79,6 +105,8 @@ struct Workspace {
     int64_t *annz;        // [arows] stored count per row
     int64_t *scan_tmp;    // block partials of the scan
     Control *ctrl;
+    int64_t *part;        // [analyze blocks * kAnalyzeFields] partial sums
+    int list_small;       // analyze also lists the rows of bins W1, W2 (two-phase path)
     uint32_t *lbits;      // [lblocks * lwidth/32] large-row presence bitmaps (zero between rows)
     double *lacc;         // [lblocks * lwidth] large-row dense accumulators
     int64_t lwidth;       // column window of the large-row path (multiple of 32)
@@ -89,17 +117,30 @@ struct Workspace {
 void launch_zero_bytes(void *p, size_t bytes, hipStream_t stream);  // bytes: multiple of 4
 void launch_zero_control(Control *ctrl, hipStream_t stream);
 // Per-row work bound, bins and bin row lists (analyze.hip).
+int64_t analyze_blocks(int32_t rows);
 void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream);
-// Symbolic phase for every bin (symbolic.hip); kernels read their row counts
-// from the control block, so no host synchronisation is needed before it.
-void launch_symbolic(const Operands &op, const Workspace &ws, int cus, hipStream_t stream);
+// dst = sum of in[0..n) (one workgroup; scan.hip).
+void launch_sum(const int64_t *in, int64_t n, unsigned long long *dst, hipStream_t stream);
+// Bin sets of the launchers: bit b selects bin b.
+constexpr unsigned kSmallBins = (1u << kBinW1) | (1u << kBinW2);
+constexpr unsigned kBigBins = (1u << kBinW3) | (1u << kBinBlock) | (1u << kBinLarge);
+constexpr unsigned kAllBins = kSmallBins | kBigBins;
+// Symbolic phase for the selected bins (symbolic.hip); kernels read their row
+// counts from the control block, so no host synchronisation is needed before it.
+void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream);
+// Single-pass kernel over all rows (fused.hip): computes the rows of bins W1,
+// W2 and reserves the symbolic counts of the others.  small_b: every B entry
+// index i has 8*i < 2^32.
+int64_t fused_tiles(int32_t rows);
+void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b, int cus,
+                  hipStream_t stream);
 // Exclusive scan of n int64 counts into out[0..n]; out[n] = total; total also
 // stored to *total_dst when non-null (scan.hip).
 size_t scan_tmp_elems(int64_t n);
 void launch_scan(const int64_t *in, int64_t *out, int64_t n, int64_t *tmp, unsigned long long *total_dst,
                  hipStream_t stream);
-// Numeric phase for the bins whose host-side counts are nonzero (numeric.hip).
-void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, int cus,
+// Numeric phase for the selected bins whose host-side counts are nonzero (numeric.hip).
+void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, unsigned bins, int cus,
                     const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream);
 // Copy each row's entries from (rowptr_in) to (rowptr_out); the count of row r
 // is rowptr_out[r+1] - rowptr_out[r] (scan.hip).

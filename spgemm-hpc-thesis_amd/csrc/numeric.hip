@@ -469,13 +469,17 @@ void launch_esc(const Operands &op, const Workspace &ws, int bin, int rows, int 
 
 }  // namespace
 
-void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, int cus,
+void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, unsigned bins, int cus,
                     const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream) {
-    launch_esc<64, kCapW1>(op, ws, kBinW1, bin_counts[kBinW1], cus, crowptr, ccol, cval, stream);
-    launch_esc<64, kCapW2>(op, ws, kBinW2, bin_counts[kBinW2], cus, crowptr, ccol, cval, stream);
-    launch_esc<64, kCapW3>(op, ws, kBinW3, bin_counts[kBinW3], cus, crowptr, ccol, cval, stream);
-    launch_esc<kTpbBlock, kCapBlock>(op, ws, kBinBlock, bin_counts[kBinBlock], cus, crowptr, ccol, cval, stream);
-    if (bin_counts[kBinLarge] > 0) {
+    if (bins & (1u << kBinW1))
+        launch_esc<64, kCapW1>(op, ws, kBinW1, bin_counts[kBinW1], cus, crowptr, ccol, cval, stream);
+    if (bins & (1u << kBinW2))
+        launch_esc<64, kCapW2>(op, ws, kBinW2, bin_counts[kBinW2], cus, crowptr, ccol, cval, stream);
+    if (bins & (1u << kBinW3))
+        launch_esc<64, kCapW3>(op, ws, kBinW3, bin_counts[kBinW3], cus, crowptr, ccol, cval, stream);
+    if (bins & (1u << kBinBlock))
+        launch_esc<kTpbBlock, kCapBlock>(op, ws, kBinBlock, bin_counts[kBinBlock], cus, crowptr, ccol, cval, stream);
+    if ((bins & (1u << kBinLarge)) && bin_counts[kBinLarge] > 0) {
         const int g = bin_counts[kBinLarge] < ws.lblocks ? bin_counts[kBinLarge] : ws.lblocks;
         hipLaunchKernelGGL((k_large_numeric<kTpbLarge, kCapLarge>), dim3(g), dim3(kTpbLarge), 0, stream, op, ws,
                            crowptr, ccol, cval);

@@ -64,6 +64,14 @@ __global__ __launch_bounds__(kScanTpb) void k_scan_down(const int64_t *in, int64
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *grand;
 }
 
+__global__ __launch_bounds__(kScanTpb) void k_sum(const int64_t *in, int64_t n, unsigned long long *dst) {
+    __shared__ int64_t s_ws[kScanTpb / 64];
+    int64_t v = 0;
+    for (int64_t i = threadIdx.x; i < n; i += kScanTpb) v += in[i];
+    v = block_sum<kScanTpb>(v, s_ws);
+    if (threadIdx.x == 0) *dst = (unsigned long long)v;
+}
+
 // One wave per row (only when exact zeros were dropped).
 __global__ __launch_bounds__(256) void k_compact(int32_t rows, const int64_t *rp_in, const int32_t *col_in,
                                                  const double *val_in, const int64_t *rp_out, int32_t *col_out,
@@ -96,6 +104,10 @@ void launch_scan(const int64_t *in, int64_t *out, int64_t n, int64_t *tmp, unsig
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanTpb), 0, stream, tmp, parts, grand, total_dst);
     const int64_t launch_parts = parts > 0 ? parts : 1;
     hipLaunchKernelGGL(k_scan_down, dim3((unsigned)launch_parts), dim3(kScanTpb), 0, stream, in, n, tmp, out, grand);
+}
+
+void launch_sum(const int64_t *in, int64_t n, unsigned long long *dst, hipStream_t stream) {
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(kScanTpb), 0, stream, in, n, dst);
 }
 
 void launch_compact(int32_t rows, const int64_t *rowptr_in, const int32_t *col_in, const double *val_in,

@@ -37,7 +37,8 @@ struct HashLds {
 template <int TPB, int CAP>
 __device__ __forceinline__ void hash_step(const Operands &op, const Workspace &ws, const RowRec *list, int count,
                                           int G, int &i, RowRec &cur, const MetaPre &pcur, MetaPre &pnext,
-                                          int &wnext, HashLds<TPB, CAP> &sh, unsigned &flags) {
+                                          int &wnext, HashLds<TPB, CAP> &sh, unsigned long long &sym,
+                                          unsigned &flags) {
     constexpr int ITEMS = CAP / TPB;
     constexpr int FILL = HashLds<TPB, CAP>::FILL;
     constexpr int TMAX = HashLds<TPB, CAP>::TMAX;
@@ -92,6 +93,7 @@ __device__ __forceinline__ void hash_step(const Operands &op, const Workspace &w
         }
         const int total = block_sum<TPB>(fresh, sh.wsi);
         if (tid == 0) ws.snnz[cur.row] = total;
+        sym += total;
         __syncthreads();
         for (int q = tid; q < T; q += TPB) sh.tab[q] = kEmptyKey;
     }
@@ -110,6 +112,7 @@ __global__ __launch_bounds__(TPB) void k_symbolic_hash(Operands op, Workspace ws
     const int count = ws.ctrl->bin_count[binid];
     const int G = (int)gridDim.x;
     unsigned flags = 0;
+    unsigned long long sym = 0;
     int i = (int)blockIdx.x;
     if (i < count) {
         // Two-step unrolled loop with ping-pong prefetch registers (pa, pb), so
@@ -119,11 +122,12 @@ __global__ __launch_bounds__(TPB) void k_symbolic_hash(Operands op, Workspace ws
         int wnext = rec_word(list, min(i + G, count - 1));
         for (;;) {
             if (i >= count) break;
-            hash_step<TPB, CAP>(op, ws, list, count, G, i, cur, pa, pb, wnext, sh, flags);
+            hash_step<TPB, CAP>(op, ws, list, count, G, i, cur, pa, pb, wnext, sh, sym, flags);
             if (i >= count) break;
-            hash_step<TPB, CAP>(op, ws, list, count, G, i, cur, pb, pa, wnext, sh, flags);
+            hash_step<TPB, CAP>(op, ws, list, count, G, i, cur, pb, pa, wnext, sh, sym, flags);
         }
     }
+    if (tid == 0 && sym) atomicAdd(&ws.ctrl->sym_total, sym);
     if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
 }
 
@@ -170,6 +174,7 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
     const int64_t W = ws.lwidth;
     uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
     unsigned flags = 0;
+    unsigned long long sym = 0;
     for (int i = (int)blockIdx.x; i < count; i += (int)gridDim.x) {
         const int row = list[i].row;
         const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
@@ -198,7 +203,9 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
         }
         const int64_t total = block_sum<TPB>(cnt, s_ws64);
         if (tid == 0) ws.snnz[row] = total;
+        sym += (unsigned long long)total;
     }
+    if (tid == 0 && sym) atomicAdd(&ws.ctrl->sym_total, sym);
     if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
 }
 
@@ -211,13 +218,13 @@ void launch_hash(const Operands &op, const Workspace &ws, int bin, int cus, hipS
 
 }  // namespace
 
-void launch_symbolic(const Operands &op, const Workspace &ws, int cus, hipStream_t stream) {
+void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream) {
     if (op.arows <= 0) return;
-    launch_hash<64, kCapW1>(op, ws, kBinW1, cus, stream);
-    launch_hash<64, kCapW2>(op, ws, kBinW2, cus, stream);
-    launch_hash<64, kCapW3>(op, ws, kBinW3, cus, stream);
-    launch_hash<kTpbBlock, kCapBlock>(op, ws, kBinBlock, cus, stream);
-    if (ws.lblocks > 0)
+    if (bins & (1u << kBinW1)) launch_hash<64, kCapW1>(op, ws, kBinW1, cus, stream);
+    if (bins & (1u << kBinW2)) launch_hash<64, kCapW2>(op, ws, kBinW2, cus, stream);
+    if (bins & (1u << kBinW3)) launch_hash<64, kCapW3>(op, ws, kBinW3, cus, stream);
+    if (bins & (1u << kBinBlock)) launch_hash<kTpbBlock, kCapBlock>(op, ws, kBinBlock, cus, stream);
+    if ((bins & (1u << kBinLarge)) && ws.lblocks > 0)
         hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge>), dim3(ws.lblocks), dim3(kTpbLarge), 0, stream, op,
                            ws);
 }

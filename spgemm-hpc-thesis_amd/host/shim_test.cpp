@@ -1,0 +1,104 @@
+// shim_test.cpp -- drives the C++17 shim the way a reference caller would.
+//
+//   shim_test A.mtx B.mtx out.csr
+//
+// Reads both MatrixMarket files with readMatrixMarket, forms C = A*B with
+// spgemm() at -np 1 (localBBlock = all of B, no halo), writes C in the binary
+// CSR layout ("SPGCSR01", rows, cols int32, nnz int64, row_ptr, col, val; all
+// little endian), then checks the exception types of the error paths.
+#include "spgemm_local.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+
+namespace {
+
+void write_csr(const std::string &path, const spgemm_amd::CsrMatrix &m) {
+    std::ofstream f(path, std::ios::binary);
+    const char magic[8] = {'S', 'P', 'G', 'C', 'S', 'R', '0', '1'};
+    f.write(magic, 8);
+    const int32_t dims[2] = {m.rows, m.cols};
+    f.write(reinterpret_cast<const char *>(dims), sizeof dims);
+    const int64_t nnz = m.rowPtr.back();
+    f.write(reinterpret_cast<const char *>(&nnz), sizeof nnz);
+    f.write(reinterpret_cast<const char *>(m.rowPtr.data()), (std::streamsize)(m.rowPtr.size() * 8));
+    f.write(reinterpret_cast<const char *>(m.columnIndices.data()), (std::streamsize)(m.columnIndices.size() * 4));
+    f.write(reinterpret_cast<const char *>(m.values.data()), (std::streamsize)(m.values.size() * 8));
+    if (!f) throw std::runtime_error("cannot write " + path);
+}
+
+template <typename E, typename F>
+bool throws(F &&f, const char *what) {
+    try {
+        f();
+    } catch (const E &) {
+        return true;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "%s: wrong exception: %s\n", what, e.what());
+        return false;
+    }
+    std::fprintf(stderr, "%s: no exception\n", what);
+    return false;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc != 4) {
+        std::fprintf(stderr, "usage: %s A.mtx B.mtx out.csr\n", argv[0]);
+        return 2;
+    }
+    using namespace spgemm_amd;
+    try {
+        const CsrMatrix a = readMatrixMarket(argv[1]);
+        const CsrMatrix b = readMatrixMarket(argv[2]);
+        Engine engine(0);
+        const CsrMatrix c = spgemm(engine, a, RowBlock{0, b.rows}, b);
+        write_csr(argv[3], c);
+        const spg_stats st = engine.lastStats();
+        std::printf("{\"rows\": %d, \"cols\": %d, \"nnz\": %lld, \"products\": %lld}\n", c.rows, c.cols,
+                    (long long)c.rowPtr.back(), (long long)st.products);
+
+        bool ok = true;
+        // A.cols != B.rows without a halo: std::invalid_argument
+        // (spgemm_common.cpp:106-110).
+        CsrMatrix bad = b;
+        bad.rows += 1;
+        bad.rowPtr.push_back(bad.rowPtr.back());
+        ok &= throws<std::invalid_argument>([&] { spgemm(engine, a, RowBlock{0, bad.rows}, bad); }, "shape");
+        // Missing halo row: MissingRowError (fail() in spgemm_exchange.cpp:82-85).
+        if (a.rowPtr.back() > 0 && b.rows > 1) {
+            CsrMatrix top;  // B without its last row: that row must come from a halo
+            top.rows = b.rows - 1;
+            top.cols = b.cols;
+            top.rowPtr.assign(b.rowPtr.begin(), b.rowPtr.end() - 1);
+            top.columnIndices.assign(b.columnIndices.begin(), b.columnIndices.begin() + top.rowPtr.back());
+            top.values.assign(b.values.begin(), b.values.begin() + top.rowPtr.back());
+            bool refs_last = false;
+            for (int col : a.columnIndices) refs_last |= col == b.rows - 1;
+            if (refs_last) {
+                HaloRows empty;
+                ok &= throws<MissingRowError>([&] { spgemm(engine, a, RowBlock{0, top.rows}, top, &empty); },
+                                              "missing row");
+                HaloRows halo;  // the same row supplied as a halo slot gives the same C
+                halo.globalRows.push_back(b.rows - 1);
+                const int64_t s = b.rowPtr[b.rows - 1], e = b.rowPtr[b.rows];
+                halo.rowPtr.push_back(e - s);
+                halo.columnIndices.assign(b.columnIndices.begin() + s, b.columnIndices.begin() + e);
+                halo.values.assign(b.values.begin() + s, b.values.begin() + e);
+                const CsrMatrix ch = spgemm(engine, a, RowBlock{0, top.rows}, top, &halo);
+                ok &= ch.rowPtr == c.rowPtr && ch.columnIndices == c.columnIndices &&
+                      std::memcmp(ch.values.data(), c.values.data(), c.values.size() * 8) == 0;
+                if (!ok) std::fprintf(stderr, "halo product differs\n");
+            }
+        }
+        ok &= throws<std::runtime_error>([&] { readMatrixMarket("/nonexistent/file.mtx"); }, "mm open");
+        std::printf(ok ? "shim ok\n" : "shim FAILED\n");
+        return ok ? 0 : 1;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "shim_test: %s\n", e.what());
+        return 1;
+    }
+}

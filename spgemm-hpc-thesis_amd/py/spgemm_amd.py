@@ -32,8 +32,10 @@ EXPORTED = [
     "spg_accumulator_create", "spg_accumulator_add", "spg_accumulator_finish", "spg_accumulator_copy",
     "spg_accumulator_destroy", "spg_matrix_read_mm", "spg_matrix_view", "spg_matrix_free",
     "spg_matrix_last_error", "spg_matrix_upload", "spg_device_matrix_view", "spg_device_matrix_free",
-    "spg_synchronize",
+    "spg_synchronize", "spg_set_path",
 ]
+
+PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE = 0, 1, 2
 
 
 @dataclass
@@ -68,11 +70,13 @@ class HaloView(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("products", C.c_int64), ("structural_nnz", C.c_int64), ("nnz", C.c_int64),
                 ("rows_per_bin", C.c_int64 * 8), ("compacted", C.c_int32), ("bins", C.c_int32),
+                ("single_pass", C.c_int32), ("reserved_", C.c_int32),
                 ("ms_total", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double)]
 
     def as_dict(self) -> dict:
         return {"products": self.products, "structural_nnz": self.structural_nnz, "nnz": self.nnz,
                 "rows_per_bin": list(self.rows_per_bin)[: self.bins], "compacted": bool(self.compacted),
+                "single_pass": bool(self.single_pass),
                 "ms_total": self.ms_total, "ms_symbolic": self.ms_symbolic, "ms_numeric": self.ms_numeric}
 
 
@@ -196,6 +200,7 @@ class Engine:
             raise SpgError(s, "spg_create failed")
         self._h = h
         self._L = L
+        self._path = PATH_AUTO
         if timing:
             L.spg_set_timing(h, 1)
 
@@ -225,6 +230,11 @@ class Engine:
 
     def set_timing(self, on: bool):
         self._check(self._L.spg_set_timing(self._h, 1 if on else 0))
+
+    def set_path(self, path: int):
+        """PATH_AUTO, PATH_SINGLE_PASS or PATH_TWO_PHASE (results are identical)."""
+        self._check(self._L.spg_set_path(self._h, int(path)))
+        self._path = int(path)
 
     def stats(self) -> dict:
         st = Stats()

@@ -39,7 +39,19 @@ def all_cases():
     return out
 
 
-def test_golden_products_bit_exact(engine, golden):
+PATHS = [("auto", S.PATH_AUTO), ("single_pass", S.PATH_SINGLE_PASS), ("two_phase", S.PATH_TWO_PHASE)]
+
+
+@pytest.fixture(params=PATHS, ids=[p[0] for p in PATHS])
+def path_engine(request, engine):
+    """The session engine pinned to one pipeline (results must not depend on it)."""
+    engine.set_path(request.param[1])
+    yield engine
+    engine.set_path(S.PATH_AUTO)
+
+
+def test_golden_products_bit_exact(path_engine, golden):
+    engine = path_engine
     for name, a, b in all_cases():
         print("case", name, flush=True)
         c = engine.multiply(to_s(a), to_s(b))
@@ -48,6 +60,9 @@ def test_golden_products_bit_exact(engine, golden):
         assert st["products"] == g["products"], name
         assert c.nnz == g["c_nnz"], name
         assert digest(c) == g["digest"], name
+        if a.nnz:
+            # small products take the single pass unless two-phase is pinned
+            assert st["single_pass"] == (engine._path != S.PATH_TWO_PHASE), name
 
 
 def test_validators_accept_engine_output(engine):
@@ -76,12 +91,15 @@ def csr(rows, cols, rp, col, val):
     return S.Csr.make(rows, cols, rp, col, val)
 
 
-def test_special_values(engine):
+def test_special_values(path_engine):
+    engine = path_engine
     unit = csr(1, 1, [0, 1], [0], [1.0])
     # 1e-200^2 underflows to zero and is dropped (spgemm_validation_smoke.cpp:74-75)
     t = csr(1, 1, [0, 1], [0], [1e-200])
     c = engine.multiply(t, t)
-    assert c.nnz == 0 and engine.stats()["compacted"]
+    assert c.nnz == 0 and c.row_ptr.tolist() == [0, 0]
+    st = engine.stats()
+    assert st["compacted"] == (not st["single_pass"])  # two-phase counted the zero
     # subnormal operands survive (spgemm_full_streaming.cpp:108-110)
     sub = csr(1, 1, [0, 1], [0], [5e-324])
     c = engine.multiply(sub, unit)
@@ -258,3 +276,26 @@ def test_rmat_s13_AA(engine):
     st = engine.stats()
     assert st["rows_per_bin"][4] > 0
     assert digest(c) == O.csr_digest(O.product(to_o(a), to_o(a)))
+
+
+SHIM = os.path.join(os.path.dirname(S.LIB_PATH), "shim_test")
+
+
+def test_cpp_shim_matches_golden(golden, tmp_path):
+    """The C++17 shim (reference-shaped spgemm/readMatrixMarket over the C ABI)
+    gives the golden product and the reference's exception types."""
+    import subprocess
+    assert os.path.exists(SHIM), "build the library first (shim_test is built beside it)"
+    for name, pa, pb in [("cage8_AA", "cage8.mtx", "cage8.mtx"),
+                         ("cage8_AR", "cage8.mtx", "cage8_restriction.mtx"),
+                         ("rectangular", "rectangular_a.mtx", "rectangular_b.mtx")]:
+        if name not in golden["products"]:
+            continue
+        out = tmp_path / f"{name}.csr"
+        print("case", name, flush=True)
+        r = subprocess.run([SHIM, cases.data_path(pa), cases.data_path(pb), str(out)],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "shim ok" in r.stdout
+        c = O.read_csr(str(out))
+        assert O.csr_digest(c) == golden["products"][name]["digest"], name

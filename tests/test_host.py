@@ -52,6 +52,23 @@ def test_create_without_device_reports_enodevice():
     assert s == 7  # SPG_ENODEVICE
 
 
+def test_cpp_shim_builds_and_fails_loudly_without_device(tmp_path):
+    """The C++17 shim program links against the library; with no GPU the
+    Engine constructor throws (no silent CPU path)."""
+    import ctypes as C
+    shim = os.path.join(os.path.dirname(S.LIB_PATH), "shim_test")
+    assert os.path.exists(shim)
+    h = C.c_void_p()
+    if S.lib().spg_create(C.byref(h), 0) == 0:
+        S.lib().spg_destroy(h)
+        pytest.skip("a GPU is present")
+    a = cases.data_path("cage8.mtx")
+    r = subprocess.run([shim, a, a, str(tmp_path / "c.csr")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "device" in r.stderr.lower(), r.stderr
+    assert not (tmp_path / "c.csr").exists()
+
+
 def test_mm_reader_matches_reference(golden):
     for rel, g in golden["mm_read"].items():
         path = cases.data_path(rel)
