@@ -93,101 +93,127 @@ __device__ __forceinline__ MetaReg<SMALLB> load_meta(const Operands &op, const W
     return m;
 }
 
-// One row (1 <= P <= kFusedCap products, 1 <= na <= kFusedCap entries) by one wave.
-// Returns the stored entry count; `distinct` receives the distinct columns.
-template <bool SMALLB>
-__device__ __forceinline__ int fused_row(const Operands &op, const Workspace &ws, FusedWave<SMALLB> &w, int64_t a0,
-                                         int na, int P, const MetaReg<SMALLB> &m0, RowOut &out, int &distinct,
-                                         unsigned &flags) {
-    using bst_t = typename FusedWave<SMALLB>::bst_t;
-    const int lane = lane_id();
+// A row's products in registers: slot j of each lane.
+struct RowItems {
+    int col[kFItems];
+    int sidx[kFItems];   // serial index, -1 for an empty slot
+    double bv[kFItems];  // B value, then the product
+    double av[kFItems];  // A value
+};
 
-    // --- the row's entries into LDS: B row starts, A values, product offsets.
-    // Lanes past the row's entries write harmless values below kFusedCap;
-    // off[na] is written last.
-    int L = 0, carry = 0;
-    for (int c0 = 0; c0 < na; c0 += 64) {
-        const MetaReg<SMALLB> m = c0 == 0 ? m0 : load_meta<SMALLB>(op, ws, a0, na, c0);
+// Entries a0 .. a0+E-1 (E <= kFusedCap) into LDS: B row start, A value and
+// the exclusive prefix of the B row lengths over all E entries (off[E] is
+// the total).  m0 holds entries 0..63.  Lanes past E write harmless values
+// below kFusedCap; off[E] is written last.
+template <bool SMALLB>
+__device__ __forceinline__ void stage_meta(const Operands &op, const Workspace &ws, FusedWave<SMALLB> &w, int64_t a0,
+                                           int E, const MetaReg<SMALLB> &m0) {
+    const int lane = lane_id();
+    int carry = 0;
+    for (int c0 = 0; c0 < E; c0 += 64) {
+        const MetaReg<SMALLB> m = c0 == 0 ? m0 : load_meta<SMALLB>(op, ws, a0, E, c0);
         const int e = c0 + lane;
-        const int len = e < na ? m.len : 0;
+        const int len = e < E ? m.len : 0;
         const int incl = wave_incl_sum_dpp(len);
         w.bst[e] = m.bst;
         w.av[e] = m.av;
         w.off[e] = carry + incl - len;
         carry += wave_last(incl);
-        L = max(L, len);
+    }
+    wave_sync();
+    if (lane == 0) w.off[E] = carry;
+    wave_sync();
+}
+
+// Expansion of the row whose entries are staged at [m, m+na) with P products:
+// slot j of each lane gets one product's column, serial index and values; the
+// B gathers are issued here and consumed later.
+template <bool SMALLB>
+__device__ __forceinline__ void expand_row(const Operands &op, FusedWave<SMALLB> &w, int m, int na, int P,
+                                           RowItems &it, unsigned &flags) {
+    using bst_t = typename FusedWave<SMALLB>::bst_t;
+    const int lane = lane_id();
+    const int sbase = uni(w.off[m]);
+    if (w.off[m + na] - sbase != P) flags |= kErrInternal;  // P comes from the same lengths
+    int L = 0;
+    for (int i = 0; i < na; i += 64) {
+        const int e = m + min(i + lane, na - 1);
+        L = max(L, w.off[e + 1] - w.off[e]);
     }
     L = wave_max_dpp(L);
-    wave_sync();
-    if (lane == 0) w.off[na] = carry;
-    if (carry != P) flags |= kErrInternal;  // P comes from the same lengths
-    wave_sync();
-
-    // --- expansion: slot j of each lane gets one product (column, value, s)
-    int col[kFItems], sidx[kFItems];
-    double pv[kFItems];
-    {
-        int Lp = 1;
-        while (Lp < L) Lp <<= 1;
-        Lp = uni(Lp);
-        const int logL = 31 - __clz(Lp);
-        const int G = Lp <= 64 ? (64 >> logL) : 1;
-        const bool entry_major = Lp <= 64 && na <= kFItems * G;
-        bst_t bs[kFItems];
-        int t[kFItems];
-        double av[kFItems];
-        bool ok[kFItems];
-        if (entry_major) {
-            // Lp lanes per A entry: contiguous reads of each B row, no search.
-            const int sub = lane & (Lp - 1);
-            const int grp = lane >> logL;
+    int Lp = 1;
+    while (Lp < L) Lp <<= 1;
+    Lp = uni(Lp);
+    const int logL = 31 - __clz(Lp);
+    const int G = Lp <= 64 ? (64 >> logL) : 1;
+    const bool entry_major = Lp <= 64 && na <= kFItems * G;
+    bst_t bs[kFItems];
+    int t[kFItems];
+    bool ok[kFItems];
+    if (entry_major) {
+        // Lp lanes per A entry: contiguous reads of each B row, no search.
+        const int sub = lane & (Lp - 1);
+        const int grp = lane >> logL;
 #pragma unroll
-            for (int j = 0; j < kFItems; ++j) {
-                const int e = j * G + grp;
-                const bool in = e < na;
-                const int ec = in ? e : 0;
-                const int o = w.off[ec];
-                ok[j] = in && sub < w.off[ec + 1] - o;
-                bs[j] = w.bst[ec];
-                av[j] = w.av[ec];
-                t[j] = sub;
-                sidx[j] = ok[j] ? o + sub : -1;
-            }
-        } else {
-            // Serial order dealt out lane by lane; entry found by binary search.
-#pragma unroll
-            for (int j = 0; j < kFItems; ++j) {
-                const int s = lane + 64 * j;
-                ok[j] = s < P;
-                const int e = ok[j] ? upper_entry(w.off, na, s) : 0;
-                bs[j] = w.bst[e];
-                av[j] = w.av[e];
-                t[j] = ok[j] ? s - w.off[e] : 0;
-                sidx[j] = ok[j] ? s : -1;
-            }
+        for (int j = 0; j < kFItems; ++j) {
+            const int e = j * G + grp;
+            const bool in = e < na;
+            const int ec = m + (in ? e : 0);
+            const int o = w.off[ec];
+            ok[j] = in && sub < w.off[ec + 1] - o;
+            bs[j] = w.bst[ec];
+            it.av[j] = w.av[ec];
+            t[j] = sub;
+            it.sidx[j] = ok[j] ? o - sbase + sub : -1;
         }
-        // Every slot's loads are issued before any product is formed; empty
-        // slots read B entry 0 (the row has products, so B is not empty).
-        double bv[kFItems];
-        if constexpr (SMALLB) {
-            // Byte offsets fit 32 bits: scalar base + vector offset addressing.
+    } else {
+        // Serial order dealt out lane by lane; entry found by binary search.
 #pragma unroll
-            for (int j = 0; j < kFItems; ++j) {
-                const uint32_t ix = ok[j] ? bs[j] + (uint32_t)t[j] : 0u;
-                col[j] = ld_byte(op.bcol, ix * 4u);
-                bv[j] = ld_byte(op.bval, ix * 8u);
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < kFItems; ++j) {
-                const int64_t ix = ok[j] ? bs[j] + t[j] : 0;
-                col[j] = op.bcol[ix];
-                bv[j] = op.bval[ix];
-            }
+        for (int j = 0; j < kFItems; ++j) {
+            const int s = lane + 64 * j;
+            ok[j] = s < P;
+            const int e = m + (ok[j] ? upper_entry(w.off + m, na, s + sbase) : 0);
+            bs[j] = w.bst[e];
+            it.av[j] = w.av[e];
+            t[j] = ok[j] ? s + sbase - w.off[e] : 0;
+            it.sidx[j] = ok[j] ? s : -1;
         }
-#pragma unroll
-        for (int j = 0; j < kFItems; ++j) pv[j] = av[j] * bv[j];
     }
+    // Every slot's loads are issued before any product is formed; empty
+    // slots read B entry 0 (the row has products, so B is not empty).
+    if constexpr (SMALLB) {
+        // Byte offsets fit 32 bits: scalar base + vector offset addressing.
+#pragma unroll
+        for (int j = 0; j < kFItems; ++j) {
+            const uint32_t ix = ok[j] ? bs[j] + (uint32_t)t[j] : 0u;
+            it.col[j] = ld_byte(op.bcol, ix * 4u);
+            it.bv[j] = ld_byte(op.bval, ix * 8u);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kFItems; ++j) {
+            const int64_t ix = ok[j] ? bs[j] + t[j] : 0;
+            it.col[j] = op.bcol[ix];
+            it.bv[j] = op.bval[ix];
+        }
+    }
+}
+
+__device__ __forceinline__ void form_products(RowItems &it) {
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) it.bv[j] = it.av[j] * it.bv[j];
+}
+
+// Sort and fold a row's P products (1 <= P <= kFusedCap) held in `it` (after
+// form_products).  Returns the stored entry count; `distinct` receives the
+// distinct columns.
+template <bool SMALLB>
+__device__ __forceinline__ int sort_fold(FusedWave<SMALLB> &w, int P, const RowItems &it, RowOut &out,
+                                         int &distinct) {
+    const int lane = lane_id();
+    const int(&col)[kFItems] = it.col;
+    const int(&sidx)[kFItems] = it.sidx;
+    const double(&pv)[kFItems] = it.bv;
 
     // --- buckets over the row's column range.  Keys are offsets from the
     // row's smallest column (unsigned, order preserving).
@@ -410,30 +436,69 @@ __global__ __launch_bounds__(kFusedTpb) void k_fused(Operands op, Workspace ws, 
     int cnt[R];
     bool fused[R];
     int64_t wave_total = 0;
-    MetaReg<SMALLB> meta[R];
+    bool joint = true;  // every row single-pass or without products
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int64_t ub = lane64(ubl, k);
         const int64_t na = lane64(ap, k + 1) - lane64(ap, k);
         fused[k] = r0 + k < op.arows && ub > 0 && ub <= kFusedCap && na <= kFusedCap;
-    }
-    if (fused[0]) meta[0] = load_meta<SMALLB>(op, ws, lane64(ap, 0), (int)(lane64(ap, 1) - lane64(ap, 0)), 0);
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
+        joint &= fused[k] || ub == 0 || r0 + k >= op.arows;
         cnt[k] = 0;
 #pragma unroll
         for (int j = 0; j < kFItems; ++j) out[k].pos[j] = -1;
-        if (k + 1 < R && fused[k + 1])
-            meta[k + 1] =
-                load_meta<SMALLB>(op, ws, lane64(ap, k + 1), (int)(lane64(ap, k + 2) - lane64(ap, k + 1)), 0);
-        if (fused[k]) {
-            int dist = 0;
-            cnt[k] = fused_row<SMALLB>(op, ws, w, lane64(ap, k), (int)(lane64(ap, k + 1) - lane64(ap, k)),
-                                       (int)lane64(ubl, k), meta[k], out[k], dist, flags);
-            distinct_sum += dist;
-            wave_total += cnt[k];
-        } else if (r0 + k < op.arows && lane64(ubl, k) > 0) {
-            wave_total += lane64(snz, k);  // reserved for the numeric kernels
+        if (!fused[k] && r0 + k < op.arows && ub > 0) wave_total += lane64(snz, k);  // reserved for the numeric kernels
+    }
+    const int64_t E = lane64(ap, R) - lane64(ap, 0);
+    joint &= E <= kFusedCap;
+    // First 64 entries of the wave's rows: row 0's metadata on either path.
+    MetaReg<SMALLB> meta0;
+    if (fused[0] || joint) meta0 = load_meta<SMALLB>(op, ws, lane64(ap, 0), (int)min(E, (int64_t)kFusedCap), 0);
+    if (joint) {
+        // All rows' metadata staged at once and all rows' gathers in flight
+        // together: one memory round trip for the rows of this wave.
+        if (E > 0) {
+            stage_meta<SMALLB>(op, ws, w, lane64(ap, 0), (int)E, meta0);
+            RowItems it[R];
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (fused[k])
+                    expand_row<SMALLB>(op, w, (int)(lane64(ap, k) - lane64(ap, 0)),
+                                       (int)(lane64(ap, k + 1) - lane64(ap, k)), (int)lane64(ubl, k), it[k], flags);
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (fused[k]) form_products(it[k]);
+            wave_sync();  // metadata reads are complete before the sort reuses its storage
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (fused[k]) {
+                    int dist = 0;
+                    cnt[k] = sort_fold<SMALLB>(w, (int)lane64(ubl, k), it[k], out[k], dist);
+                    distinct_sum += dist;
+                    wave_total += cnt[k];
+                }
+            }
+        }
+    } else {
+        // Row by row; the next row's metadata is loaded during this one.
+        MetaReg<SMALLB> meta[R];
+        meta[0] = meta0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (k + 1 < R && fused[k + 1])
+                meta[k + 1] =
+                    load_meta<SMALLB>(op, ws, lane64(ap, k + 1), (int)(lane64(ap, k + 2) - lane64(ap, k + 1)), 0);
+            if (fused[k]) {
+                const int na = (int)(lane64(ap, k + 1) - lane64(ap, k));
+                stage_meta<SMALLB>(op, ws, w, lane64(ap, k), na, meta[k]);
+                RowItems it;
+                expand_row<SMALLB>(op, w, 0, na, (int)lane64(ubl, k), it, flags);
+                form_products(it);
+                wave_sync();
+                int dist = 0;
+                cnt[k] = sort_fold<SMALLB>(w, (int)lane64(ubl, k), it, out[k], dist);
+                distinct_sum += dist;
+                wave_total += cnt[k];
+            }
         }
     }
     if (lane == 0) {
