@@ -5,15 +5,23 @@ Default workload: one rank's share of the large uniform product --
 A and B 1,048,576 x 1,048,576 with 16 uniform random columns per row, fp64
 values U[-1,1) (seeds 42 and 43); C = A.B through the plain drop-in path with
 operands already resident in HBM.  A step is one complete product (analyze,
-symbolic, scan, numeric; output left in HBM).
+single-pass kernel or symbolic + numeric phases; output left in HBM).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
 
+Workloads (--workload):
+  uniform_2p20x16   2^20 x 2^20, 16 uniform columns per row, A.B (default)
+  uniform_16384x8   16384 x 16384, 8 per row, A.B
+  rmat_s20_AA       R-MAT scale 20, edge factor 32, (.57,.19,.19,.05), A.A; the
+                    output (~2.3e10 entries) exceeds device memory, so a step is
+                    one sweep over row chunks (each chunk's C overwrites the last)
+  rmat_s20_rect     the same R-MAT A times B 2^20 x 65536 with 24 uniform columns per row
+
 value     = 2*P / step time (GFLOP/s), P = scalar products (the reference's flop basis).
-roofline  = HBM-bound: algorithmic bytes of the dominant kernel (the single-pass
-            kernel k_fused: 32 B per row + 20 B per A entry + 12 B per scalar
-            product + 12 B per C entry) / its average duration, measured with HIP
-            events on the engine's stream around that launch.
+roofline  = HBM-bound: algorithmic bytes of the dominant kernel / its average
+            duration, measured with HIP events on the engine's stream around
+            that launch.  For the single-pass kernel k_fused: 32 B per row +
+            20 B per A entry + 12 B per scalar product + 12 B per C entry.
 cpu_baseline = the reference's own spgemm() (oracle/_ref/refdrv, -np 1, OpenMP
             guided/64) on a bounded row sample, or the C oracle when that binary
             is absent.
@@ -24,7 +32,6 @@ import argparse
 import json
 import os
 import statistics
-import subprocess
 import sys
 import tempfile
 import time
@@ -36,29 +43,40 @@ sys.path.insert(0, os.path.join(ROOT, "spgemm-hpc-thesis_amd", "py"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-
-WORKLOADS = {
-    # name: (rows, cols, nnz per row, seed A, seed B)
-    "uniform_2p20x16": (1 << 20, 1 << 20, 16, 42, 43),
-    "uniform_16384x8": (16384, 16384, 8, 42, 43),
-}
+CHUNK_PRODUCTS = 3 << 30  # scalar products per row chunk of a sweep (bounds C per chunk)
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_operands(name):
+def _uniform(rows, cols, k, seed):
     import workloads as W
-    rows, cols, k, sa, sb = WORKLOADS[name]
-    a = W.uniform_random(rows, cols, k, seed=sa)
-    b = W.uniform_random(cols, cols, k, seed=sb)
-    return a, b
+    return W.uniform_random(rows, cols, k, seed=seed)
+
+
+def _rmat():
+    import workloads as W
+    return W.rmat(20, 32, seed=42)
+
+
+WORKLOADS = {
+    "uniform_2p20x16": dict(kind="single", desc={"rows": 1 << 20, "cols": 1 << 20, "nnz_per_row": 16, "seeds": [42, 43]},
+                            make=lambda: (_uniform(1 << 20, 1 << 20, 16, 42), _uniform(1 << 20, 1 << 20, 16, 43))),
+    "uniform_16384x8": dict(kind="single", desc={"rows": 16384, "cols": 16384, "nnz_per_row": 8, "seeds": [42, 43]},
+                            make=lambda: (_uniform(16384, 16384, 8, 42), _uniform(16384, 16384, 8, 43))),
+    "rmat_s20_AA": dict(kind="chunked", desc={"rmat_scale": 20, "edge_factor": 32, "abcd": [0.57, 0.19, 0.19, 0.05],
+                                              "seed": 42, "product": "A.A"},
+                        make=lambda: (lambda a: (a, a))(_rmat())),
+    "rmat_s20_rect": dict(kind="single", desc={"rmat_scale": 20, "edge_factor": 32, "seed": 42,
+                                               "B": "2^20 x 65536, 24 uniform columns per row, seed 43"},
+                          make=lambda: (_rmat(), _uniform(1 << 20, 65536, 24, 43))),
+}
 
 
 def traffic_from_profiles(workload):
-    """HBM bytes per numeric launch measured with rocprofv3 PMC counters
-    (profiles/traffic_<workload>.json, written from the FETCH_SIZE/WRITE_SIZE runs)."""
+    """HBM bytes per launch of the dominant kernel, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (profiles/traffic_<workload>.json)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
@@ -74,10 +92,11 @@ def cpu_baseline(a, b, products_total, target_s=15.0):
     import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     rows_total = a.rows
-    # sample: leading rows of A with ~(target) seconds of work at ~50 Mproducts/s/thread
+    # sample: leading rows of A with about target_s seconds of work for the
+    # reference (~3 M products/s per thread with its per-row hash maps)
     per_row = products_total / max(rows_total, 1)
-    budget_products = 25e6 * threads * target_s / 8.0
-    sample_rows = int(max(1024, min(rows_total, budget_products / max(per_row, 1))))
+    budget_products = 3e6 * threads * target_s / 6.0
+    sample_rows = int(max(256, min(rows_total, budget_products / max(per_row, 1))))
     sub_p = int(a.row_ptr[sample_rows])
     asub = O.Csr.make(sample_rows, a.cols, a.row_ptr[: sample_rows + 1], a.col[:sub_p], a.val[:sub_p])
     bo = O.Csr.make(b.rows, b.cols, b.row_ptr, b.col, b.val)
@@ -106,11 +125,44 @@ def cpu_baseline(a, b, products_total, target_s=15.0):
                       f"products), warmup 1, 5 timed repeats, compute P90; OpenMP guided/64"}
 
 
+def oracle_rows(a, b, rows):
+    """Serial-order oracle on selected rows of A (bit-exact reference values)."""
+    import oracle as O
+    rp = [0]
+    cols, vals = [], []
+    for r in rows:
+        s, e = a.row_ptr[r], a.row_ptr[r + 1]
+        cols.append(a.col[s:e])
+        vals.append(a.val[s:e])
+        rp.append(rp[-1] + int(e - s))
+    return O.product(O.Csr.make(len(rows), a.cols, rp, np.concatenate(cols), np.concatenate(vals)),
+                     O.Csr.make(b.rows, b.cols, b.row_ptr, b.col, b.val), threads=max(1, min(16, os.cpu_count() or 1)))
+
+
+def rows_equal(c, r_c, ref, i):
+    return (np.array_equal(c.col[c.row_ptr[r_c]:c.row_ptr[r_c + 1]], ref.col[ref.row_ptr[i]:ref.row_ptr[i + 1]]) and
+            np.array_equal(c.val[c.row_ptr[r_c]:c.row_ptr[r_c + 1]].view(np.uint64),
+                           ref.val[ref.row_ptr[i]:ref.row_ptr[i + 1]].view(np.uint64)))
+
+
+def chunk_bounds(rowp, budget):
+    """Row chunks [r0, r1) whose scalar products stay within budget (a row
+    above the budget gets a chunk of its own)."""
+    cs = np.concatenate([[0], np.cumsum(rowp)])
+    bounds, r0, n = [], 0, len(rowp)
+    while r0 < n:
+        r1 = int(np.searchsorted(cs, cs[r0] + budget, side="right")) - 1
+        r1 = min(max(r1, r0 + 1), n)
+        bounds.append((r0, r1))
+        r0 = r1
+    return bounds
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="uniform_2p20x16", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -118,67 +170,86 @@ def main():
         print("bench.py runs on one GPU only: this engine computes one rank's local product; "
               "--gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    wl = WORKLOADS[args.workload]
+    chunked = wl["kind"] == "chunked"
+    steps = args.steps if args.steps is not None else (1 if chunked else 20)
+    warmup = args.warmup if args.warmup is not None else (0 if chunked else 3)
 
     import spgemm_amd as S
     import workloads as W
 
     log(f"bench: generating {args.workload}")
-    a, b = make_operands(args.workload)
-    P = W.products(a, b)
+    a, b = wl["make"]()
+    rowp = W.row_products(a, b)
+    P = int(rowp.sum())
     eng = S.Engine(0, timing=True)
     da, db = eng.upload(a), eng.upload(b)  # resident in HBM before timing
     eng.synchronize()
+    bounds = chunk_bounds(rowp, CHUNK_PRODUCTS) if chunked else [(0, a.rows)]
+    log(f"bench: P={P}, {len(bounds)} row chunk(s)")
+
+    def one_step():
+        nnz_total = 0
+        kms = []
+        for r0, r1 in bounds:
+            nnz_total += eng.multiply_device(da, db, (r0, r1) if chunked else None)
+            kms.append(eng.stats()["ms_numeric"])
+            if chunked:
+                log(f"bench: rows [{r0}, {r1}) done")
+        return nnz_total, kms
 
     log("bench: warmup")
     nnz = 0
-    for _ in range(args.warmup):
-        nnz = eng.multiply_device(da, db)
+    for _ in range(warmup):
+        nnz, _ = one_step()
     eng.synchronize()
 
-    log(f"bench: timing {args.steps} steps")
-    step_ms, numeric_ms, symbolic_ms = [], [], []
+    log(f"bench: timing {steps} steps")
+    step_ms, kernel_ms, sym_ms = [], [], []
     eng.synchronize()  # one GPU, one process: no barrier beyond the device sync
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         ts = time.perf_counter()
-        nnz = eng.multiply_device(da, db)  # returns after the product is complete on the device
+        nnz, kms = one_step()  # returns after the product is complete on the device
         step_ms.append((time.perf_counter() - ts) * 1e3)
-        st = eng.stats()
-        numeric_ms.append(st["ms_numeric"])
-        symbolic_ms.append(st["ms_symbolic"])
+        kernel_ms.append(sum(kms))
+        sym_ms.append(eng.stats()["ms_symbolic"])
     eng.synchronize()
     total_s = time.perf_counter() - t0
-    ms_per_step = total_s * 1e3 / args.steps
+    ms_per_step = total_s * 1e3 / steps
     st = eng.stats()
 
-    # parity spot check on sampled rows (bit-exact against the oracle)
-    import oracle as O
-    c = eng.result_host()
+    # parity spot check: sampled rows (and the heaviest rows of a sweep)
+    # bit-exact against the serial-order oracle
     rng = np.random.Generator(np.random.PCG64(11))
-    rows = np.sort(rng.choice(a.rows, size=min(1024, a.rows), replace=False))
-    rp = [0]
-    cols, vals = [], []
-    for r in rows:
-        s, e = a.row_ptr[r], a.row_ptr[r + 1]
-        cols.append(a.col[s:e]); vals.append(a.val[s:e]); rp.append(rp[-1] + int(e - s))
-    sub = O.product(O.Csr.make(len(rows), a.cols, rp, np.concatenate(cols), np.concatenate(vals)),
-                    O.Csr.make(b.rows, b.cols, b.row_ptr, b.col, b.val))
-    exact = all(
-        np.array_equal(c.col[c.row_ptr[r]:c.row_ptr[r + 1]], sub.col[sub.row_ptr[i]:sub.row_ptr[i + 1]]) and
-        np.array_equal(c.val[c.row_ptr[r]:c.row_ptr[r + 1]].view(np.uint64),
-                       sub.val[sub.row_ptr[i]:sub.row_ptr[i + 1]].view(np.uint64))
-        for i, r in enumerate(rows))
+    sample = set(rng.choice(a.rows, size=min(512 if not chunked else 128, a.rows), replace=False).tolist())
+    if chunked:
+        sample.update(int(h) for h in np.argsort(rowp)[-2:])
+    sample = np.array(sorted(sample))
+    log(f"bench: parity check on {len(sample)} rows")
+    ref = oracle_rows(a, b, sample)
+    exact = True
+    pos = {int(r): i for i, r in enumerate(sample)}
+    for r0, r1 in bounds:
+        mine = [int(r) for r in sample if r0 <= r < r1]
+        if not mine:
+            continue
+        if chunked:
+            eng.multiply_device(da, db, (r0, r1))
+        for r in mine:
+            c = eng.result_rows(r - r0, r - r0 + 1)
+            exact &= rows_equal(c, 0, ref, pos[r])
 
-    model_bytes = W.streaming_bytes(a, b.rows, P, nnz)
-    numeric_avg = statistics.mean(numeric_ms)
     single = bool(st["single_pass"])
-    if single:
+    if single and not chunked:
         kernel = "k_fused<small B>"
         kernel_bytes = 32 * a.rows + 20 * a.nnz + 12 * P + 12 * nnz
     else:
-        kernel = "k_numeric_esc<64,256>"
+        kernel = "numeric phase (single-pass and big-row kernels)" if single else "numeric phase"
         kernel_bytes = 40 * a.rows + 20 * a.nnz + 12 * P + 12 * nnz
-    achieved = kernel_bytes / (numeric_avg * 1e-3) / 1e9
+    kernel_avg = statistics.mean(kernel_ms)
+    achieved = kernel_bytes / (kernel_avg * 1e-3) / 1e9
+    model_bytes = W.streaming_bytes(a, b.rows, P, nnz)
     traffic = traffic_from_profiles(args.workload)
     cpu = None
     if not args.no_cpu_baseline:
@@ -188,30 +259,32 @@ def main():
         except Exception as e:  # reported, never fatal
             cpu = {"value": None, "unit": "GFLOP/s", "cores": 0, "kind": "reference", "sample": f"failed: {e}"}
 
-    rows_, cols_, k, sa, sb = WORKLOADS[args.workload]
+    cfg = {"workload": args.workload, **wl["desc"], "products": P, "c_nnz": nnz, "a_nnz": a.nnz,
+           "path": "plain drop-in (spg_multiply_begin)" if not chunked else
+                   f"row chunks of <= {CHUNK_PRODUCTS} products (spg_multiply_rows), {len(bounds)} chunks",
+           "p50_ms": round(statistics.median(step_ms), 4),
+           "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
+           "pipeline_model_gbs": round(model_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+           "single_pass": single,
+           "analyze_symbolic_ms_last": round(statistics.mean(sym_ms), 4),
+           "dominant_kernel_ms_avg": round(kernel_avg, 4),
+           "rows_per_bin": st["rows_per_bin"],
+           "max_row_products": int(rowp.max()) if len(rowp) else 0,
+           "parity": f"bit-exact on {len(sample)} sampled rows" if exact else "MISMATCH"}
     out = {
         "metric": "SpGEMM GFLOP/s (2 x scalar products / product time), single product, plain drop-in",
         "value": round(2.0 * P / (ms_per_step * 1e-3) / 1e9, 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded uniform random CSR, generated on the host, resident in HBM before timing)",
-        "config": {"workload": args.workload, "rows": rows_, "cols": cols_, "nnz_per_row": k,
-                   "seeds": [sa, sb], "products": P, "c_nnz": nnz, "path": "plain drop-in (spg_multiply_begin)",
-                   "p50_ms": round(statistics.median(step_ms), 4),
-                   "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
-                   "pipeline_model_gbs": round(model_bytes / (ms_per_step * 1e-3) / 1e9, 1),
-                   "single_pass": single,
-                   "analyze_symbolic_ms_avg": round(statistics.mean(symbolic_ms), 4),
-                   "dominant_kernel_ms_avg": round(numeric_avg, 4),
-                   "rows_per_bin": st["rows_per_bin"],
-                   "parity": f"bit-exact on {len(rows)} sampled rows" if exact else "MISMATCH"},
+        "data": "synthetic (seeded CSR generated on the host, resident in HBM before timing)",
+        "config": cfg,
         "roofline": {"bound": "hbm", "kernel": kernel,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

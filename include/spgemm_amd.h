@@ -134,9 +134,25 @@ spg_status spg_last_stats(const spg_context *ctx, spg_stats *out);
  * nnz(C).  The result stays on the context until the next multiply. */
 spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B_local,
                               int32_t b_first_row, const spg_halo_view *halo, int64_t *nnz_c);
+/* Rows [row_begin, row_end) of the same product: C's rows row_begin.. as a
+ * (row_end - row_begin)-row CSR (row_ptr starting at 0), bit-identical to
+ * those rows of the whole product.  For outputs larger than device memory,
+ * taken in row chunks (the reference computes per-rank row blocks of A,
+ * sliceRows in src/common/spgemm_common.cpp; here the chunks belong to one
+ * device).  No direct counterpart in the reference's spgemm(). */
+spg_status spg_multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B_local,
+                             int32_t b_first_row, const spg_halo_view *halo, int32_t row_begin, int32_t row_end,
+                             int64_t *nnz_c);
 /* Copy the result of the last multiply into caller-owned host buffers of
  * sizes A->rows+1, nnz_c and nnz_c (the shim resizes its std::vectors first). */
 spg_status spg_multiply_finish(spg_context *ctx, int64_t *row_ptr, int32_t *col_idx, double *values);
+/* Rows [row_begin, row_end) of the last result into host buffers: row_ptr
+ * (row_end - row_begin + 1 entries, starting at 0) and up to `capacity`
+ * entries; *nnz_rows receives the rows' entry count (SPG_EINVAL with the
+ * count when capacity is too small).  For spot checks and streaming of large
+ * outputs; no counterpart in the reference. */
+spg_status spg_result_rows(spg_context *ctx, int32_t row_begin, int32_t row_end, int64_t *row_ptr, int32_t *col_idx,
+                           double *values, int64_t capacity, int64_t *nnz_rows);
 /* Device view of the last result (pointers owned by the context, valid until
  * the next multiply on it or spg_destroy). */
 spg_status spg_result_device(spg_context *ctx, spg_csr_view *out);

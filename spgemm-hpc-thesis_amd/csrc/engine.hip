@@ -427,8 +427,11 @@ spg_status spg_last_stats(const spg_context *ctx, spg_stats *out) {
     return SPG_OK;
 }
 
-spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int32_t b_first_row,
-                              const spg_halo_view *halo, int64_t *nnz_c) {
+namespace {
+
+// C rows [row_begin, row_end) of A times B (the whole product for [0, A->rows)).
+spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int32_t b_first_row,
+                         const spg_halo_view *halo, int32_t row_begin, int32_t row_end, int64_t *nnz_c) {
     if (!ctx) return SPG_EINVAL;
     if (!A || !B || !nnz_c) return set_err(ctx, SPG_EINVAL, "null argument");
     ctx->have_result = false;
@@ -441,12 +444,15 @@ spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg
         return set_err(ctx, SPG_EINVAL, "SpGEMM requires A.cols == B.rows");
     if (b_first_row < 0 || (int64_t)b_first_row + B->rows > (int64_t)A->cols)
         return set_err(ctx, SPG_EINVAL, "local B block outside A's column range");
+    if (row_begin < 0 || row_end < row_begin || row_end > A->rows)
+        return set_err(ctx, SPG_EINVAL, "row range outside A");
 
     Operands op{};
-    op.arows = A->rows;
+    op.arows = row_end - row_begin;
     op.nnz_a = A->nnz;
     op.bcols = B->cols;
     if ((s = stage(ctx, A, ctx->a_ptr, ctx->a_col, ctx->a_val, op.aptr, op.acol, op.aval)) != SPG_OK) return s;
+    op.aptr += row_begin;  // entries keep their absolute positions
     const int64_t *bptr = nullptr;
     const int32_t *bcol = nullptr;
     const double *bval = nullptr;
@@ -514,9 +520,23 @@ spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg
     // Every B entry index i reachable through the directory has 8*i < 2^32:
     // the single-pass kernel then addresses B with 32-bit offsets.
     const bool small_b = b_span < (int64_t(1) << 29);
-    if ((s = run_product(ctx, op, A->rows, B->cols, small_b)) != SPG_OK) return s;
+    if ((s = run_product(ctx, op, op.arows, B->cols, small_b)) != SPG_OK) return s;
     *nnz_c = ctx->res_nnz;
     return SPG_OK;
+}
+
+}  // namespace
+
+spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int32_t b_first_row,
+                              const spg_halo_view *halo, int64_t *nnz_c) {
+    if (!ctx) return SPG_EINVAL;
+    if (!A) return set_err(ctx, SPG_EINVAL, "null argument");
+    return multiply_rows(ctx, A, B, b_first_row, halo, 0, A->rows, nnz_c);
+}
+
+spg_status spg_multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int32_t b_first_row,
+                             const spg_halo_view *halo, int32_t row_begin, int32_t row_end, int64_t *nnz_c) {
+    return multiply_rows(ctx, A, B, b_first_row, halo, row_begin, row_end, nnz_c);
 }
 
 spg_status spg_multiply_finish(spg_context *ctx, int64_t *row_ptr, int32_t *col_idx, double *values) {
@@ -533,6 +553,33 @@ spg_status spg_multiply_finish(spg_context *ctx, int64_t *row_ptr, int32_t *col_
                 "download val");
     }
     SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return SPG_OK;
+}
+
+spg_status spg_result_rows(spg_context *ctx, int32_t row_begin, int32_t row_end, int64_t *row_ptr, int32_t *col_idx,
+                           double *values, int64_t capacity, int64_t *nnz_rows) {
+    if (!ctx) return SPG_EINVAL;
+    if (!ctx->have_result) return set_err(ctx, SPG_ESTATE, "no product available");
+    if (!row_ptr || !nnz_rows || row_begin < 0 || row_end < row_begin || row_end > ctx->res_rows)
+        return set_err(ctx, SPG_EINVAL, "bad row range or null buffer");
+    const int c = ctx->cur;
+    const size_t n = (size_t)(row_end - row_begin) + 1;
+    SPG_HIP(hipMemcpyAsync(row_ptr, ctx->rowptr[c].as<int64_t>() + row_begin, n * 8, hipMemcpyDeviceToHost, ctx->stream),
+            "download row_ptr");
+    SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    const int64_t base = row_ptr[0];
+    const int64_t cnt = row_ptr[n - 1] - base;
+    *nnz_rows = cnt;
+    if (cnt > capacity) return set_err(ctx, SPG_EINVAL, "output buffers too small for the row range");
+    if (cnt > 0) {
+        if (!col_idx || !values) return set_err(ctx, SPG_EINVAL, "null output buffer");
+        SPG_HIP(hipMemcpyAsync(col_idx, ctx->ccol[c].as<int32_t>() + base, (size_t)cnt * 4, hipMemcpyDeviceToHost,
+                               ctx->stream), "download col");
+        SPG_HIP(hipMemcpyAsync(values, ctx->cval[c].as<double>() + base, (size_t)cnt * 8, hipMemcpyDeviceToHost,
+                               ctx->stream), "download val");
+        SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    }
+    for (size_t i = 0; i < n; ++i) row_ptr[i] -= base;
     return SPG_OK;
 }
 

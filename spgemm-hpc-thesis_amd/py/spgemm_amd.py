@@ -32,7 +32,7 @@ EXPORTED = [
     "spg_accumulator_create", "spg_accumulator_add", "spg_accumulator_finish", "spg_accumulator_copy",
     "spg_accumulator_destroy", "spg_matrix_read_mm", "spg_matrix_view", "spg_matrix_free",
     "spg_matrix_last_error", "spg_matrix_upload", "spg_device_matrix_view", "spg_device_matrix_free",
-    "spg_synchronize", "spg_set_path",
+    "spg_synchronize", "spg_set_path", "spg_multiply_rows", "spg_result_rows",
 ]
 
 PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE = 0, 1, 2
@@ -123,6 +123,10 @@ def lib():
         L.spg_device_matrix_view.argtypes = [vp, P(CsrView)]
         L.spg_device_matrix_free.argtypes = [vp]
         L.spg_synchronize.argtypes = [vp]
+        L.spg_set_path.argtypes = [vp, C.c_int]
+        L.spg_result_rows.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, C.c_int64, P(C.c_int64)]
+        L.spg_multiply_rows.argtypes = [vp, P(CsrView), P(CsrView), C.c_int32, P(HaloView), C.c_int32, C.c_int32,
+                                        P(C.c_int64)]
         _lib = L
     return _lib
 
@@ -282,14 +286,36 @@ class Engine:
     def synchronize(self):
         self._check(self._L.spg_synchronize(self._h))
 
-    def multiply_device(self, a: DeviceMatrix, b: DeviceMatrix) -> int:
-        """Product of HBM-resident operands; the result stays on the device."""
-        return self.begin(a.view(), b.view())
+    def multiply_device(self, a: DeviceMatrix, b: DeviceMatrix, rows: tuple[int, int] | None = None) -> int:
+        """Product of HBM-resident operands (or C rows [r0, r1) of it); the
+        result stays on the device."""
+        if rows is None:
+            return self.begin(a.view(), b.view())
+        n = C.c_int64()
+        av, bv = a.view(), b.view()
+        self._check(self._L.spg_multiply_rows(self._h, C.byref(av), C.byref(bv), 0, None, int(rows[0]),
+                                              int(rows[1]), C.byref(n)))
+        return int(n.value)
 
     def result_view(self) -> CsrView:
         v = CsrView()
         self._check(self._L.spg_result_device(self._h, C.byref(v)))
         return v
+
+    def result_rows(self, r0: int, r1: int) -> Csr:
+        """Rows [r0, r1) of the last result (row_ptr starting at 0)."""
+        rp = np.zeros(r1 - r0 + 1, dtype=np.int64)
+        n = C.c_int64()
+        s = self._L.spg_result_rows(self._h, r0, r1, rp.ctypes.data, None, None, 0, C.byref(n))
+        cnt = int(n.value)
+        if s != 0 and cnt == 0:
+            self._check(s)
+        col = np.zeros(max(cnt, 1), dtype=np.int32)
+        val = np.zeros(max(cnt, 1), dtype=np.float64)
+        self._check(self._L.spg_result_rows(self._h, r0, r1, rp.ctypes.data, col.ctypes.data, val.ctypes.data, cnt,
+                                            C.byref(n)))
+        v = self.result_view()
+        return Csr(r1 - r0, v.cols, rp, col[:cnt], val[:cnt])
 
     def result_host(self) -> Csr:
         v = self.result_view()
