@@ -16,6 +16,10 @@ Workloads (--workload):
                     output (~2.3e10 entries) exceeds device memory, so a step is
                     one sweep over row chunks (each chunk's C overwrites the last)
   rmat_s20_rect     the same R-MAT A times B 2^20 x 65536 with 24 uniform columns per row
+  batched_4096      4096 independent jobs (rows log-uniform in [256, 65536], 4..32 uniform
+                    columns per row; every third job a Galerkin triple R.A.P on a 27-point
+                    stencil with 2x2x2 aggregation) as one batched pipeline replayed from a
+                    HIP graph, against sequential plain launches of the same jobs
 
 value     = 2*P / step time (GFLOP/s), P = scalar products (the reference's flop basis).
 roofline  = HBM-bound: algorithmic bytes of the dominant kernel / its average
@@ -68,6 +72,9 @@ WORKLOADS = {
     "rmat_s20_AA": dict(kind="chunked", desc={"rmat_scale": 20, "edge_factor": 32, "abcd": [0.57, 0.19, 0.19, 0.05],
                                               "seed": 42, "product": "A.A"},
                         make=lambda: (lambda a: (a, a))(_rmat())),
+    "batched_4096": dict(kind="batched", desc={"jobs": 4096, "seed": 42, "rows": "log-uniform [256, 65536]",
+                                               "nnz_per_row": "uniform 4..32", "galerkin_fraction": "1/3 (n in 16..48)"},
+                         make=None),
     "rmat_s20_rect": dict(kind="single", desc={"rmat_scale": 20, "edge_factor": 32, "seed": 42,
                                                "B": "2^20 x 65536, 24 uniform columns per row, seed 43"},
                           make=lambda: (_rmat(), _uniform(1 << 20, 65536, 24, 43))),
@@ -158,6 +165,149 @@ def chunk_bounds(rowp, budget):
     return bounds
 
 
+def run_batched(args, wl):
+    """Batched jobs/s against sequential plain launches (same resident operands,
+    identical bit-exact results)."""
+    import oracle as O
+    import spgemm_amd as S
+    import workloads as W
+
+    njobs = args.jobs if args.jobs else wl["desc"]["jobs"]
+    procs = max(1, min(16, os.cpu_count() or 1))
+    log(f"bench: generating {njobs} jobs on {procs} processes")
+    jobs = W.batch_jobs(njobs, seed=wl["desc"]["seed"], processes=procs)
+    ab = [i for i, j in enumerate(jobs) if j[0] == "ab"]
+    rap = [i for i, j in enumerate(jobs) if j[0] == "rap"]
+    a1 = [jobs[i][1] for i in ab] + [jobs[i][1] for i in rap]
+    b1 = [jobs[i][2] for i in ab] + [jobs[i][2] for i in rap]
+    p_list = [jobs[i][3] for i in rap]
+    P1 = sum(W.products(a, b) for a, b in zip(a1, b1))
+    eng = S.Engine(0, timing=True)
+    log("bench: building batch arenas")
+    w1 = S.Batch(eng, a1, b1)
+    w1.run()
+    w2 = S.Batch.chained(eng, w1, [len(ab) + k for k in range(len(rap))], p_list)
+    w2.run()
+    log("bench: autotune")
+    c1 = w1.autotune(2)
+    c2 = w2.autotune(2)
+    w1.set_graph(True)
+    w2.set_graph(True)
+
+    def batch_step():
+        n1 = w1.run()
+        w2.refresh()
+        n2 = w2.run()
+        return n1 + n2
+
+    steps = args.steps if args.steps is not None else 5
+    warmup = args.warmup if args.warmup is not None else 2
+    for _ in range(warmup):
+        batch_step()
+    eng.synchronize()
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        nnz = batch_step()
+        times.append(time.perf_counter() - t0)
+    batch_s = statistics.median(times)
+
+    # sequential plain launches: one drop-in multiply per product on resident
+    # operands; the triples' second product takes the first one's device result
+    log("bench: sequential plain launches")
+    seq_eng = S.Engine(0)
+    ra_eng = S.Engine(0)
+    dev = [(seq_eng.upload(jobs[i][1]), seq_eng.upload(jobs[i][2]), seq_eng.upload(jobs[i][3]) if jobs[i][0] == "rap"
+            else None) for i in range(njobs)]
+    seq_eng.synchronize()
+
+    def seq_all():
+        for i in range(njobs):
+            da, db, dp = dev[i]
+            if dp is None:
+                seq_eng.multiply_device(da, db)
+            else:
+                ra_eng.multiply_device(da, db)
+                seq_eng.begin(ra_eng.result_view(), dp.view())
+    seq_all()  # warm
+    seq_times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        seq_all()
+        seq_times.append(time.perf_counter() - t0)
+    seq_s = min(seq_times)
+
+    # bit-exact: sampled jobs from the batch equal their standalone products
+    rng = np.random.Generator(np.random.PCG64(5))
+    check = sorted(rng.choice(njobs, size=min(64, njobs), replace=False).tolist())
+    exact = True
+    pos1 = {j: k for k, j in enumerate(ab + rap)}
+    pos2 = {j: k for k, j in enumerate(rap)}
+    for i in check:
+        da, db, dp = dev[i]
+        if dp is None:
+            seq_eng.multiply_device(da, db)
+            exact &= O.csr_digest(_o(seq_eng.result_host())) == O.csr_digest(_o(w1.result(pos1[i])))
+        else:
+            ra_eng.multiply_device(da, db)
+            seq_eng.begin(ra_eng.result_view(), dp.view())
+            exact &= O.csr_digest(_o(seq_eng.result_host())) == O.csr_digest(_o(w2.result(pos2[i])))
+    # and a few jobs against the serial-order oracle
+    for i in check[:8]:
+        kind = jobs[i][0]
+        a, b = jobs[i][1], jobs[i][2]
+        ref = O.product(_o(a), _o(b))
+        if kind == "ab":
+            exact &= O.csr_digest(ref) == O.csr_digest(_o(w1.result(pos1[i])))
+        else:
+            ref = O.product(ref, _o(jobs[i][3]))
+            exact &= O.csr_digest(ref) == O.csr_digest(_o(w2.result(pos2[i])))
+
+    P_total = P1 + sum(W.products(_o(w1.result(len(ab) + k)), p_list[k]) for k in range(len(rap)))
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline (C oracle, first 96 jobs)")
+        nsample = min(96, njobs)
+        t0 = time.perf_counter()
+        for i in range(nsample):
+            r = O.product(_o(jobs[i][1]), _o(jobs[i][2]), threads=procs)
+            if jobs[i][0] == "rap":
+                O.product(r, _o(jobs[i][3]), threads=procs)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(nsample / dt, 3), "unit": "jobs/s", "cores": procs, "kind": "port",
+               "sample": f"first {nsample} jobs, serial-order C restatement of spgemm() with OpenMP static rows"}
+    out = {
+        "metric": f"batched SpGEMM jobs/s ({njobs} independent jobs, 1/3 Galerkin R.A.P), one batched "
+                  "pipeline replayed from a HIP graph",
+        "value": round(njobs / batch_s, 2),
+        "unit": "jobs/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(batch_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded, generated on the host, uploaded once into resident batch arenas)",
+        "config": {"workload": "batched_4096", **wl["desc"], "jobs": njobs, "products": int(P_total), "c_nnz": int(nnz),
+                   "sequential_jobs_per_s": round(njobs / seq_s, 2),
+                   "speedup_vs_sequential": round(seq_s / batch_s, 3),
+                   "gflops": round(2.0 * P_total / batch_s / 1e9, 2),
+                   "autotune_choice": {"wave1": c1, "wave2": c2},
+                   "parity": ("bit-exact: 64 sampled jobs equal their standalone products, 8 equal the oracle"
+                              if exact else "MISMATCH")},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _o(m):
+    import oracle as O
+    return O.Csr.make(m.rows, m.cols, m.row_ptr, m.col, m.val)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,12 +315,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="uniform_2p20x16", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--jobs", type=int, default=0, help="batched workload: number of jobs (default 4096)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this engine computes one rank's local product; "
               "--gpus must be 1", file=sys.stderr)
         sys.exit(2)
     wl = WORKLOADS[args.workload]
+    if wl["kind"] == "batched":
+        run_batched(args, wl)
+        return
     chunked = wl["kind"] == "chunked"
     steps = args.steps if args.steps is not None else (1 if chunked else 20)
     warmup = args.warmup if args.warmup is not None else (0 if chunked else 3)

@@ -169,6 +169,41 @@ spg_status spg_device_matrix_free(spg_device_matrix *m);
 /* Wait for all work queued on the context's stream. */
 spg_status spg_synchronize(spg_context *ctx);
 
+/* ---- batched products ----------------------------------------------------
+ * No counterpart in the reference (each rank multiplies once); this is the
+ * batched mode of the engine.  A batch packs n independent products
+ * C_i = A_i * B_i into one resident arena (the B_i stacked, the A_i stacked
+ * with their columns shifted to their B_i's rows) and computes all of them
+ * with one device pipeline.  Each C_i is bit-identical to its standalone
+ * product.  The batch owns its result buffers and runs on the creating
+ * context's stream.  Operands are host views (uploaded once). */
+typedef struct spg_batch spg_batch;
+spg_status spg_batch_create(spg_context *ctx, int32_t n, const spg_csr_view *A, const spg_csr_view *B,
+                            spg_batch **out);
+/* Chained batch: A_i is the result of product src_index[i] of batch `src`
+ * (which must have run); the second product of (R.A).P.  spg_batch_refresh
+ * copies the source's current results into this batch's A operands on the
+ * device (the source's result structure must be unchanged). */
+spg_status spg_batch_create_chained(spg_context *ctx, const spg_batch *src, int32_t n, const int32_t *src_index,
+                                    const spg_csr_view *B, spg_batch **out);
+spg_status spg_batch_refresh(spg_batch *b);
+/* Run every product; *nnz_total receives the entries of all C_i. */
+spg_status spg_batch_run(spg_batch *b, int64_t *nnz_total);
+/* After a first run, replay the planned pipeline from a captured HIP graph
+ * (no host synchronisation inside the pipeline). */
+spg_status spg_batch_set_graph(spg_batch *b, int enabled);
+/* Time the pipeline variants (single pass with 2 or 1 rows per wave,
+ * two-phase) over `repeats` runs each and keep the fastest; *choice receives
+ * its index. */
+spg_status spg_batch_autotune(spg_batch *b, int repeats, int *choice);
+/* Results of the last run: entry count of product i, and a host copy of C_i
+ * (row_ptr: rows_i + 1 entries starting at 0). */
+spg_status spg_batch_product_nnz(const spg_batch *b, int32_t i, int64_t *nnz);
+spg_status spg_batch_copy(spg_batch *b, int32_t i, int64_t *row_ptr, int32_t *col_idx, double *values);
+spg_status spg_batch_info(const spg_batch *b, int32_t i, int32_t *rows, int32_t *cols);
+spg_status spg_batch_last_stats(const spg_batch *b, spg_stats *out);
+spg_status spg_batch_destroy(spg_batch *b);
+
 /* ---- staged accumulation (trident::LocalAccumulator) -------------------
  * Replaces trident::LocalAccumulator{LocalAccumulator(rows, cols);
  * addProduct(a, b); finish();} (src/trident/common/matrix_blocks.hpp:23-32,

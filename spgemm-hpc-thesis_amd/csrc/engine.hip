@@ -7,8 +7,7 @@
 //   distinct-column counts), scan -> C row_ptr      -> sync 1: errors, nnz(C)
 //   numeric phase                                   -> sync 2: stored entries
 //   compaction, only when exact zeros were dropped.
-#include "internal.hpp"
-#include "spgemm_amd.h"
+#include "engine_internal.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -20,75 +19,12 @@
 
 using namespace spg;
 
-namespace {
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    bool zero_on_grow = false;
-    hipError_t ensure(size_t want, hipStream_t stream) {
-        if (want <= bytes && p) return hipSuccess;
-        if (p) {
-            hipError_t e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) return e;
-            hipFree(p);
-            p = nullptr;
-            bytes = 0;
-        }
-        size_t n = want < 256 ? 256 : want;
-        n = (n + 255) & ~size_t(255);
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) {
-            p = nullptr;
-            (void)hipGetLastError();
-            return e;
-        }
-        bytes = n;
-        if (zero_on_grow) launch_zero_bytes(p, n, stream);
-        return hipSuccess;
-    }
-    void release() {
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <typename T>
-    T *as() const { return static_cast<T *>(p); }
-};
-
-}  // namespace
-
-struct spg_context {
-    int device = 0;
-    int cus = 256;
-    size_t total_mem = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::string err;
-    Control *h_ctrl = nullptr;  // pinned mirror
-    // uploaded operands (host inputs) and the halo directory
-    DevBuf a_ptr, a_col, a_val, b_ptr, b_col, b_val, dir_start, dir_end;
-    // workspace
-    DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc, tile_status, tile_distinct, part;
-    int path = SPG_PATH_AUTO;
-    int64_t lbits_width = 0;
-    // results (ping-pong for compaction)
-    DevBuf rowptr[2], ccol[2], cval[2];
-    int cur = 0;
-    bool have_result = false;
-    int32_t res_rows = 0, res_cols = 0;
-    int64_t res_nnz = 0;
-    spg_stats stats{};
-};
-
 struct spg_accumulator {
     spg_context *ctx = nullptr;
     int32_t rows = 0, cols = 0;
 };
 
-namespace {
+namespace spg {
 
 spg_status set_err(spg_context *c, spg_status s, const std::string &msg) {
     if (c) c->err = msg;
@@ -101,12 +37,6 @@ spg_status hip_err(spg_context *c, hipError_t e, const char *what) {
     if (e == hipErrorOutOfMemory) return set_err(c, SPG_ENOMEM, m);
     return set_err(c, SPG_EDEVICE, m);
 }
-
-#define SPG_HIP(call, what)                                  \
-    do {                                                     \
-        hipError_t e_ = (call);                              \
-        if (e_ != hipSuccess) return hip_err(ctx, e_, what); \
-    } while (0)
 
 spg_status check_host_csr(spg_context *ctx, const spg_csr_view *m, const char *name) {
     if (m->rows < 0 || m->cols < 0 || m->nnz < 0) return set_err(ctx, SPG_EINVAL, std::string(name) + ": negative dimension");
@@ -121,6 +51,222 @@ spg_status check_host_csr(spg_context *ctx, const spg_csr_view *m, const char *n
             return set_err(ctx, SPG_EINVAL, std::string(name) + ": row_ptr is not nondecreasing");
     return SPG_OK;
 }
+
+spg_status read_control(spg_context *ctx) {
+    SPG_HIP(hipMemcpyAsync(ctx->h_ctrl, ctx->ctrl.p, sizeof(Control), hipMemcpyDeviceToHost, ctx->stream),
+            "read control block");
+    SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return SPG_OK;
+}
+
+spg_status flag_error(spg_context *ctx, unsigned long long flags) {
+    if (flags & kErrLongRow) return set_err(ctx, SPG_EOVERFLOW, "a B row has more than INT32_MAX entries");
+    if (flags & kErrMissing)
+        return set_err(ctx, SPG_EMISSING_ROW, "missing a remote sparse row required by A");
+    if (flags & kErrRange)
+        return set_err(ctx, SPG_ERANGE, "A contains a column index outside B's row range");
+    if (flags & kErrInternal) return set_err(ctx, SPG_EDEVICE, "internal capacity invariant violated");
+    return SPG_OK;
+}
+
+spg_status plan_workspace(spg_context *ctx, ProductPlan &p) {
+    hipStream_t st = ctx->stream;
+    const int32_t rows = p.rows;
+    const size_t R = rows > 0 ? (size_t)rows : 1;
+    SPG_HIP(ctx->ub.ensure(R * 8, st), "hipMalloc(ub)");
+    const size_t E = p.op.nnz_a > 0 ? (size_t)p.op.nnz_a : 1;
+    SPG_HIP(ctx->ebst.ensure(E * 8, st), "hipMalloc(entry starts)");
+    SPG_HIP(ctx->elen.ensure(E * 4, st), "hipMalloc(entry lengths)");
+    SPG_HIP(ctx->bin_rows.ensure(R * sizeof(RowRec) * kNumBins, st), "hipMalloc(bin lists)");
+    SPG_HIP(ctx->snnz.ensure(R * 8, st), "hipMalloc(snnz)");
+    SPG_HIP(ctx->annz.ensure(R * 8, st), "hipMalloc(annz)");
+    SPG_HIP(ctx->scan_tmp.ensure(scan_tmp_elems(rows) * 8, st), "hipMalloc(scan)");
+    SPG_HIP(ctx->ctrl.ensure(sizeof(Control), st), "hipMalloc(ctrl)");
+    SPG_HIP(ctx->part.ensure((size_t)(analyze_blocks(rows) + 1) * kAnalyzeFields * 8, st), "hipMalloc(partials)");
+    SPG_HIP(ctx->rowptr[0].ensure(((size_t)rows + 1) * 8, st), "hipMalloc(row_ptr C)");
+
+    Workspace &ws = p.ws;
+    ws = Workspace{};
+    ws.ebst = ctx->ebst.as<int64_t>();
+    ws.elen = ctx->elen.as<int32_t>();
+    ws.ub = ctx->ub.as<int64_t>();
+    ws.bin_recs = ctx->bin_rows.as<RowRec>();
+    ws.list_stride = (int64_t)R;
+    ws.snnz = ctx->snnz.as<int64_t>();
+    ws.annz = ctx->annz.as<int64_t>();
+    ws.scan_tmp = ctx->scan_tmp.as<int64_t>();
+    ws.ctrl = ctx->ctrl.as<Control>();
+    ws.part = ctx->part.as<int64_t>();
+    // Large-row path: one column window and one presence bitmap per resident
+    // workgroup (kept all-zero between rows); the accumulators are allocated
+    // only when large rows exist.
+    int64_t width = ((int64_t)p.cols + 31) / 32 * 32;
+    const int64_t kMaxWindow = int64_t(1) << 22;
+    if (width > kMaxWindow) width = kMaxWindow;
+    if (width < 32) width = 32;
+    ws.lwidth = width;
+    const size_t per_block = (size_t)width * 8 + (size_t)width / 8;
+    size_t budget = ctx->total_mem / 16;
+    if (budget > (size_t(4) << 30)) budget = size_t(4) << 30;
+    size_t nb = budget / per_block;
+    const size_t cap = (size_t)ctx->cus * 2;
+    if (nb > cap) nb = cap;
+    if (nb < 1) nb = 1;
+    ws.lblocks = (int)nb;
+    ctx->lbits.zero_on_grow = true;
+    if (ctx->lbits_width != width) {  // bitmaps are laid out per window width
+        ctx->lbits.release();
+        ctx->lbits_width = width;
+    }
+    SPG_HIP(ctx->lbits.ensure(nb * (size_t)(width / 32) * 4, st), "hipMalloc(large bitmaps)");
+    ws.lbits = ctx->lbits.as<uint32_t>();
+    ws.lacc = ctx->lacc.as<double>();
+    return SPG_OK;
+}
+
+void enqueue_counts(spg_context *ctx, const ProductPlan &p, unsigned bins, bool scan) {
+    hipStream_t st = ctx->stream;
+    launch_zero_control(p.ws.ctrl, st);
+    launch_analyze(p.op, p.ws, st);
+    launch_symbolic(p.op, p.ws, bins, ctx->cus, st);
+    if (scan) launch_scan(p.ws.snnz, ctx->rowptr[0].as<int64_t>(), p.rows, p.ws.scan_tmp, &p.ws.ctrl->structural, st);
+}
+
+spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h) {
+    hipStream_t st = ctx->stream;
+    for (int b = 0; b < 8; ++b) p.bin_count[b] = h.bin_count[b];
+    p.capacity = p.fused ? (int64_t)(h.ub_small + h.sym_total) : (int64_t)h.structural;
+    SPG_HIP(ctx->ccol[0].ensure((size_t)p.capacity * 4, st), "hipMalloc(C col)");
+    SPG_HIP(ctx->cval[0].ensure((size_t)p.capacity * 8, st), "hipMalloc(C val)");
+    if (h.bin_count[kBinLarge] > 0) {
+        SPG_HIP(ctx->lacc.ensure((size_t)p.ws.lblocks * (size_t)p.ws.lwidth * 8, st), "hipMalloc(large accumulators)");
+        p.ws.lacc = ctx->lacc.as<double>();
+    }
+    p.fo = FusedOut{};
+    if (p.fused) {
+        p.fo.ntiles = fused_tiles(p.rows, p.fused_rows);
+        const size_t nt = (size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1);
+        SPG_HIP(ctx->tile_status.ensure(nt * 8, st), "hipMalloc(tile status)");
+        SPG_HIP(ctx->tile_distinct.ensure(nt * 8, st), "hipMalloc(tile counts)");
+        p.fo.rowptr = ctx->rowptr[0].as<int64_t>();
+        p.fo.ccol = ctx->ccol[0].as<int32_t>();
+        p.fo.cval = ctx->cval[0].as<double>();
+        p.fo.status = ctx->tile_status.as<uint64_t>();
+        p.fo.distinct = ctx->tile_distinct.as<int64_t>();
+    }
+    return SPG_OK;
+}
+
+void enqueue_values(spg_context *ctx, const ProductPlan &p) {
+    hipStream_t st = ctx->stream;
+    if (p.fused) {
+        launch_zero_bytes(p.fo.status, (size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1) * 8, st);
+        if (p.rows == 0) launch_zero_bytes(p.fo.rowptr, 8, st);
+        launch_fused(p.op, p.ws, p.fo, p.small_b, p.fused_rows, st);
+    }
+    launch_numeric(p.op, p.ws, p.bin_count, p.fused ? kBigBins : kAllBins, ctx->cus, ctx->rowptr[0].as<int64_t>(),
+                   ctx->ccol[0].as<int32_t>(), ctx->cval[0].as<double>(), st);
+    if (p.fused && p.fo.ntiles > 0) launch_sum(p.fo.distinct, p.fo.ntiles, &p.ws.ctrl->distinct, st);
+}
+
+spg_status finish_product(spg_context *ctx, const ProductPlan &p, const Control &h, spg_stats &stats) {
+    hipStream_t st = ctx->stream;
+    spg_status s;
+    if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
+    // Entries of the row layout: structural counts (two-phase path), or the
+    // single-pass rows' stored counts plus the big rows' structural counts.
+    // The numeric kernels report the entries they store (ctrl.actual); the
+    // single-pass rows' stored total is the layout minus the big rows' share.
+    const int64_t layout = p.fused ? (int64_t)h.reserved : (int64_t)h.structural;
+    stats.products = (int64_t)h.products;
+    for (int b = 0; b < kNumBins; ++b) stats.rows_per_bin[b] = h.bin_rows[b];
+    stats.bins = kNumBins;
+    stats.structural_nnz = p.fused ? (int64_t)(h.distinct + h.sym_total) : (int64_t)h.structural;
+    stats.single_pass = p.fused ? 1 : 0;
+    const int64_t nnz = p.fused ? (int64_t)(h.reserved - h.sym_total + h.actual) : (int64_t)h.actual;
+    int result = 0;
+    if (nnz != layout) {
+        // Exact zeros were dropped: pack rows to their final offsets.
+        const int o2 = 1;
+        SPG_HIP(ctx->rowptr[o2].ensure(((size_t)p.rows + 1) * 8, st), "hipMalloc(row_ptr C2)");
+        SPG_HIP(ctx->ccol[o2].ensure((size_t)nnz * 4, st), "hipMalloc(C2 col)");
+        SPG_HIP(ctx->cval[o2].ensure((size_t)nnz * 8, st), "hipMalloc(C2 val)");
+        launch_scan(p.ws.annz, ctx->rowptr[o2].as<int64_t>(), p.rows, p.ws.scan_tmp, nullptr, st);
+        launch_compact(p.rows, ctx->rowptr[0].as<int64_t>(), ctx->ccol[0].as<int32_t>(), ctx->cval[0].as<double>(),
+                       ctx->rowptr[o2].as<int64_t>(), ctx->ccol[o2].as<int32_t>(), ctx->cval[o2].as<double>(), st);
+        SPG_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+        result = o2;
+        stats.compacted = 1;
+    }
+    SPG_HIP(hipGetLastError(), "kernel launch");
+    stats.nnz = nnz;
+    ctx->cur = result;
+    ctx->res_rows = p.rows;
+    ctx->res_cols = p.cols;
+    ctx->res_nnz = nnz;
+    ctx->have_result = true;
+    ctx->stats = stats;
+    return SPG_OK;
+}
+
+// The whole product on the device.  On success the result is
+// rowptr[cur], ccol[cur], cval[cur] with res_nnz entries.
+spg_status run_product(spg_context *ctx, ProductPlan &p) {
+    hipStream_t st = ctx->stream;
+    ctx->have_result = false;
+    spg_stats stats{};
+    spg_status s;
+    p.fused_rows = ctx->fused_rows;
+    if ((s = plan_workspace(ctx, p)) != SPG_OK) return s;
+    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+    // Single-pass path: only the rows of the big bins are listed and counted
+    // ahead; the rows of bins W1, W2 are computed in one pass by the fused kernel.
+    const bool try_fused = ctx->path != SPG_PATH_TWO_PHASE;
+    p.ws.list_small = try_fused ? 0 : 1;
+    enqueue_counts(ctx, p, try_fused ? kBigBins : kAllBins, !try_fused);
+    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
+    if ((s = read_control(ctx)) != SPG_OK) return s;
+    Control h = *ctx->h_ctrl;
+    if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
+    // C capacity of the single-pass path: the small rows' work bounds plus the
+    // big rows' exact counts.  Taken when it fits half of the device memory
+    // (always when the caller asked for it).
+    p.fused = false;
+    if (try_fused) {
+        const uint64_t fused_cap = h.ub_small + h.sym_total;
+        p.fused = ctx->path == SPG_PATH_SINGLE_PASS || fused_cap * 12ull <= (uint64_t)(ctx->total_mem / 2);
+        if (!p.fused) {
+            // Two-phase after all: list the small rows too (analyze again) and
+            // count them.
+            p.ws.list_small = 1;
+            enqueue_counts(ctx, p, kAllBins, true);
+            if ((s = read_control(ctx)) != SPG_OK) return s;
+            h = *ctx->h_ctrl;
+            if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
+        }
+    }
+    if ((s = plan_values(ctx, p, h)) != SPG_OK) return s;
+    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");
+    enqueue_values(ctx, p);
+    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[3], st), "hipEventRecord");
+    if ((s = read_control(ctx)) != SPG_OK) return s;
+    h = *ctx->h_ctrl;
+    if ((s = finish_product(ctx, p, h, stats)) != SPG_OK) return s;
+    if (ctx->timing) {
+        float t = 0;
+        SPG_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[3]), "hipEventElapsedTime");
+        ctx->stats.ms_total = t;
+        SPG_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
+        ctx->stats.ms_symbolic = t;
+        SPG_HIP(hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
+        ctx->stats.ms_numeric = t;
+    }
+    return SPG_OK;
+}
+
+}  // namespace spg
+
+namespace {
 
 // Device pointers of a view, uploading host data into the given buffers.
 spg_status stage(spg_context *ctx, const spg_csr_view *m, DevBuf &p, DevBuf &c, DevBuf &v, const int64_t *&dp,
@@ -143,190 +289,6 @@ spg_status stage(spg_context *ctx, const spg_csr_view *m, DevBuf &p, DevBuf &c, 
     dp = p.as<int64_t>();
     dc = c.as<int32_t>();
     dv = v.as<double>();
-    return SPG_OK;
-}
-
-spg_status read_control(spg_context *ctx) {
-    SPG_HIP(hipMemcpyAsync(ctx->h_ctrl, ctx->ctrl.p, sizeof(Control), hipMemcpyDeviceToHost, ctx->stream),
-            "read control block");
-    SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    return SPG_OK;
-}
-
-spg_status flag_error(spg_context *ctx, unsigned long long flags) {
-    if (flags & kErrLongRow) return set_err(ctx, SPG_EOVERFLOW, "a B row has more than INT32_MAX entries");
-    if (flags & kErrMissing)
-        return set_err(ctx, SPG_EMISSING_ROW, "missing a remote sparse row required by A");
-    if (flags & kErrRange)
-        return set_err(ctx, SPG_ERANGE, "A contains a column index outside B's row range");
-    if (flags & kErrInternal) return set_err(ctx, SPG_EDEVICE, "internal capacity invariant violated");
-    return SPG_OK;
-}
-
-// The whole product on the device.  On success the result is
-// rowptr[cur], ccol[cur], cval[cur] with res_nnz entries.
-spg_status run_product(spg_context *ctx, Operands op, int32_t rows, int32_t cols, bool small_b) {
-    hipStream_t st = ctx->stream;
-    ctx->have_result = false;
-    spg_stats stats{};
-    const size_t R = rows > 0 ? (size_t)rows : 1;
-    SPG_HIP(ctx->ub.ensure(R * 8, st), "hipMalloc(ub)");
-    const size_t E = op.nnz_a > 0 ? (size_t)op.nnz_a : 1;
-    SPG_HIP(ctx->ebst.ensure(E * 8, st), "hipMalloc(entry starts)");
-    SPG_HIP(ctx->elen.ensure(E * 4, st), "hipMalloc(entry lengths)");
-    SPG_HIP(ctx->bin_rows.ensure(R * sizeof(RowRec) * kNumBins, st), "hipMalloc(bin lists)");
-    SPG_HIP(ctx->snnz.ensure(R * 8, st), "hipMalloc(snnz)");
-    SPG_HIP(ctx->annz.ensure(R * 8, st), "hipMalloc(annz)");
-    SPG_HIP(ctx->scan_tmp.ensure(scan_tmp_elems(rows) * 8, st), "hipMalloc(scan)");
-    SPG_HIP(ctx->ctrl.ensure(sizeof(Control), st), "hipMalloc(ctrl)");
-    SPG_HIP(ctx->part.ensure((size_t)(analyze_blocks(rows) + 1) * kAnalyzeFields * 8, st), "hipMalloc(partials)");
-    const int out = 0;
-    SPG_HIP(ctx->rowptr[out].ensure(((size_t)rows + 1) * 8, st), "hipMalloc(row_ptr C)");
-
-    Workspace ws{};
-    ws.ebst = ctx->ebst.as<int64_t>();
-    ws.elen = ctx->elen.as<int32_t>();
-    ws.ub = ctx->ub.as<int64_t>();
-    ws.bin_recs = ctx->bin_rows.as<RowRec>();
-    ws.list_stride = (int64_t)R;
-    ws.snnz = ctx->snnz.as<int64_t>();
-    ws.annz = ctx->annz.as<int64_t>();
-    ws.scan_tmp = ctx->scan_tmp.as<int64_t>();
-    ws.ctrl = ctx->ctrl.as<Control>();
-    ws.part = ctx->part.as<int64_t>();
-    // Large-row path: one column window and one presence bitmap per resident
-    // workgroup (kept all-zero between rows); the accumulators are allocated
-    // only when large rows exist.
-    int64_t width = ((int64_t)cols + 31) / 32 * 32;
-    const int64_t kMaxWindow = int64_t(1) << 22;
-    if (width > kMaxWindow) width = kMaxWindow;
-    if (width < 32) width = 32;
-    ws.lwidth = width;
-    {
-        const size_t per_block = (size_t)width * 8 + (size_t)width / 8;
-        size_t budget = ctx->total_mem / 16;
-        if (budget > (size_t(4) << 30)) budget = size_t(4) << 30;
-        size_t nb = budget / per_block;
-        const size_t cap = (size_t)ctx->cus * 2;
-        if (nb > cap) nb = cap;
-        if (nb < 1) nb = 1;
-        ws.lblocks = (int)nb;
-        ctx->lbits.zero_on_grow = true;
-        if (ctx->lbits_width != width) {  // bitmaps are laid out per window width
-            ctx->lbits.release();
-            ctx->lbits_width = width;
-        }
-        SPG_HIP(ctx->lbits.ensure(nb * (size_t)(width / 32) * 4, st), "hipMalloc(large bitmaps)");
-        ws.lbits = ctx->lbits.as<uint32_t>();
-    }
-
-    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
-    // Single-pass path: only the rows of the big bins are listed and counted
-    // ahead; the rows of bins W1, W2 are computed in one pass by the fused kernel.
-    const bool try_fused = ctx->path != SPG_PATH_TWO_PHASE;
-    ws.list_small = try_fused ? 0 : 1;
-    launch_zero_control(ws.ctrl, st);
-    launch_analyze(op, ws, st);
-    launch_symbolic(op, ws, try_fused ? kBigBins : kAllBins, ctx->cus, st);
-    if (!try_fused) launch_scan(ws.snnz, ctx->rowptr[out].as<int64_t>(), rows, ws.scan_tmp, &ws.ctrl->structural, st);
-    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
-    spg_status s = read_control(ctx);
-    if (s != SPG_OK) return s;
-    Control h = *ctx->h_ctrl;
-    if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
-    stats.products = (int64_t)h.products;
-    for (int b = 0; b < kNumBins; ++b) stats.rows_per_bin[b] = h.bin_rows[b];
-    stats.bins = kNumBins;
-
-    // C capacity of the single-pass path: the small rows' work bounds plus the
-    // big rows' exact counts.  Taken when it fits half of the device memory
-    // (always when the caller asked for it and it can be allocated).
-    const uint64_t fused_cap = h.ub_small + h.sym_total;
-    bool fused = false;
-    if (try_fused) {
-        fused = ctx->path == SPG_PATH_SINGLE_PASS || fused_cap * 12ull <= (uint64_t)(ctx->total_mem / 2);
-        if (!fused) {
-            // Two-phase after all: list the small rows too (analyze again) and
-            // count them.
-            ws.list_small = 1;
-            launch_zero_control(ws.ctrl, st);
-            launch_analyze(op, ws, st);
-            launch_symbolic(op, ws, kAllBins, ctx->cus, st);
-            launch_scan(ws.snnz, ctx->rowptr[out].as<int64_t>(), rows, ws.scan_tmp, &ws.ctrl->structural, st);
-            if ((s = read_control(ctx)) != SPG_OK) return s;
-            h = *ctx->h_ctrl;
-            if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
-        }
-    }
-    const int64_t nrows_cap = fused ? (int64_t)fused_cap : (int64_t)h.structural;
-    SPG_HIP(ctx->ccol[out].ensure((size_t)nrows_cap * 4, st), "hipMalloc(C col)");
-    SPG_HIP(ctx->cval[out].ensure((size_t)nrows_cap * 8, st), "hipMalloc(C val)");
-    if (h.bin_count[kBinLarge] > 0) {
-        SPG_HIP(ctx->lacc.ensure((size_t)ws.lblocks * (size_t)width * 8, st), "hipMalloc(large accumulators)");
-        ws.lacc = ctx->lacc.as<double>();
-    }
-    FusedOut fo{};
-    if (fused) {
-        fo.ntiles = fused_tiles(rows);
-        SPG_HIP(ctx->tile_status.ensure((size_t)(fo.ntiles > 0 ? fo.ntiles : 1) * 8, st), "hipMalloc(tile status)");
-        SPG_HIP(ctx->tile_distinct.ensure((size_t)(fo.ntiles > 0 ? fo.ntiles : 1) * 8, st), "hipMalloc(tile counts)");
-        launch_zero_bytes(ctx->tile_status.p, (size_t)(fo.ntiles > 0 ? fo.ntiles : 1) * 8, st);
-        if (rows == 0) launch_zero_bytes(ctx->rowptr[out].p, 8, st);
-        fo.rowptr = ctx->rowptr[out].as<int64_t>();
-        fo.ccol = ctx->ccol[out].as<int32_t>();
-        fo.cval = ctx->cval[out].as<double>();
-        fo.status = ctx->tile_status.as<uint64_t>();
-        fo.distinct = ctx->tile_distinct.as<int64_t>();
-    }
-    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");
-    if (fused) launch_fused(op, ws, fo, small_b, ctx->cus, st);
-    launch_numeric(op, ws, h.bin_count, fused ? kBigBins : kAllBins, ctx->cus, ctx->rowptr[out].as<int64_t>(),
-                   ctx->ccol[out].as<int32_t>(), ctx->cval[out].as<double>(), st);
-    if (fused && fo.ntiles > 0) launch_sum(fo.distinct, fo.ntiles, &ws.ctrl->distinct, st);
-    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[3], st), "hipEventRecord");
-    if ((s = read_control(ctx)) != SPG_OK) return s;
-    h = *ctx->h_ctrl;
-    if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
-    // Entries of the row layout: structural counts (two-phase path), or the
-    // single-pass rows' stored counts plus the big rows' structural counts.
-    // The numeric kernels report the entries they store (ctrl.actual); the
-    // single-pass rows' stored total is the layout minus the big rows' share.
-    const int64_t layout = fused ? (int64_t)h.reserved : (int64_t)h.structural;
-    stats.structural_nnz = fused ? (int64_t)(h.distinct + h.sym_total) : (int64_t)h.structural;
-    stats.single_pass = fused ? 1 : 0;
-    int64_t nnz = fused ? (int64_t)(h.reserved - h.sym_total + h.actual) : (int64_t)h.actual;
-    int result = out;
-    if (nnz != layout) {
-        // Exact zeros were dropped: pack rows to their final offsets.
-        const int o2 = 1;
-        SPG_HIP(ctx->rowptr[o2].ensure(((size_t)rows + 1) * 8, st), "hipMalloc(row_ptr C2)");
-        SPG_HIP(ctx->ccol[o2].ensure((size_t)nnz * 4, st), "hipMalloc(C2 col)");
-        SPG_HIP(ctx->cval[o2].ensure((size_t)nnz * 8, st), "hipMalloc(C2 val)");
-        launch_scan(ws.annz, ctx->rowptr[o2].as<int64_t>(), rows, ws.scan_tmp, nullptr, st);
-        launch_compact(rows, ctx->rowptr[out].as<int64_t>(), ctx->ccol[out].as<int32_t>(),
-                       ctx->cval[out].as<double>(), ctx->rowptr[o2].as<int64_t>(), ctx->ccol[o2].as<int32_t>(),
-                       ctx->cval[o2].as<double>(), st);
-        SPG_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
-        result = o2;
-        stats.compacted = 1;
-    }
-    SPG_HIP(hipGetLastError(), "kernel launch");
-    stats.nnz = nnz;
-    if (ctx->timing) {
-        float t = 0;
-        SPG_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[3]), "hipEventElapsedTime");
-        stats.ms_total = t;
-        SPG_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
-        stats.ms_symbolic = t;
-        SPG_HIP(hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
-        stats.ms_numeric = t;
-    }
-    ctx->cur = result;
-    ctx->res_rows = rows;
-    ctx->res_cols = cols;
-    ctx->res_nnz = nnz;
-    ctx->have_result = true;
-    ctx->stats = stats;
     return SPG_OK;
 }
 
@@ -520,7 +482,12 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
     // Every B entry index i reachable through the directory has 8*i < 2^32:
     // the single-pass kernel then addresses B with 32-bit offsets.
     const bool small_b = b_span < (int64_t(1) << 29);
-    if ((s = run_product(ctx, op, op.arows, B->cols, small_b)) != SPG_OK) return s;
+    ProductPlan plan;
+    plan.op = op;
+    plan.rows = op.arows;
+    plan.cols = B->cols;
+    plan.small_b = small_b;
+    if ((s = run_product(ctx, plan)) != SPG_OK) return s;
     *nnz_c = ctx->res_nnz;
     return SPG_OK;
 }

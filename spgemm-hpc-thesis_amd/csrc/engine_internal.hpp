@@ -1,0 +1,119 @@
+// engine_internal.hpp -- the context object and the product pipeline shared by
+// the drop-in entry points (engine.hip) and the batched ones (batch.hip).
+#pragma once
+
+#include "internal.hpp"
+#include "spgemm_amd.h"
+
+#include <string>
+
+namespace spg {
+
+// Growth-only device buffer (no allocation once a context is warm).
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    bool zero_on_grow = false;
+    hipError_t ensure(size_t want, hipStream_t stream) {
+        if (want <= bytes && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return e;
+            hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t n = want < 256 ? 256 : want;
+        n = (n + 255) & ~size_t(255);
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return e;
+        }
+        bytes = n;
+        if (zero_on_grow) launch_zero_bytes(p, n, stream);
+        return hipSuccess;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Everything one product needs on the device, fixed once its counts are
+// known.  A plan can be replayed (eagerly or from a captured HIP graph) for
+// operands of the same structure without host synchronisation.
+struct ProductPlan {
+    Operands op{};
+    Workspace ws{};
+    int32_t rows = 0, cols = 0;
+    bool small_b = false;
+    bool fused = false;  // single-pass kernel for bins W1, W2
+    int fused_rows = kFusedRows;
+    int bin_count[8] = {};
+    int64_t capacity = 0;  // C entries allocated
+    FusedOut fo{};
+};
+
+}  // namespace spg
+
+struct spg_context {
+    int device = 0;
+    int cus = 256;
+    size_t total_mem = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    spg::Control *h_ctrl = nullptr;  // pinned mirror
+    // uploaded operands (host inputs) and the halo directory
+    spg::DevBuf a_ptr, a_col, a_val, b_ptr, b_col, b_val, dir_start, dir_end;
+    // workspace
+    spg::DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc, tile_status, tile_distinct, part;
+    int path = SPG_PATH_AUTO;
+    int fused_rows = spg::kFusedRows;
+    int64_t lbits_width = 0;
+    // results (ping-pong for compaction)
+    spg::DevBuf rowptr[2], ccol[2], cval[2];
+    int cur = 0;
+    bool have_result = false;
+    int32_t res_rows = 0, res_cols = 0;
+    int64_t res_nnz = 0;
+    spg_stats stats{};
+};
+
+namespace spg {
+
+spg_status set_err(spg_context *c, spg_status s, const std::string &msg);
+spg_status hip_err(spg_context *c, hipError_t e, const char *what);
+spg_status check_host_csr(spg_context *ctx, const spg_csr_view *m, const char *name);
+spg_status read_control(spg_context *ctx);
+spg_status flag_error(spg_context *ctx, unsigned long long flags);
+
+#define SPG_HIP(call, what)                                       \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return spg::hip_err(ctx, e_, what); \
+    } while (0)
+
+// Workspace buffers for p.op / p.rows / p.cols; fills p.ws.
+spg_status plan_workspace(spg_context *ctx, ProductPlan &p);
+// Zero the control block, analyze, symbolic kernels of `bins` (and the scan
+// of all structural counts into C's row offsets when `scan`).
+void enqueue_counts(spg_context *ctx, const ProductPlan &p, unsigned bins, bool scan);
+// C buffers and the single-pass outputs for a plan whose counts are in h.
+spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h);
+// Single-pass kernel and numeric kernels of the plan.
+void enqueue_values(spg_context *ctx, const ProductPlan &p);
+// Compaction when exact zeros left gaps; publishes the result on the context.
+spg_status finish_product(spg_context *ctx, const ProductPlan &p, const Control &h, spg_stats &stats);
+// The whole product with its two host synchronisations; leaves the plan
+// filled for replays.
+spg_status run_product(spg_context *ctx, ProductPlan &p);
+
+}  // namespace spg

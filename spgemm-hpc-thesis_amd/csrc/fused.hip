@@ -408,10 +408,9 @@ __device__ __forceinline__ int64_t lane64(int64_t v, int k) {
                      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k));
 }
 
-template <bool SMALLB>
+template <bool SMALLB, int R>
 __global__ __launch_bounds__(kFusedTpb) void k_fused(Operands op, Workspace ws, FusedOut fo) {
     constexpr int NW = kFusedTpb / 64;
-    constexpr int R = kFusedRows;
     __shared__ FusedWave<SMALLB> s_w[NW];
     __shared__ int64_t s_wtot[NW];
     __shared__ int64_t s_base;
@@ -566,20 +565,22 @@ __global__ __launch_bounds__(kFusedTpb) void k_fused(Operands op, Workspace ws, 
 
 }  // namespace
 
-int64_t fused_tiles(int32_t rows) {
-    constexpr int64_t tile = (int64_t)(kFusedTpb / 64) * kFusedRows;
+int64_t fused_tiles(int32_t rows, int rows_per_wave) {
+    const int64_t tile = (int64_t)(kFusedTpb / 64) * rows_per_wave;
     return ((int64_t)rows + tile - 1) / tile;
 }
 
-void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b, int cus,
+void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b, int rows_per_wave,
                   hipStream_t stream) {
     if (op.arows <= 0) return;
-    (void)cus;
-    const dim3 grid((unsigned)fo.ntiles);
-    if (small_b)
-        hipLaunchKernelGGL(k_fused<true>, grid, dim3(kFusedTpb), 0, stream, op, ws, fo);
-    else
-        hipLaunchKernelGGL(k_fused<false>, grid, dim3(kFusedTpb), 0, stream, op, ws, fo);
+    const dim3 grid((unsigned)fo.ntiles), block(kFusedTpb);
+    if (rows_per_wave == 1) {
+        if (small_b) hipLaunchKernelGGL((k_fused<true, 1>), grid, block, 0, stream, op, ws, fo);
+        else hipLaunchKernelGGL((k_fused<false, 1>), grid, block, 0, stream, op, ws, fo);
+    } else {
+        if (small_b) hipLaunchKernelGGL((k_fused<true, 2>), grid, block, 0, stream, op, ws, fo);
+        else hipLaunchKernelGGL((k_fused<false, 2>), grid, block, 0, stream, op, ws, fo);
+    }
 }
 
 }  // namespace spg

@@ -31,7 +31,7 @@ constexpr int kTpbLarge = 256;
 // (kFusedTpb / 64) * kFusedRows rows (these are exactly the rows of bins W1, W2).
 constexpr int kFusedCap = kCapW2;
 constexpr int kFusedTpb = 256;
-constexpr int kFusedRows = 2;
+constexpr int kFusedRows = 2;  // default rows per wave of a tile (1 or 2)
 
 // Error flags raised by kernels (OR-ed into Control::flags).
 enum : unsigned {
@@ -131,8 +131,9 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
 // Single-pass kernel over all rows (fused.hip): computes the rows of bins W1,
 // W2 and reserves the symbolic counts of the others.  small_b: every B entry
 // index i has 8*i < 2^32.
-int64_t fused_tiles(int32_t rows);
-void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b, int cus,
+// rows_per_wave: 1 or 2 (rows of a tile per wave; the tile has kFusedTpb/64 waves).
+int64_t fused_tiles(int32_t rows, int rows_per_wave);
+void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b, int rows_per_wave,
                   hipStream_t stream);
 // Exclusive scan of n int64 counts into out[0..n]; out[n] = total; total also
 // stored to *total_dst when non-null (scan.hip).

@@ -33,6 +33,9 @@ EXPORTED = [
     "spg_accumulator_destroy", "spg_matrix_read_mm", "spg_matrix_view", "spg_matrix_free",
     "spg_matrix_last_error", "spg_matrix_upload", "spg_device_matrix_view", "spg_device_matrix_free",
     "spg_synchronize", "spg_set_path", "spg_multiply_rows", "spg_result_rows",
+    "spg_batch_create", "spg_batch_create_chained", "spg_batch_refresh", "spg_batch_run", "spg_batch_set_graph",
+    "spg_batch_autotune", "spg_batch_product_nnz", "spg_batch_copy", "spg_batch_info", "spg_batch_last_stats",
+    "spg_batch_destroy",
 ]
 
 PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE = 0, 1, 2
@@ -125,6 +128,17 @@ def lib():
         L.spg_synchronize.argtypes = [vp]
         L.spg_set_path.argtypes = [vp, C.c_int]
         L.spg_result_rows.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, C.c_int64, P(C.c_int64)]
+        L.spg_batch_create.argtypes = [vp, C.c_int32, P(CsrView), P(CsrView), P(vp)]
+        L.spg_batch_create_chained.argtypes = [vp, vp, C.c_int32, P(C.c_int32), P(CsrView), P(vp)]
+        L.spg_batch_refresh.argtypes = [vp]
+        L.spg_batch_run.argtypes = [vp, P(C.c_int64)]
+        L.spg_batch_set_graph.argtypes = [vp, C.c_int]
+        L.spg_batch_autotune.argtypes = [vp, C.c_int, P(C.c_int)]
+        L.spg_batch_product_nnz.argtypes = [vp, C.c_int32, P(C.c_int64)]
+        L.spg_batch_copy.argtypes = [vp, C.c_int32, vp, vp, vp]
+        L.spg_batch_info.argtypes = [vp, C.c_int32, P(C.c_int32), P(C.c_int32)]
+        L.spg_batch_last_stats.argtypes = [vp, P(Stats)]
+        L.spg_batch_destroy.argtypes = [vp]
         L.spg_multiply_rows.argtypes = [vp, P(CsrView), P(CsrView), C.c_int32, P(HaloView), C.c_int32, C.c_int32,
                                         P(C.c_int64)]
         _lib = L
@@ -320,3 +334,88 @@ class Engine:
     def result_host(self) -> Csr:
         v = self.result_view()
         return self.finish(v.rows, v.cols, v.nnz)
+
+
+class Batch:
+    """Many independent products computed by one device pipeline
+    (spg_batch_*).  Each result is bit-identical to the standalone product."""
+
+    def __init__(self, engine: "Engine", a_list=None, b_list=None, _handle=None, _keep=None):
+        self._e = engine
+        self._L = engine._L
+        if _handle is not None:
+            self._h = _handle
+            self._keep = _keep
+            return
+        n = len(a_list)
+        av = (CsrView * max(n, 1))()
+        bv = (CsrView * max(n, 1))()
+        for i, (a, b) in enumerate(zip(a_list, b_list)):
+            av[i] = host_view(a)
+            bv[i] = host_view(b)
+        h = C.c_void_p()
+        engine._check(self._L.spg_batch_create(engine._h, n, av, bv, C.byref(h)))
+        self._h = h
+        self._keep = None
+        self.n = n
+
+    @classmethod
+    def chained(cls, engine: "Engine", src: "Batch", src_index, b_list) -> "Batch":
+        """A_i = result src_index[i] of batch src (which must have run)."""
+        n = len(b_list)
+        idx = (C.c_int32 * max(n, 1))(*[int(x) for x in src_index])
+        bv = (CsrView * max(n, 1))()
+        for i, b in enumerate(b_list):
+            bv[i] = host_view(b)
+        h = C.c_void_p()
+        engine._check(engine._L.spg_batch_create_chained(engine._h, src._h, n, idx, bv, C.byref(h)))
+        out = cls(engine, _handle=h, _keep=[src])
+        out.n = n
+        return out
+
+    def refresh(self):
+        self._e._check(self._L.spg_batch_refresh(self._h))
+
+    def run(self) -> int:
+        n = C.c_int64()
+        self._e._check(self._L.spg_batch_run(self._h, C.byref(n)))
+        return int(n.value)
+
+    def set_graph(self, on: bool):
+        self._e._check(self._L.spg_batch_set_graph(self._h, 1 if on else 0))
+
+    def autotune(self, repeats: int = 2) -> int:
+        c = C.c_int()
+        self._e._check(self._L.spg_batch_autotune(self._h, repeats, C.byref(c)))
+        return int(c.value)
+
+    def product_nnz(self, i: int) -> int:
+        n = C.c_int64()
+        self._e._check(self._L.spg_batch_product_nnz(self._h, i, C.byref(n)))
+        return int(n.value)
+
+    def result(self, i: int) -> Csr:
+        r, c = C.c_int32(), C.c_int32()
+        self._e._check(self._L.spg_batch_info(self._h, i, C.byref(r), C.byref(c)))
+        nnz = self.product_nnz(i)
+        rp = np.zeros(r.value + 1, dtype=np.int64)
+        col = np.zeros(max(nnz, 1), dtype=np.int32)
+        val = np.zeros(max(nnz, 1), dtype=np.float64)
+        self._e._check(self._L.spg_batch_copy(self._h, i, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return Csr(int(r.value), int(c.value), rp, col[:nnz], val[:nnz])
+
+    def stats(self) -> dict:
+        st = Stats()
+        self._e._check(self._L.spg_batch_last_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.spg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

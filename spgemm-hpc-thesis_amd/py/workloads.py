@@ -146,3 +146,52 @@ def streaming_bytes(a: Csr, b_rows: int, p: int, c_nnz: int) -> int:
 def compulsory_bytes(a: Csr, b: Csr, c_nnz: int) -> int:
     """A, B and C each moved once."""
     return (8 * (a.rows + 1) + 12 * a.nnz) + (8 * (b.rows + 1) + 12 * b.nnz) + (8 * (a.rows + 1) + 12 * c_nnz)
+
+
+def variable_uniform(rows: int, cols: int, kmin: int, kmax: int, rng: np.random.Generator) -> Csr:
+    """Rows with k ~ U{kmin..kmax} distinct uniform columns each (ascending),
+    values U[-1, 1): a uniform kmax-subset per row, then a uniform k-subset of it."""
+    K = min(kmax, cols)
+    c = rng.integers(0, cols, size=(rows, K), dtype=np.int64)
+    c.sort(axis=1)
+    while K > 1:
+        bad = np.nonzero((np.diff(c, axis=1) == 0).any(axis=1))[0]
+        if len(bad) == 0:
+            break
+        redo = rng.integers(0, cols, size=(len(bad), K), dtype=np.int64)
+        redo.sort(axis=1)
+        c[bad] = redo
+    k = np.minimum(rng.integers(kmin, kmax + 1, size=rows), K)
+    order = np.argsort(rng.random((rows, K)), axis=1)
+    rank = np.empty_like(order)
+    np.put_along_axis(rank, order, np.arange(K)[None, :].repeat(rows, axis=0), axis=1)
+    keep = rank < k[:, None]
+    rp = np.zeros(rows + 1, dtype=np.int64)
+    np.cumsum(k, out=rp[1:])
+    return Csr.make(rows, cols, rp, c[keep].astype(np.int32), 2.0 * rng.random(int(rp[-1])) - 1.0)
+
+
+def batch_job(i: int, seed: int = 42, min_rows: int = 256, max_rows: int = 65536, n_lo: int = 16, n_hi: int = 48):
+    """Job i of the batched workload (its own random stream, so jobs can be
+    generated in parallel or alone).  Job i with i % 3 == 2 is a Galerkin
+    triple ("rap", R, A, P): A the 27-point stencil on an m^3 grid (m uniform
+    in n_lo..n_hi), P the 2x2x2 aggregation, R = P^T.  Every other job is a
+    plain product ("ab", A, B) with rows = cols log-uniform in
+    [min_rows, max_rows] and U{4..32} uniform columns per row."""
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, i])))
+    if i % 3 == 2:
+        m = int(rng.integers(n_lo, n_hi + 1))
+        p = aggregation(m)
+        return ("rap", transpose(p), stencil27(m), p)
+    rows = int(np.exp(rng.uniform(np.log(min_rows), np.log(max_rows))))
+    return ("ab", variable_uniform(rows, rows, 4, 32, rng), variable_uniform(rows, rows, 4, 32, rng))
+
+
+def batch_jobs(n: int = 4096, seed: int = 42, processes: int = 1, **kw):
+    """Jobs 0..n-1 of the batched workload (see batch_job)."""
+    if processes <= 1:
+        return [batch_job(i, seed, **kw) for i in range(n)]
+    import functools
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(processes) as pool:
+        return pool.map(functools.partial(batch_job, seed=seed, **kw), range(n), chunksize=16)

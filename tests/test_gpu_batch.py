@@ -1,0 +1,99 @@
+"""Batched products (spg_batch_*): every C_i of a batch, chained batch, graph
+replay and autotuned pipeline is bit-identical to the standalone product of
+the same operands, and the standalone products match the serial-order oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+import spgemm_amd as S
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def digest(m) -> str:
+    return O.csr_digest(O.Csr.make(m.rows, m.cols, m.row_ptr, m.col, m.val))
+
+
+def small_jobs(n=24, seed=7):
+    return W.batch_jobs(n, seed=seed, max_rows=2048, n_hi=12)
+
+
+def waves(jobs):
+    ab = [(j[1], j[2]) for j in jobs if j[0] == "ab"]
+    rap = [j[1:] for j in jobs if j[0] == "rap"]
+    a1 = [a for a, _ in ab] + [r for r, _, _ in rap]
+    b1 = [b for _, b in ab] + [a for _, a, _ in rap]
+    return ab, rap, a1, b1
+
+
+def test_batch_results_equal_standalone(engine):
+    jobs = small_jobs()
+    ab, rap, a1, b1 = waves(jobs)
+    batch = S.Batch(engine, a1, b1)
+    total = batch.run()
+    got = [batch.result(i) for i in range(len(a1))]
+    assert total == sum(g.nnz for g in got)
+    for i, (a, b) in enumerate(zip(a1, b1)):
+        print("product", i, a.rows, a.nnz, flush=True)
+        alone = engine.multiply(a, b)
+        assert digest(got[i]) == digest(alone), i
+        if i % 5 == 0:
+            ref = O.product(O.Csr.make(a.rows, a.cols, a.row_ptr, a.col, a.val),
+                            O.Csr.make(b.rows, b.cols, b.row_ptr, b.col, b.val))
+            assert O.csr_digest(ref) == digest(got[i]), i
+    # second product of each Galerkin triple, chained on the device
+    k0 = len(ab)
+    chain = S.Batch.chained(engine, batch, [k0 + k for k in range(len(rap))], [p for _, _, p in rap])
+    chain.run()
+    for k, (r, a, p) in enumerate(rap):
+        ra = engine.multiply(r, a)
+        rap_alone = engine.multiply(ra, p)
+        c = chain.result(k)
+        assert digest(c) == digest(rap_alone), k
+        # closed form: a coarse row of R.A.P sums the row sums of A over its
+        # 2x2x2 fine cells (integer stencil values: exact)
+        rs_a = np.add.reduceat(a.val, a.row_ptr[:-1]) if a.nnz else np.zeros(a.rows)
+        rs_a[np.diff(a.row_ptr) == 0] = 0
+        want = np.zeros(p.cols)
+        np.add.at(want, p.col, rs_a)
+        have = np.zeros(c.rows)
+        for i in range(c.rows):
+            have[i] = c.val[c.row_ptr[i]:c.row_ptr[i + 1]].sum()
+        assert np.array_equal(want, have), k
+    batch.close()
+    chain.close()
+
+
+def test_batch_graph_replay_and_autotune(engine):
+    jobs = small_jobs(18, seed=11)
+    _, _, a1, b1 = waves(jobs)
+    batch = S.Batch(engine, a1, b1)
+    batch.run()
+    ref = [digest(batch.result(i)) for i in range(len(a1))]
+    batch.set_graph(True)
+    for rep in range(3):  # first graph run captures, later ones replay
+        print("graph run", rep, flush=True)
+        batch.run()
+        assert [digest(batch.result(i)) for i in range(len(a1))] == ref
+    choice = batch.autotune(1)
+    assert choice in (0, 1, 2)
+    for rep in range(2):
+        batch.run()
+        assert [digest(batch.result(i)) for i in range(len(a1))] == ref
+    batch.close()
+
+
+def test_batch_errors(engine):
+    a = S.Csr.make(2, 3, [0, 1, 2], [0, 2], [1.0, 1.0])
+    b = S.Csr.make(4, 2, [0, 1, 1, 1, 2], [0, 1], [1.0, 1.0])
+    with pytest.raises(S.SpgError) as e:
+        S.Batch(engine, [a], [b])
+    assert e.value.name == "SPG_EINVAL"
+    bad = S.Csr.make(1, 3, [0, 1], [7], [1.0])  # column outside B_i's rows
+    ok_b = S.Csr.make(3, 2, [0, 1, 1, 2], [0, 1], [1.0, 1.0])
+    batch = S.Batch(engine, [bad], [ok_b])
+    with pytest.raises(S.SpgError) as e:
+        batch.run()
+    assert e.value.name == "SPG_ERANGE"
+    batch.close()
