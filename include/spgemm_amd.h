@@ -102,6 +102,10 @@ const char *spg_last_error(const spg_context *ctx);
 /* Kernels of this context run on `stream` (a hipStream_t; NULL = the context's
  * own stream). */
 spg_status spg_set_stream(spg_context *ctx, void *stream);
+/* The context's current stream (a hipStream_t). */
+spg_status spg_context_stream(const spg_context *ctx, void **stream);
+/* Make the context's device current on the calling host thread. */
+spg_status spg_context_bind(spg_context *ctx);
 /* Which pipeline computes the small rows (<= 256 products and A entries):
  *   SPG_PATH_AUTO         single pass when its C capacity (small rows' product
  *                         bounds + big rows' exact counts) fits half of the
@@ -176,10 +180,15 @@ spg_status spg_synchronize(spg_context *ctx);
  * with their columns shifted to their B_i's rows) and computes all of them
  * with one device pipeline.  Each C_i is bit-identical to its standalone
  * product.  The batch owns its result buffers and runs on the creating
- * context's stream.  Operands are host views (uploaded once). */
+ * context's stream.  Operands are all host views (packed on the host and
+ * uploaded once) or all device views (packed on the device; they need to stay
+ * valid only during the call). */
 typedef struct spg_batch spg_batch;
 spg_status spg_batch_create(spg_context *ctx, int32_t n, const spg_csr_view *A, const spg_csr_view *B,
                             spg_batch **out);
+/* Replace a batch's products (same rules as spg_batch_create), reusing its
+ * device buffers; its previous results are discarded. */
+spg_status spg_batch_rebuild(spg_batch *b, int32_t n, const spg_csr_view *A, const spg_csr_view *B);
 /* Chained batch: A_i is the result of product src_index[i] of batch `src`
  * (which must have run); the second product of (R.A).P.  spg_batch_refresh
  * copies the source's current results into this batch's A operands on the
@@ -203,6 +212,42 @@ spg_status spg_batch_copy(spg_batch *b, int32_t i, int64_t *row_ptr, int32_t *co
 spg_status spg_batch_info(const spg_batch *b, int32_t i, int32_t *rows, int32_t *cols);
 spg_status spg_batch_last_stats(const spg_batch *b, spg_stats *out);
 spg_status spg_batch_destroy(spg_batch *b);
+/* Run the batch on `stream` (NULL: the creating context's stream). */
+spg_status spg_batch_set_stream(spg_batch *b, void *stream);
+/* Message of the batch's last failure (the batch computes on its own
+ * context, so its errors are not reported on the creating one). */
+const char *spg_batch_last_error(const spg_batch *b);
+
+/* ---- served products ------------------------------------------------------
+ * No counterpart in the reference; the serving mode of the engine.  Requests
+ * (A, B) are submitted from any host thread and copied; a builder thread
+ * admits them in arrival order while their memory estimate (operands + 12 B
+ * per scalar product, the symbolic phase's bound on C) fits budget_bytes,
+ * packs them into batch arenas and uploads them on its own stream; a runner
+ * thread computes the batches on a second stream (continuous batching: the
+ * next batch is admitted and uploaded while the current one computes).
+ * Results stay in device memory, charged to the budget, until released; each
+ * is bit-identical to computing the product alone. */
+typedef struct spg_server spg_server;
+spg_status spg_server_create(int device, uint64_t budget_bytes, int max_batch, spg_server **out);
+/* Host views (copied); SPG_ENOMEM when the request alone exceeds the budget,
+ * SPG_ERANGE when an A column lies outside B's rows. */
+spg_status spg_server_submit(spg_server *s, const spg_csr_view *A, const spg_csr_view *B, int64_t *request_id);
+/* Register a matrix once (uploaded, resident, charged to the budget) and
+ * submit products of registered matrices by handle. */
+spg_status spg_server_register(spg_server *s, const spg_csr_view *host, int64_t *handle);
+spg_status spg_server_unregister(spg_server *s, int64_t handle);
+spg_status spg_server_submit_resident(spg_server *s, int64_t a_handle, int64_t b_handle, int64_t *request_id);
+/* Block until request `id` is computed; *latency_s = completion - submission. */
+spg_status spg_server_wait(spg_server *s, int64_t id, int64_t *nnz, double *latency_s);
+/* Copy a completed result to host buffers sized from spg_server_wait's nnz
+ * (row_ptr: A.rows + 1 entries). */
+spg_status spg_server_result(spg_server *s, int64_t id, int64_t *row_ptr, int32_t *col_idx, double *values);
+/* Free a completed request's device result and its budget share. */
+spg_status spg_server_release(spg_server *s, int64_t id);
+spg_status spg_server_stats(spg_server *s, int64_t *batches, int64_t *completed, double *busy_s,
+                            uint64_t *charged_bytes);
+spg_status spg_server_destroy(spg_server *s);
 
 /* ---- staged accumulation (trident::LocalAccumulator) -------------------
  * Replaces trident::LocalAccumulator{LocalAccumulator(rows, cols);

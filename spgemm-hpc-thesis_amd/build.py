@@ -32,10 +32,10 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wextra", f"-I{INCLUDE}"]
 
 HIP_SOURCES = ["analyze.hip", "symbolic.hip", "numeric.hip", "fused.hip", "scan.hip", "engine.hip", "batch.hip"]
-CXX_SOURCES = ["host_matrix.cpp"]
+CXX_SOURCES = ["host_matrix.cpp", "server.cpp"]
 
 
 def _run(cmd, verbose):
@@ -62,7 +62,7 @@ def build_library(verbose: bool = False) -> str:
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -51,6 +51,37 @@ __global__ void k_refresh(int64_t nrows, const int64_t *dst_rp, const int64_t *s
     }
 }
 
+// One chunk of arena packing: entries [e0, e1) of a product (columns shifted
+// and checked, values copied), or row offsets [e0, e1) of a product (shifted
+// by the product's first arena entry).
+struct PackChunk {
+    const int64_t *src_rp;
+    const int32_t *src_col;
+    const double *src_val;
+    int64_t e0, e1;
+    int64_t dst_off;  // arena entry (entries) or arena row (row offsets) of the product's first
+    int64_t add;      // row offsets: the product's first arena entry
+    int32_t shift;    // entries: column shift
+    int32_t lim;      // entries: columns must lie in [0, lim) or become -1 (outside the directory)
+    int32_t rows;     // 1: row-offset chunk
+    int32_t pad;
+};
+
+__global__ void k_pack(const PackChunk *chunks, int64_t *dst_rp, int32_t *dst_col, double *dst_val) {
+    const PackChunk c = chunks[blockIdx.x];
+    for (int64_t q = c.e0 + threadIdx.x; q < c.e1; q += blockDim.x) {
+        if (c.rows) {
+            dst_rp[c.dst_off + q] = c.src_rp[q] + c.add;
+        } else {
+            const int32_t v = c.src_col[q];
+            dst_col[c.dst_off + q] = c.lim == INT_MAX ? v : ((v >= 0 && v < c.lim) ? v + c.shift : -1);
+            dst_val[c.dst_off + q] = c.src_val[q];
+        }
+    }
+}
+
+constexpr int64_t kPackChunk = 1 << 16;
+
 }  // namespace
 
 struct spg_batch {
@@ -156,6 +187,105 @@ spg_status stack_b(spg_context *ctx, spg_batch *b, int32_t n, const spg_csr_view
     return SPG_OK;
 }
 
+// Packs n device views into an arena (row offsets, entries) with the given
+// per-product first arena row/entry, column shift and column limit.
+spg_status pack_device(spg_context *ctx, int32_t n, const spg_csr_view *M, const std::vector<int64_t> &row0,
+                       const std::vector<int64_t> &ent0, const std::vector<int32_t> &shift,
+                       const std::vector<int32_t> &lim, DevBuf &rp, DevBuf &col, DevBuf &val, DevBuf &desc) {
+    std::vector<PackChunk> ch;
+    for (int32_t i = 0; i < n; ++i) {
+        const spg_csr_view &m = M[i];
+        for (int64_t e = 0; e < m.nnz; e += kPackChunk) {
+            PackChunk c{};
+            c.src_rp = m.row_ptr;
+            c.src_col = m.col_idx;
+            c.src_val = m.values;
+            c.e0 = e;
+            c.e1 = std::min<int64_t>(m.nnz, e + kPackChunk);
+            c.dst_off = ent0[(size_t)i];
+            c.shift = shift[(size_t)i];
+            c.lim = lim[(size_t)i];
+            ch.push_back(c);
+        }
+        // row offsets 1..rows (offset 0 is the previous product's last)
+        for (int64_t r = (i == 0 ? 0 : 1); r <= m.rows; r += kPackChunk) {
+            PackChunk c{};
+            c.src_rp = m.row_ptr;
+            c.e0 = r;
+            c.e1 = std::min<int64_t>((int64_t)m.rows + 1, r + kPackChunk);
+            c.dst_off = row0[(size_t)i];
+            c.add = ent0[(size_t)i];
+            c.rows = 1;
+            ch.push_back(c);
+        }
+    }
+    if (ch.empty()) return SPG_OK;
+    SPG_HIP(desc.ensure(ch.size() * sizeof(PackChunk), ctx->stream), "hipMalloc(pack descriptors)");
+    SPG_HIP(hipMemcpyAsync(desc.p, ch.data(), ch.size() * sizeof(PackChunk), hipMemcpyHostToDevice, ctx->stream),
+            "upload pack descriptors");
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)ch.size()), dim3(256), 0, ctx->stream, desc.as<PackChunk>(),
+                       rp.as<int64_t>(), col.as<int32_t>(), val.as<double>());
+    SPG_HIP(hipGetLastError(), "pack kernel");
+    return SPG_OK;
+}
+
+// Arena from device views: the caller guarantees they stay valid during the call.
+spg_status create_from_device(spg_context *ctx, spg_batch *b, int32_t n, const spg_csr_view *A,
+                              const spg_csr_view *B) {
+    std::vector<int64_t> arow0, aent0, brow0, bent0;
+    std::vector<int32_t> ashift, alim, bshift, blim;
+    int64_t ar = 0, ae = 0, br = 0, be = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (A[i].location != SPG_DEVICE || B[i].location != SPG_DEVICE)
+            return set_err(ctx, SPG_EINVAL, "batch operands must be all host or all device views");
+        spg_status s = check_host_csr(ctx, &A[i], "A_i");
+        if (s == SPG_OK) s = check_host_csr(ctx, &B[i], "B_i");
+        if (s != SPG_OK) return s;
+        if (A[i].cols != B[i].rows)
+            return set_err(ctx, SPG_EINVAL, "SpGEMM requires A.cols == B.rows (product " + std::to_string(i) + ")");
+        b->a_row0.push_back(ar);
+        b->rows.push_back(A[i].rows);
+        b->cols.push_back(B[i].cols);
+        b->max_cols = std::max(b->max_cols, B[i].cols);
+        arow0.push_back(ar);
+        aent0.push_back(ae);
+        brow0.push_back(br);
+        bent0.push_back(be);
+        ashift.push_back((int32_t)br);
+        alim.push_back(A[i].cols);
+        bshift.push_back(0);
+        blim.push_back(INT_MAX);
+        ar += A[i].rows;
+        ae += A[i].nnz;
+        br += B[i].rows;
+        be += B[i].nnz;
+    }
+    if (ar > INT32_MAX || br > INT32_MAX) return set_err(ctx, SPG_EOVERFLOW, "batch rows exceed INT32_MAX");
+    b->total_rows = ar;
+    b->a_nnz = ae;
+    b->b_rows = br;
+    b->b_nnz = be;
+    hipStream_t st = ctx->stream;
+    SPG_HIP(b->a_ptr.ensure((size_t)(ar + 1) * 8, st), "hipMalloc(batch arena)");
+    SPG_HIP(b->a_col.ensure((size_t)(ae ? ae : 1) * 4, st), "hipMalloc(batch arena)");
+    SPG_HIP(b->a_val.ensure((size_t)(ae ? ae : 1) * 8, st), "hipMalloc(batch arena)");
+    SPG_HIP(b->b_ptr.ensure((size_t)(br + 1) * 8, st), "hipMalloc(batch arena)");
+    SPG_HIP(b->b_col.ensure((size_t)(be ? be : 1) * 4, st), "hipMalloc(batch arena)");
+    SPG_HIP(b->b_val.ensure((size_t)(be ? be : 1) * 8, st), "hipMalloc(batch arena)");
+    launch_zero_bytes(b->a_ptr.p, 8, st);
+    launch_zero_bytes(b->b_ptr.p, 8, st);
+    spg_status s;
+    DevBuf desc_a, desc_b;
+    if ((s = pack_device(ctx, n, A, arow0, aent0, ashift, alim, b->a_ptr, b->a_col, b->a_val, desc_a)) == SPG_OK)
+        s = pack_device(ctx, n, B, brow0, bent0, bshift, blim, b->b_ptr, b->b_col, b->b_val, desc_b);
+    hipError_t e = hipStreamSynchronize(st);
+    desc_a.release();
+    desc_b.release();
+    if (s != SPG_OK) return s;
+    if (e != hipSuccess) return hip_err(ctx, e, "pack batch arena");
+    return SPG_OK;
+}
+
 spg_status capture(spg_batch *b) {
     spg_context *ctx = b->own;
     b->drop_graph();
@@ -174,38 +304,37 @@ spg_status capture(spg_batch *b) {
 
 extern "C" {
 
-spg_status spg_batch_create(spg_context *ctx, int32_t n, const spg_csr_view *A, const spg_csr_view *B,
-                            spg_batch **out) {
-    if (!ctx) return SPG_EINVAL;
-    if (!out || n < 0 || (n > 0 && (!A || !B))) return set_err(ctx, SPG_EINVAL, "null argument");
-    *out = nullptr;
-    SPG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
-    spg_batch *b = nullptr;
-    spg_status s = new_batch(ctx, &b);
-    if (s != SPG_OK) return s;
+namespace {
+
+// (Re)builds a batch's arena from n products; the batch keeps its buffers and
+// private context, so a rebuilt batch allocates nothing once warm.
+spg_status build_arena(spg_context *ctx, spg_batch *b, int32_t n, const spg_csr_view *A, const spg_csr_view *B) {
+    b->drop_graph();
+    b->planned = false;
+    b->plan = ProductPlan{};
     b->n = n;
-    std::vector<int64_t> b_row0;
-    if ((s = stack_b(ctx, b, n, B, b_row0)) != SPG_OK) {
-        delete b;
-        return s;
+    b->a_row0.clear();
+    b->rows.clear();
+    b->cols.clear();
+    b->max_cols = 0;
+    b->own->have_result = false;
+    spg_status s;
+    if (n > 0 && A[0].location == SPG_DEVICE) {
+        if ((s = create_from_device(ctx, b, n, A, B)) != SPG_OK) return s;
+        set_operands(b);
+        return SPG_OK;
     }
+    std::vector<int64_t> b_row0;
+    if ((s = stack_b(ctx, b, n, B, b_row0)) != SPG_OK) return s;
     std::vector<int64_t> ap(1, 0);
     std::vector<int32_t> ac;
     std::vector<double> av;
     for (int32_t i = 0; i < n; ++i) {
         const spg_csr_view &m = A[i];
-        if (m.location != SPG_HOST) {
-            delete b;
-            return set_err(ctx, SPG_EINVAL, "batch operands take host views");
-        }
-        if ((s = check_host_csr(ctx, &m, "A_i")) != SPG_OK) {
-            delete b;
-            return s;
-        }
-        if (m.cols != B[i].rows) {
-            delete b;
+        if (m.location != SPG_HOST) return set_err(ctx, SPG_EINVAL, "batch operands must be all host or all device views");
+        if ((s = check_host_csr(ctx, &m, "A_i")) != SPG_OK) return s;
+        if (m.cols != B[i].rows)
             return set_err(ctx, SPG_EINVAL, "SpGEMM requires A.cols == B.rows (product " + std::to_string(i) + ")");
-        }
         b->a_row0.push_back((int64_t)ap.size() - 1);
         b->rows.push_back(m.rows);
         const int64_t base = ap.back();
@@ -219,25 +348,46 @@ spg_status spg_batch_create(spg_context *ctx, int32_t n, const spg_csr_view *A, 
         }
         av.insert(av.end(), m.values, m.values + m.nnz);
     }
-    if ((int64_t)ap.size() - 1 > INT32_MAX) {
-        delete b;
-        return set_err(ctx, SPG_EOVERFLOW, "batch rows exceed INT32_MAX");
-    }
+    if ((int64_t)ap.size() - 1 > INT32_MAX) return set_err(ctx, SPG_EOVERFLOW, "batch rows exceed INT32_MAX");
     b->total_rows = (int64_t)ap.size() - 1;
     b->a_nnz = ap.back();
     if ((s = upload(ctx, b->a_ptr, ap.data(), ap.size() * 8)) != SPG_OK ||
         (s = upload(ctx, b->a_col, ac.data(), ac.size() * 4)) != SPG_OK ||
-        (s = upload(ctx, b->a_val, av.data(), av.size() * 8)) != SPG_OK) {
+        (s = upload(ctx, b->a_val, av.data(), av.size() * 8)) != SPG_OK)
+        return s;
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_err(ctx, e, "upload batch");
+    set_operands(b);
+    return SPG_OK;
+}
+
+}  // namespace
+
+spg_status spg_batch_create(spg_context *ctx, int32_t n, const spg_csr_view *A, const spg_csr_view *B,
+                            spg_batch **out) {
+    if (!ctx) return SPG_EINVAL;
+    if (!out || n < 0 || (n > 0 && (!A || !B))) return set_err(ctx, SPG_EINVAL, "null argument");
+    *out = nullptr;
+    SPG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    spg_batch *b = nullptr;
+    spg_status s = new_batch(ctx, &b);
+    if (s != SPG_OK) return s;
+    if ((s = build_arena(ctx, b, n, A, B)) != SPG_OK) {
         delete b;
         return s;
     }
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        delete b;
-        return hip_err(ctx, hipGetLastError(), "upload batch");
-    }
-    set_operands(b);
     *out = b;
     return SPG_OK;
+}
+
+spg_status spg_batch_rebuild(spg_batch *b, int32_t n, const spg_csr_view *A, const spg_csr_view *B) {
+    if (!b) return SPG_EINVAL;
+    spg_context *ctx = b->parent;
+    if (b->src) return set_err(ctx, SPG_ESTATE, "a chained batch cannot be rebuilt");
+    if (n < 0 || (n > 0 && (!A || !B))) return set_err(ctx, SPG_EINVAL, "null argument");
+    SPG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    SPG_HIP(hipStreamSynchronize(b->own->stream), "hipStreamSynchronize");
+    return build_arena(ctx, b, n, A, B);
 }
 
 spg_status spg_batch_create_chained(spg_context *ctx, const spg_batch *src, int32_t n, const int32_t *src_index,
@@ -329,17 +479,17 @@ spg_status spg_batch_run(spg_batch *b, int64_t *nnz_total) {
     spg_context *ctx = b->own;
     spg_status s;
     if (!b->planned || !b->use_graph) {
-        if ((s = run_product(ctx, b->plan)) != SPG_OK) return set_err(b->parent, s, ctx->err);
+        if ((s = run_product(ctx, b->plan)) != SPG_OK) return s;
         b->planned = true;
     } else {
-        if (!b->exec && (s = capture(b)) != SPG_OK) return set_err(b->parent, s, ctx->err);
+        if (!b->exec && (s = capture(b)) != SPG_OK) return s;
         ctx->have_result = false;
         if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
         SPG_HIP(hipGraphLaunch(b->exec, ctx->stream), "hipGraphLaunch");
         if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
-        if ((s = read_control(ctx)) != SPG_OK) return set_err(b->parent, s, ctx->err);
+        if ((s = read_control(ctx)) != SPG_OK) return s;
         spg_stats stats{};
-        if ((s = finish_product(ctx, b->plan, *ctx->h_ctrl, stats)) != SPG_OK) return set_err(b->parent, s, ctx->err);
+        if ((s = finish_product(ctx, b->plan, *ctx->h_ctrl, stats)) != SPG_OK) return s;
         if (ctx->timing) {
             float t = 0;
             SPG_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[3]), "hipEventElapsedTime");
@@ -349,6 +499,14 @@ spg_status spg_batch_run(spg_batch *b, int64_t *nnz_total) {
     *nnz_total = ctx->res_nnz;
     return SPG_OK;
 }
+
+spg_status spg_batch_set_stream(spg_batch *b, void *stream) {
+    if (!b) return SPG_EINVAL;
+    b->own->stream = stream ? static_cast<hipStream_t>(stream) : b->parent->stream;
+    return SPG_OK;
+}
+
+const char *spg_batch_last_error(const spg_batch *b) { return b ? b->own->err.c_str() : "null batch"; }
 
 spg_status spg_batch_set_graph(spg_batch *b, int enabled) {
     if (!b) return SPG_EINVAL;
@@ -409,27 +567,28 @@ spg_status spg_batch_autotune(spg_batch *b, int repeats, int *choice) {
 spg_status spg_batch_product_nnz(const spg_batch *b, int32_t i, int64_t *nnz) {
     if (!b || !nnz) return SPG_EINVAL;
     const spg_context *ctx = b->own;
-    if (!ctx->have_result) return set_err(b->parent, SPG_ESTATE, "the batch has not run");
-    if (i < 0 || i >= b->n) return set_err(b->parent, SPG_EINVAL, "product index out of range");
+    if (!ctx->have_result) return SPG_ESTATE;
+    if (i < 0 || i >= b->n) return SPG_EINVAL;
     int64_t v[2] = {0, 0};
     const int64_t r0 = b->a_row0[(size_t)i];
     hipError_t e = hipMemcpy(v, ctx->rowptr[ctx->cur].as<int64_t>() + r0, 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess)
         e = hipMemcpy(v + 1, ctx->rowptr[ctx->cur].as<int64_t>() + r0 + b->rows[(size_t)i], 8, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return hip_err(b->parent, e, "download row offsets");
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return SPG_EDEVICE;
+    }
     *nnz = v[1] - v[0];
     return SPG_OK;
 }
 
 spg_status spg_batch_copy(spg_batch *b, int32_t i, int64_t *row_ptr, int32_t *col_idx, double *values) {
     if (!b || !row_ptr) return SPG_EINVAL;
-    if (i < 0 || i >= b->n) return set_err(b->parent, SPG_EINVAL, "product index out of range");
+    if (i < 0 || i >= b->n) return set_err(b->own, SPG_EINVAL, "product index out of range");
     const int64_t r0 = b->a_row0[(size_t)i];
     int64_t cnt = 0;
-    spg_status s = spg_result_rows(b->own, (int32_t)r0, (int32_t)(r0 + b->rows[(size_t)i]), row_ptr, col_idx, values,
-                                   INT64_MAX, &cnt);
-    if (s != SPG_OK) return set_err(b->parent, s, b->own->err);
-    return SPG_OK;
+    return spg_result_rows(b->own, (int32_t)r0, (int32_t)(r0 + b->rows[(size_t)i]), row_ptr, col_idx, values,
+                           INT64_MAX, &cnt);
 }
 
 spg_status spg_batch_info(const spg_batch *b, int32_t i, int32_t *rows, int32_t *cols) {

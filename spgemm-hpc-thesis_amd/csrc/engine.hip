@@ -370,6 +370,18 @@ spg_status spg_set_stream(spg_context *ctx, void *stream) {
     return SPG_OK;
 }
 
+spg_status spg_context_stream(const spg_context *ctx, void **stream) {
+    if (!ctx || !stream) return SPG_EINVAL;
+    *stream = ctx->stream;
+    return SPG_OK;
+}
+
+spg_status spg_context_bind(spg_context *ctx) {
+    if (!ctx) return SPG_EINVAL;
+    SPG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    return SPG_OK;
+}
+
 spg_status spg_set_path(spg_context *ctx, int path) {
     if (!ctx) return SPG_EINVAL;
     if (path < SPG_PATH_AUTO || path > SPG_PATH_TWO_PHASE) return set_err(ctx, SPG_EINVAL, "unknown path");

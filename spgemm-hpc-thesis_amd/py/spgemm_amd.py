@@ -35,7 +35,10 @@ EXPORTED = [
     "spg_synchronize", "spg_set_path", "spg_multiply_rows", "spg_result_rows",
     "spg_batch_create", "spg_batch_create_chained", "spg_batch_refresh", "spg_batch_run", "spg_batch_set_graph",
     "spg_batch_autotune", "spg_batch_product_nnz", "spg_batch_copy", "spg_batch_info", "spg_batch_last_stats",
-    "spg_batch_destroy",
+    "spg_batch_destroy", "spg_batch_rebuild", "spg_batch_set_stream", "spg_batch_last_error", "spg_context_stream", "spg_context_bind",
+    "spg_server_create", "spg_server_submit", "spg_server_register", "spg_server_unregister",
+    "spg_server_submit_resident", "spg_server_wait", "spg_server_result", "spg_server_release", "spg_server_stats",
+    "spg_server_destroy",
 ]
 
 PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE = 0, 1, 2
@@ -139,6 +142,18 @@ def lib():
         L.spg_batch_info.argtypes = [vp, C.c_int32, P(C.c_int32), P(C.c_int32)]
         L.spg_batch_last_stats.argtypes = [vp, P(Stats)]
         L.spg_batch_destroy.argtypes = [vp]
+        L.spg_batch_last_error.argtypes = [vp]
+        L.spg_batch_last_error.restype = C.c_char_p
+        L.spg_server_create.argtypes = [C.c_int, C.c_uint64, C.c_int, P(vp)]
+        L.spg_server_submit.argtypes = [vp, P(CsrView), P(CsrView), P(C.c_int64)]
+        L.spg_server_register.argtypes = [vp, P(CsrView), P(C.c_int64)]
+        L.spg_server_unregister.argtypes = [vp, C.c_int64]
+        L.spg_server_submit_resident.argtypes = [vp, C.c_int64, C.c_int64, P(C.c_int64)]
+        L.spg_server_wait.argtypes = [vp, C.c_int64, P(C.c_int64), P(C.c_double)]
+        L.spg_server_result.argtypes = [vp, C.c_int64, vp, vp, vp]
+        L.spg_server_release.argtypes = [vp, C.c_int64]
+        L.spg_server_stats.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_uint64)]
+        L.spg_server_destroy.argtypes = [vp]
         L.spg_multiply_rows.argtypes = [vp, P(CsrView), P(CsrView), C.c_int32, P(HaloView), C.c_int32, C.c_int32,
                                         P(C.c_int64)]
         _lib = L
@@ -373,12 +388,16 @@ class Batch:
         out.n = n
         return out
 
+    def _bcheck(self, s: int):
+        if s != 0:
+            raise SpgError(s, self._L.spg_batch_last_error(self._h).decode(errors="replace"))
+
     def refresh(self):
         self._e._check(self._L.spg_batch_refresh(self._h))
 
     def run(self) -> int:
         n = C.c_int64()
-        self._e._check(self._L.spg_batch_run(self._h, C.byref(n)))
+        self._bcheck(self._L.spg_batch_run(self._h, C.byref(n)))
         return int(n.value)
 
     def set_graph(self, on: bool):
@@ -386,12 +405,12 @@ class Batch:
 
     def autotune(self, repeats: int = 2) -> int:
         c = C.c_int()
-        self._e._check(self._L.spg_batch_autotune(self._h, repeats, C.byref(c)))
+        self._bcheck(self._L.spg_batch_autotune(self._h, repeats, C.byref(c)))
         return int(c.value)
 
     def product_nnz(self, i: int) -> int:
         n = C.c_int64()
-        self._e._check(self._L.spg_batch_product_nnz(self._h, i, C.byref(n)))
+        self._bcheck(self._L.spg_batch_product_nnz(self._h, i, C.byref(n)))
         return int(n.value)
 
     def result(self, i: int) -> Csr:
@@ -401,7 +420,7 @@ class Batch:
         rp = np.zeros(r.value + 1, dtype=np.int64)
         col = np.zeros(max(nnz, 1), dtype=np.int32)
         val = np.zeros(max(nnz, 1), dtype=np.float64)
-        self._e._check(self._L.spg_batch_copy(self._h, i, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+        self._bcheck(self._L.spg_batch_copy(self._h, i, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
         return Csr(int(r.value), int(c.value), rp, col[:nnz], val[:nnz])
 
     def stats(self) -> dict:
@@ -412,6 +431,85 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None):
             self._L.spg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Served products with continuous batching (spg_server_*).  submit() and
+    submit_resident() return request ids; wait() blocks until the product is
+    computed; result() copies it out; release() frees it."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 200 << 30, max_batch: int = 4096):
+        self._L = lib()
+        h = C.c_void_p()
+        s = self._L.spg_server_create(device, budget_bytes, max_batch, C.byref(h))
+        if s != 0:
+            raise SpgError(s, "spg_server_create failed")
+        self._h = h
+        self._shape = {}
+        self._rshape = {}
+
+    def _check(self, s: int, what: str):
+        if s != 0:
+            raise SpgError(s, what)
+
+    def submit(self, a: Csr, b: Csr) -> int:
+        rid = C.c_int64()
+        av, bv = host_view(a), host_view(b)
+        self._check(self._L.spg_server_submit(self._h, C.byref(av), C.byref(bv), C.byref(rid)), "submit")
+        self._shape[rid.value] = (a.rows, b.cols)
+        return int(rid.value)
+
+    def register(self, m: Csr) -> int:
+        h = C.c_int64()
+        v = host_view(m)
+        self._check(self._L.spg_server_register(self._h, C.byref(v), C.byref(h)), "register")
+        self._rshape[h.value] = (m.rows, m.cols)
+        return int(h.value)
+
+    def unregister(self, handle: int):
+        self._check(self._L.spg_server_unregister(self._h, handle), "unregister")
+
+    def submit_resident(self, ha: int, hb: int) -> int:
+        rid = C.c_int64()
+        self._check(self._L.spg_server_submit_resident(self._h, ha, hb, C.byref(rid)), "submit_resident")
+        self._shape[rid.value] = (self._rshape[ha][0], self._rshape[hb][1])
+        return int(rid.value)
+
+    def wait(self, rid: int):
+        """(nnz, latency seconds) of a completed request."""
+        n, lat = C.c_int64(), C.c_double()
+        self._check(self._L.spg_server_wait(self._h, rid, C.byref(n), C.byref(lat)), f"request {rid}")
+        return int(n.value), float(lat.value)
+
+    def result(self, rid: int) -> Csr:
+        nnz, _ = self.wait(rid)
+        rows, cols = self._shape[rid]
+        rp = np.zeros(rows + 1, dtype=np.int64)
+        col = np.zeros(max(nnz, 1), dtype=np.int32)
+        val = np.zeros(max(nnz, 1), dtype=np.float64)
+        self._check(self._L.spg_server_result(self._h, rid, rp.ctypes.data, col.ctypes.data, val.ctypes.data),
+                    "result")
+        return Csr(rows, cols, rp, col[:nnz], val[:nnz])
+
+    def release(self, rid: int):
+        self._check(self._L.spg_server_release(self._h, rid), "release")
+        self._shape.pop(rid, None)
+
+    def stats(self) -> dict:
+        b, c, busy, ch = C.c_int64(), C.c_int64(), C.c_double(), C.c_uint64()
+        self._check(self._L.spg_server_stats(self._h, C.byref(b), C.byref(c), C.byref(busy), C.byref(ch)), "stats")
+        return {"batches": b.value, "completed": c.value, "busy_s": busy.value, "charged_bytes": ch.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.spg_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -16,7 +16,7 @@ def digest(m) -> str:
 
 
 def small_jobs(n=24, seed=7):
-    return W.batch_jobs(n, seed=seed, max_rows=2048, n_hi=12)
+    return W.batch_jobs(n, seed=seed, max_rows=2048, n_lo=6, n_hi=12)
 
 
 def waves(jobs):

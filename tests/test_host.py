@@ -52,6 +52,17 @@ def test_create_without_device_reports_enodevice():
     assert s == 7  # SPG_ENODEVICE
 
 
+def test_server_and_batch_need_a_device():
+    import ctypes as C
+    h = C.c_void_p()
+    if S.lib().spg_create(C.byref(h), 0) == 0:
+        S.lib().spg_destroy(h)
+        pytest.skip("a GPU is present")
+    with pytest.raises(S.SpgError) as e:
+        S.Server(0, budget_bytes=1 << 30)
+    assert e.value.name == "SPG_ENODEVICE"
+
+
 def test_cpp_shim_builds_and_fails_loudly_without_device(tmp_path):
     """The C++17 shim program links against the library; with no GPU the
     Engine constructor throws (no silent CPU path)."""
