@@ -19,11 +19,6 @@
 
 using namespace spg;
 
-struct spg_accumulator {
-    spg_context *ctx = nullptr;
-    int32_t rows = 0, cols = 0;
-};
-
 namespace spg {
 
 spg_status set_err(spg_context *c, spg_status s, const std::string &msg) {
@@ -637,19 +632,6 @@ spg_status spg_device_matrix_free(spg_device_matrix *m) {
 spg_status spg_synchronize(spg_context *ctx) {
     if (!ctx) return SPG_EINVAL;
     SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    return SPG_OK;
-}
-
-// Staged accumulation is not available yet on the device.
-spg_status spg_accumulator_create(spg_context *ctx, int32_t, int32_t, spg_accumulator **acc) {
-    if (acc) *acc = nullptr;
-    return set_err(ctx, SPG_ESTATE, "staged accumulation is not implemented");
-}
-spg_status spg_accumulator_add(spg_accumulator *, const spg_csr_view *, const spg_csr_view *) { return SPG_ESTATE; }
-spg_status spg_accumulator_finish(spg_accumulator *, int64_t *) { return SPG_ESTATE; }
-spg_status spg_accumulator_copy(spg_accumulator *, int64_t *, int32_t *, double *) { return SPG_ESTATE; }
-spg_status spg_accumulator_destroy(spg_accumulator *acc) {
-    delete acc;
     return SPG_OK;
 }
 

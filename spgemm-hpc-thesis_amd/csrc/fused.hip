@@ -209,7 +209,7 @@ __device__ __forceinline__ void form_products(RowItems &it) {
 // distinct columns.
 template <bool SMALLB>
 __device__ __forceinline__ int sort_fold(FusedWave<SMALLB> &w, int P, const RowItems &it, RowOut &out,
-                                         int &distinct) {
+                                         int &distinct, int keep_zeros) {
     const int lane = lane_id();
     const int(&col)[kFItems] = it.col;
     const int(&sidx)[kFItems] = it.sidx;
@@ -339,7 +339,7 @@ __device__ __forceinline__ int sort_fold(FusedWave<SMALLB> &w, int P, const RowI
     uint32_t packed_nz = 0, packed_head = 0;
 #pragma unroll
     for (int j = 0; j < kFItems; ++j) {
-        packed_nz |= (head[j] && vv[j] != 0.0 ? 1u : 0u) << (8 * j);
+        packed_nz |= (head[j] && (keep_zeros || vv[j] != 0.0) ? 1u : 0u) << (8 * j);
         packed_head |= (head[j] ? 1u : 0u) << (8 * j);
     }
     const uint32_t incl = (uint32_t)wave_incl_sum_dpp((int)packed_nz);
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(kFusedTpb) void k_fused(Operands op, Workspace ws, 
             for (int k = 0; k < R; ++k) {
                 if (fused[k]) {
                     int dist = 0;
-                    cnt[k] = sort_fold<SMALLB>(w, (int)lane64(ubl, k), it[k], out[k], dist);
+                    cnt[k] = sort_fold<SMALLB>(w, (int)lane64(ubl, k), it[k], out[k], dist, op.keep_zeros);
                     distinct_sum += dist;
                     wave_total += cnt[k];
                 }
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(kFusedTpb) void k_fused(Operands op, Workspace ws, 
                 form_products(it);
                 wave_sync();
                 int dist = 0;
-                cnt[k] = sort_fold<SMALLB>(w, (int)lane64(ubl, k), it, out[k], dist);
+                cnt[k] = sort_fold<SMALLB>(w, (int)lane64(ubl, k), it, out[k], dist, op.keep_zeros);
                 distinct_sum += dist;
                 wave_total += cnt[k];
             }

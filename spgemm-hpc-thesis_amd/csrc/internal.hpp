@@ -85,6 +85,7 @@ struct Operands {
     const int32_t *bcol;
     const double *bval;
     int32_t bcols;
+    int32_t keep_zeros;  // 1: store coordinates whose sum is exactly 0.0 (staged accumulation)
 };
 
 // One listed row of a bin: its index, its A-entry count and its first A entry

@@ -165,7 +165,7 @@ __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws,
             if (x == 0 || s_col[x - 1] != c) {
                 v = s_a[x];
                 for (int y = x + 1; y < P && s_col[y] == c; ++y) v = v + s_a[y];
-                nz = v != 0.0;
+                nz = op.keep_zeros || v != 0.0;
             }
         }
         int tot;
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                 while (m) {
                     const int b = __ffs((int)m) - 1;
                     m &= m - 1u;
-                    if (acc[q * 32 + b] != 0.0) ++mine;
+                    if (op.keep_zeros || acc[q * 32 + b] != 0.0) ++mine;
                 }
             }
             int64_t wtotal;
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                     const int b = __ffs((int)m) - 1;
                     m &= m - 1u;
                     const double v = acc[q * 32 + b];
-                    if (v != 0.0) {
+                    if (op.keep_zeros || v != 0.0) {
                         if (o < lim) {
                             ccol[cbase + o] = (int32_t)(w0 + q * 32 + b);
                             cval[cbase + o] = v;

@@ -95,6 +95,54 @@ int main(int argc, char **argv) {
             }
         }
         ok &= throws<std::runtime_error>([&] { readMatrixMarket("/nonexistent/file.mtx"); }, "mm open");
+
+        // LocalAccumulator: A's columns split into two contiguous ranges.  When
+        // every row of A lists its columns in ascending order, the staged sums
+        // follow the same serial order as the direct product, so finish()
+        // must equal spgemm() bit for bit.
+        bool sorted_rows = true;
+        for (int r = 0; r < a.rows; ++r)
+            for (int64_t e = a.rowPtr[r] + 1; e < a.rowPtr[r + 1]; ++e)
+                sorted_rows &= a.columnIndices[e - 1] < a.columnIndices[e];
+        if (sorted_rows && b.rows > 1) {
+            const int split = b.rows / 2;
+            LocalAccumulator acc(engine, a.rows, b.cols);
+            for (int stage = 0; stage < 2; ++stage) {
+                const int k0 = stage == 0 ? 0 : split, k1 = stage == 0 ? split : b.rows;
+                CsrMatrix as, bs;
+                as.rows = a.rows;
+                as.cols = k1 - k0;
+                as.rowPtr.push_back(0);
+                for (int r = 0; r < a.rows; ++r) {
+                    for (int64_t e = a.rowPtr[r]; e < a.rowPtr[r + 1]; ++e) {
+                        const int col = a.columnIndices[e];
+                        if (col >= k0 && col < k1) {
+                            as.columnIndices.push_back(col - k0);
+                            as.values.push_back(a.values[e]);
+                        }
+                    }
+                    as.rowPtr.push_back((int64_t)as.values.size());
+                }
+                bs.rows = k1 - k0;
+                bs.cols = b.cols;
+                bs.rowPtr.push_back(0);
+                for (int r = k0; r < k1; ++r) bs.rowPtr.push_back(bs.rowPtr.back() + b.rowPtr[r + 1] - b.rowPtr[r]);
+                bs.columnIndices.assign(b.columnIndices.begin() + b.rowPtr[k0], b.columnIndices.begin() + b.rowPtr[k1]);
+                bs.values.assign(b.values.begin() + b.rowPtr[k0], b.values.begin() + b.rowPtr[k1]);
+                acc.addProduct(as, bs);
+            }
+            const CsrMatrix ca = acc.finish();
+            const bool same = ca.rowPtr == c.rowPtr && ca.columnIndices == c.columnIndices &&
+                              std::memcmp(ca.values.data(), c.values.data(), c.values.size() * 8) == 0;
+            if (!same) std::fprintf(stderr, "staged accumulation differs from the direct product\n");
+            ok &= same;
+            CsrMatrix one_row;  // a.rows + 1 rows: matrix_blocks.cpp:156-158 rejects it
+            one_row.rows = a.rows + 1;
+            one_row.cols = b.rows;
+            one_row.rowPtr.assign((size_t)a.rows + 2, 0);
+            ok &= throws<std::invalid_argument>([&] { acc.addProduct(one_row, b); }, "accumulator shapes");
+            std::printf("accumulate %s\n", same ? "ok" : "FAILED");
+        }
         std::printf(ok ? "shim ok\n" : "shim FAILED\n");
         return ok ? 0 : 1;
     } catch (const std::exception &e) {

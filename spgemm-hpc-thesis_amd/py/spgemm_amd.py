@@ -517,3 +517,42 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Accumulator:
+    """Staged accumulation on the device (trident::LocalAccumulator,
+    src/trident/common/matrix_blocks.cpp:152-200): add_product(a, b) adds a*b
+    into per-row running sums in call order; finish() drops exact zeros."""
+
+    def __init__(self, engine: "Engine", rows: int, cols: int):
+        self._e = engine
+        self._L = engine._L
+        h = C.c_void_p()
+        engine._check(self._L.spg_accumulator_create(engine._h, rows, cols, C.byref(h)))
+        self._h = h
+        self.rows, self.cols = rows, cols
+
+    def add_product(self, a: Csr, b: Csr):
+        av, bv = host_view(a), host_view(b)
+        self._e._check(self._L.spg_accumulator_add(self._h, C.byref(av), C.byref(bv)))
+
+    def finish(self) -> Csr:
+        n = C.c_int64()
+        self._e._check(self._L.spg_accumulator_finish(self._h, C.byref(n)))
+        nnz = int(n.value)
+        rp = np.zeros(self.rows + 1, dtype=np.int64)
+        col = np.zeros(max(nnz, 1), dtype=np.int32)
+        val = np.zeros(max(nnz, 1), dtype=np.float64)
+        self._e._check(self._L.spg_accumulator_copy(self._h, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return Csr(self.rows, self.cols, rp, col[:nnz], val[:nnz])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.spg_accumulator_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
