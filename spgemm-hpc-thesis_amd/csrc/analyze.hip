@@ -43,43 +43,79 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
     s_ub[lane] = 0ull;
     wave_sync();
     unsigned flags = 0;
-    for (int64_t base = E0; base < E1; base += 64) {
-        const int64_t e = base + lane;
-        bool in = e < E1;
-        if (in && e >= op.nnz_a) {  // row_ptr inconsistent with nnz(A)
-            flags |= kErrInternal;
-            in = false;
-        }
-        int64_t len = 0;
-        if (in) {
-            int64_t s, en;
-            brow(op, __builtin_nontemporal_load(&op.acol[e]), s, en, flags);
-            len = en - s;
-            if (len > INT_MAX) {
-                flags |= kErrLongRow;
-                len = 0;
+    // Two chunks of 64 entries per step; both chunks' column loads and B-row
+    // bound gathers are issued (unconditionally, at clamped indices) before
+    // either chunk is reduced, so a wave has four loads in flight per lane.
+    if (op.nnz_a <= 0 && E1 > E0) flags |= kErrInternal;  // row_ptr inconsistent with nnz(A)
+    for (int64_t base = E0; op.nnz_a > 0 && base < E1; base += 128) {
+        int64_t ev[2], sv[2], lv[2];
+        bool inv[2];
+        int32_t kv[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            ev[h] = base + 64 * h + lane;
+            inv[h] = ev[h] < E1;
+            if (inv[h] && ev[h] >= op.nnz_a) {  // row_ptr inconsistent with nnz(A)
+                flags |= kErrInternal;
+                inv[h] = false;
             }
-            __builtin_nontemporal_store(s, &ws.ebst[e]);
-            __builtin_nontemporal_store((int32_t)len, &ws.elen[e]);
+            const int64_t ec = max((int64_t)0, min(ev[h], min(E1, op.nnz_a) - 1));
+            kv[h] = __builtin_nontemporal_load(&op.acol[ec]);
         }
-        // row (lane) holding entry e: largest l <= last with a0_l <= e
-        int lo = 0;
+        int64_t bs[2], be[2];
 #pragma unroll
-        for (int step = 32; step > 0; step >>= 1) {
-            const int mid = lo + step;
-            const int64_t am = __shfl(a0, mid > 63 ? 63 : mid, 64);
-            if (mid <= last && am <= e) lo = mid;
+        for (int h = 0; h < 2; ++h) {
+            const bool ok = kv[h] >= 0 && kv[h] < op.ndir;
+            const int32_t kk = ok ? kv[h] : 0;
+            bs[h] = op.ndir > 0 ? op.bstart[kk] : -1;
+            be[h] = op.ndir > 0 ? op.bend[kk] : -1;
+            if (inv[h] && !ok) flags |= kErrRange;
+            sv[h] = ok ? bs[h] : -1;
         }
-        const int key = in ? lo : 64 + lane;  // inactive lanes never join a segment
-        int64_t v = len;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int64_t vv = __shfl_up(v, d, 64);
-            const int kk = __shfl_up(key, d, 64);
-            if (lane >= d && kk == key) v += vv;
+        for (int h = 0; h < 2; ++h) {
+            int64_t s0 = 0, len = 0;
+            if (inv[h] && kv[h] >= 0 && kv[h] < op.ndir) {
+                if (sv[h] < 0) {
+                    flags |= kErrMissing;
+                } else {
+                    s0 = sv[h];
+                    len = be[h] > s0 ? be[h] - s0 : 0;
+                    if (len > INT_MAX) {
+                        flags |= kErrLongRow;
+                        len = 0;
+                    }
+                }
+            }
+            lv[h] = len;
+            if (inv[h]) {
+                __builtin_nontemporal_store(s0, &ws.ebst[ev[h]]);
+                __builtin_nontemporal_store((int32_t)len, &ws.elen[ev[h]]);
+            }
         }
-        const int next = __shfl_down(key, 1, 64);
-        if (in && (lane == 63 || next != key)) atomicAdd(&s_ub[key], (unsigned long long)v);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t e = ev[h];
+            const bool in = inv[h];
+            // row (lane) holding entry e: largest l <= last with a0_l <= e
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int mid = lo + step;
+                const int64_t am = __shfl(a0, mid > 63 ? 63 : mid, 64);
+                if (mid <= last && am <= e) lo = mid;
+            }
+            const int key = in ? lo : 64 + lane;  // inactive lanes never join a segment
+            int64_t v = lv[h];
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int64_t vv = __shfl_up(v, d, 64);
+                const int kk = __shfl_up(key, d, 64);
+                if (lane >= d && kk == key) v += vv;
+            }
+            const int next = __shfl_down(key, 1, 64);
+            if (in && (lane == 63 || next != key)) atomicAdd(&s_ub[key], (unsigned long long)v);
+        }
     }
     wave_sync();
     const int64_t ub = (int64_t)s_ub[lane];

@@ -30,7 +30,10 @@ constexpr int kTpbLarge = 256;
 // computed by one wave each, kFusedRows consecutive rows per wave, in tiles of
 // (kFusedTpb / 64) * kFusedRows rows (these are exactly the rows of bins W1, W2).
 constexpr int kFusedCap = kCapW2;
-constexpr int kFusedTpb = 256;
+#ifndef SPG_FUSED_TPB
+#define SPG_FUSED_TPB 512
+#endif
+constexpr int kFusedTpb = SPG_FUSED_TPB;
 constexpr int kFusedRows = 2;  // default rows per wave of a tile (1 or 2)
 
 // Error flags raised by kernels (OR-ed into Control::flags).
