@@ -4,9 +4,9 @@
 // on distinct output columns and its share of scalarMultiplicationCount
 // (src/common/spgemm_common.cpp:474-487).  One wave takes a tile of 64
 // consecutive rows and streams the tile's A entries with all 64 lanes
-// (coalesced, balanced whatever the row lengths), finds each entry's row by a
-// 6-step search over the lanes' row starts, and reduces lengths per row with a
-// segmented shuffle scan.  Rows are appended to their bin's list with one
+// (coalesced, balanced whatever the row lengths), finds each entry's row with a
+// prefix maximum over the row starts that fall in its 64-entry chunk, and sums
+// lengths per row with LDS atomics.  Rows are appended to their bin's list with one
 // atomic per bin per wave.  The (start, length) of the B row every A entry
 // references is written out as a coalesced stream, so the later phases never
 // gather B's row offsets again.
@@ -29,9 +29,16 @@ __global__ void k_zero_words(uint32_t *p, size_t n) {
 
 constexpr int kAnalyzeTpb = 256;
 
-// One wave: rows tile*64 .. tile*64+63.
+__global__ void k_dir32(const int64_t *bptr, int64_t n, uint32_t *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (uint32_t)bptr[i];
+}
+
+// One wave: rows tile*64 .. tile*64+63.  DIR32: B row k is
+// [bdir32[k], bdir32[k+1]) (B's own row_ptr, 32-bit copy).
+template <bool DIR32>
 __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace &ws, int64_t tile,
-                                             unsigned long long *s_ub, int64_t *s_part) {
+                                             unsigned long long *s_ub, int *s_mark, int64_t *s_part) {
     const int lane = lane_id();
     const int64_t r = tile * 64 + lane;
     const bool valid = r < op.arows;
@@ -67,8 +74,13 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
         for (int h = 0; h < 2; ++h) {
             const bool ok = kv[h] >= 0 && kv[h] < op.ndir;
             const int32_t kk = ok ? kv[h] : 0;
-            bs[h] = op.ndir > 0 ? op.bstart[kk] : -1;
-            be[h] = op.ndir > 0 ? op.bend[kk] : -1;
+            if (DIR32) {
+                bs[h] = (int64_t)ws.bdir32[kk];
+                be[h] = (int64_t)ws.bdir32[kk + 1];
+            } else {
+                bs[h] = op.ndir > 0 ? op.bstart[kk] : -1;
+                be[h] = op.ndir > 0 ? op.bend[kk] : -1;
+            }
             if (inv[h] && !ok) flags |= kErrRange;
             sv[h] = ok ? bs[h] : -1;
         }
@@ -95,26 +107,19 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int64_t e = ev[h];
-            const bool in = inv[h];
-            // row (lane) holding entry e: largest l <= last with a0_l <= e
-            int lo = 0;
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const int mid = lo + step;
-                const int64_t am = __shfl(a0, mid > 63 ? 63 : mid, 64);
-                if (mid <= last && am <= e) lo = mid;
-            }
-            const int key = in ? lo : 64 + lane;  // inactive lanes never join a segment
-            int64_t v = lv[h];
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int64_t vv = __shfl_up(v, d, 64);
-                const int kk = __shfl_up(key, d, 64);
-                if (lane >= d && kk == key) v += vv;
-            }
-            const int next = __shfl_down(key, 1, 64);
-            if (in && (lane == 63 || next != key)) atomicAdd(&s_ub[key], (unsigned long long)v);
+            // Row (lane) holding entry e = base + 64h + lane: the largest row
+            // l <= last whose first entry a0_l <= e.  Rows that start inside
+            // the chunk mark their start position; a prefix maximum over the
+            // positions, seeded with the last row starting at or before the
+            // chunk, gives every entry's row.
+            const int64_t cb = base + 64 * h;
+            s_mark[lane] = -1;
+            wave_sync();
+            if (lane <= last && a0 > cb && a0 < cb + 64) atomicMax(&s_mark[a0 - cb], lane);
+            const int seed = wave_max_dpp(lane <= last && a0 <= cb ? lane : -1);
+            wave_sync();
+            const int row = max(seed, wave_incl_max_dpp(s_mark[lane]));
+            if (inv[h]) atomicAdd(&s_ub[row], (unsigned long long)lv[h]);
         }
     }
     wave_sync();
@@ -158,15 +163,17 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
     }
 }
 
+template <bool DIR32>
 __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace ws) {
     constexpr int NW = kAnalyzeTpb / 64;
     __shared__ unsigned long long s_ub[NW][64];
+    __shared__ int s_mark[NW][64];
     __shared__ int64_t s_part[NW][kAnalyzeFields];
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
     const int64_t tile = (int64_t)blockIdx.x * NW + w;
     if (lane < kAnalyzeFields) s_part[w][lane] = 0;
-    if (tile * 64 < op.arows) analyze_tile(op, ws, tile, s_ub[w], s_part[w]);  // uniform per wave
+    if (tile * 64 < op.arows) analyze_tile<DIR32>(op, ws, tile, s_ub[w], s_mark[w], s_part[w]);  // uniform per wave
     __syncthreads();
     if (threadIdx.x < kAnalyzeFields) {
         int64_t v = 0;
@@ -204,6 +211,13 @@ void launch_zero_control(Control *ctrl, hipStream_t stream) {
     hipLaunchKernelGGL(k_zero_control, dim3(1), dim3(64), 0, stream, ctrl);
 }
 
+void launch_dir32(const int64_t *bptr, int64_t n, uint32_t *out, hipStream_t stream) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_dir32, dim3((unsigned)blocks), dim3(256), 0, stream, bptr, n, out);
+}
+
 int64_t analyze_blocks(int32_t rows) {
     const int64_t tiles = ((int64_t)rows + 63) / 64;
     return (tiles + kAnalyzeTpb / 64 - 1) / (kAnalyzeTpb / 64);
@@ -212,7 +226,10 @@ int64_t analyze_blocks(int32_t rows) {
 void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream) {
     if (op.arows <= 0) return;
     const int64_t blocks = analyze_blocks(op.arows);
-    hipLaunchKernelGGL(k_analyze, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
+    if (ws.bdir32)
+        hipLaunchKernelGGL(k_analyze<true>, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
+    else
+        hipLaunchKernelGGL(k_analyze<false>, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
     hipLaunchKernelGGL(k_analyze_reduce, dim3(1), dim3(256), 0, stream, ws.part, blocks, ws.ctrl);
 }
 

@@ -155,6 +155,17 @@ __device__ __forceinline__ int wave_max_dpp(int v) {
     return wave_last(v);
 }
 
+// Inclusive prefix maximum over the lanes (lane l gets max of lanes 0..l).
+__device__ __forceinline__ int wave_incl_max_dpp(int v) {
+    v = max(v, dpp_i<0x111, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x112, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x114, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x118, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x142, 0xa>(INT_MIN, v));
+    v = max(v, dpp_i<0x143, 0xc>(INT_MIN, v));
+    return v;
+}
+
 __device__ __forceinline__ int wave_min_dpp(int v) {
     v = min(v, dpp_i<0x111, 0xf>(INT_MAX, v));
     v = min(v, dpp_i<0x112, 0xf>(INT_MAX, v));

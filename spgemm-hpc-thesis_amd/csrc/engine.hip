@@ -92,6 +92,12 @@ spg_status plan_workspace(spg_context *ctx, ProductPlan &p) {
     ws.scan_tmp = ctx->scan_tmp.as<int64_t>();
     ws.ctrl = ctx->ctrl.as<Control>();
     ws.part = ctx->part.as<int64_t>();
+    // B's row_ptr as the directory (bend = bstart + 1) with 32-bit entry
+    // indices: analyze gathers a 4-byte copy of it (half the cache lines).
+    if (p.small_b && p.op.ndir > 0 && p.op.bend == p.op.bstart + 1) {
+        SPG_HIP(ctx->dir32.ensure(((size_t)p.op.ndir + 1) * 4, st), "hipMalloc(B directory)");
+        ws.bdir32 = ctx->dir32.as<uint32_t>();
+    }
     // Large-row path: one column window and one presence bitmap per resident
     // workgroup (kept all-zero between rows); the accumulators are allocated
     // only when large rows exist.
@@ -122,6 +128,7 @@ spg_status plan_workspace(spg_context *ctx, ProductPlan &p) {
 void enqueue_counts(spg_context *ctx, const ProductPlan &p, unsigned bins, bool scan) {
     hipStream_t st = ctx->stream;
     launch_zero_control(p.ws.ctrl, st);
+    if (p.ws.bdir32) launch_dir32(p.op.bstart, (int64_t)p.op.ndir + 1, p.ws.bdir32, st);
     launch_analyze(p.op, p.ws, st);
     launch_symbolic(p.op, p.ws, bins, ctx->cus, st);
     if (scan) launch_scan(p.ws.snnz, ctx->rowptr[0].as<int64_t>(), p.rows, p.ws.scan_tmp, &p.ws.ctrl->structural, st);
@@ -348,7 +355,7 @@ spg_status spg_destroy(spg_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->a_ptr, &ctx->a_col, &ctx->a_val, &ctx->b_ptr, &ctx->b_col, &ctx->b_val,
                       &ctx->dir_start, &ctx->dir_end, &ctx->ebst, &ctx->elen, &ctx->ub, &ctx->bin_rows, &ctx->snnz,
-                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->tile_status, &ctx->tile_distinct, &ctx->part, &ctx->rowptr[0],
+                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->tile_status, &ctx->tile_distinct, &ctx->part, &ctx->dir32, &ctx->rowptr[0],
                       &ctx->rowptr[1], &ctx->ccol[0], &ctx->ccol[1], &ctx->cval[0], &ctx->cval[1]};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t &e : ctx->ev)

@@ -74,7 +74,7 @@ struct spg_context {
     // uploaded operands (host inputs) and the halo directory
     spg::DevBuf a_ptr, a_col, a_val, b_ptr, b_col, b_val, dir_start, dir_end;
     // workspace
-    spg::DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc, tile_status, tile_distinct, part;
+    spg::DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc, tile_status, tile_distinct, part, dir32;
     int path = SPG_PATH_AUTO;
     int fused_rows = spg::kFusedRows;
     int64_t lbits_width = 0;

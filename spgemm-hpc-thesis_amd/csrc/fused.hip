@@ -40,6 +40,18 @@ constexpr uint64_t kStAgg = 1ull << 62;   // tile status: count of the tile only
 constexpr uint64_t kStIncl = 1ull << 63;  // tile status: count of tiles 0..t
 constexpr uint64_t kStVal = kStAgg - 1;
 constexpr int kSpinLimit = 1 << 22;       // never reached unless a tile fails to publish
+#ifndef SPG_LB_PROBE
+#define SPG_LB_PROBE 1
+#endif
+// Wait between probes while the nearest earlier tile is unpublished, in units
+// of 64 clocks.  Polling faster slows the whole product: on the 2^20-row
+// 16-per-row product, 127 (the longest single wait) beat 2, 8 and 32 by
+// 0.2-0.25 ms, and two waits per probe lost it again.
+#ifndef SPG_LB_SLEEP
+#define SPG_LB_SLEEP 127
+#endif
+
+constexpr int kLbProbe = SPG_LB_PROBE;    // look-back loads per lane per probe (64 tiles each)
 
 // A wave's LDS.  The row's entry metadata is dead once its gathers are
 // issued, so it shares storage with the sorted products.
@@ -369,7 +381,7 @@ __device__ __forceinline__ void st_status(uint64_t *p, uint64_t v) {
 
 // Exclusive entry offset of tile t (one wave): publish the tile's count, then
 // sum the counts of earlier tiles back to the nearest published inclusive
-// total, 64 tiles per probe.
+// total, 64 * kLbProbe tiles per probe.
 __device__ int64_t tile_lookback(uint64_t *status, int64_t t, int64_t agg, unsigned &flags) {
     const int lane = lane_id();
     if (lane == 0) st_status(&status[t], (t == 0 ? kStIncl : kStAgg) | (uint64_t)agg);
@@ -377,22 +389,35 @@ __device__ int64_t tile_lookback(uint64_t *status, int64_t t, int64_t agg, unsig
     int64_t excl = 0;
     int64_t p = t - 1;
     for (int spins = 0;;) {
-        const int64_t q = p - lane;
-        const uint64_t s = q >= 0 ? ld_status(&status[q]) : kStIncl;  // before tile 0: total 0
-        const uint64_t incl_m = __ballot((s & kStIncl) != 0);
-        const uint64_t ready_m = __ballot((s & (kStIncl | kStAgg)) != 0);
-        const int fn = ~ready_m ? __ffsll((long long)~ready_m) - 1 : 64;
-        const int fi = incl_m ? __ffsll((long long)incl_m) - 1 : 64;
-        const int64_t v = (int64_t)(s & kStVal);
-        if (fi < fn) {
-            excl += wave_sum(lane <= fi ? v : (int64_t)0);
-            break;
+        // kLbProbe coalesced loads in flight per lane: load j reads the
+        // statuses at distances 64j + lane behind p.
+        uint64_t s[kLbProbe];
+#pragma unroll
+        for (int j = 0; j < kLbProbe; ++j) {
+            const int64_t q = p - lane - 64 * j;
+            s[j] = q >= 0 ? ld_status(&status[q]) : kStIncl;  // before tile 0: total 0
         }
+        // fn: distance of the nearest unpublished tile, fi: of the nearest
+        // inclusive total (the smallest load index j holding one wins).
+        int fn = 64 * kLbProbe, fi = 64 * kLbProbe;
+#pragma unroll
+        for (int j = kLbProbe - 1; j >= 0; --j) {
+            const uint64_t nr_m = __ballot((s[j] & (kStIncl | kStAgg)) == 0);
+            const uint64_t incl_m = __ballot((s[j] & kStIncl) != 0);
+            if (nr_m) fn = 64 * j + __ffsll((long long)nr_m) - 1;
+            if (incl_m) fi = 64 * j + __ffsll((long long)incl_m) - 1;
+        }
+        const int lim = fi < fn ? fi + 1 : fn;  // statuses summed: distances < lim
+        int64_t part = 0;
+#pragma unroll
+        for (int j = 0; j < kLbProbe; ++j)
+            if (64 * j + lane < lim) part += (int64_t)(s[j] & kStVal);
+        if (lim > 0) excl += wave_sum(part);
+        if (fi < fn) break;
         if (fn > 0) {
-            excl += wave_sum(lane < fn ? v : (int64_t)0);
             p -= fn;
         } else {
-            __builtin_amdgcn_s_sleep(2);
+            __builtin_amdgcn_s_sleep(SPG_LB_SLEEP);
             if (++spins > kSpinLimit) {
                 flags |= kErrInternal;
                 break;

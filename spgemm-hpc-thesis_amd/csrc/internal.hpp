@@ -102,6 +102,8 @@ struct RowRec {
 struct Workspace {
     int64_t *ebst;        // [nnz(A)] start of the B row each A entry references
     int32_t *elen;        // [nnz(A)] length of that B row (0 for a bad/missing row)
+    uint32_t *bdir32;     // [ndir+1] B's row offsets in 32 bits when the directory is B's own
+                          // row_ptr and every entry index fits, else null
     int64_t *ub;          // [arows] per-row product count
     RowRec *bin_recs;     // [kNumBins * list_stride] row records per bin
     int64_t list_stride;  // >= arows
@@ -123,6 +125,7 @@ void launch_zero_control(Control *ctrl, hipStream_t stream);
 // Per-row work bound, bins and bin row lists (analyze.hip).
 int64_t analyze_blocks(int32_t rows);
 void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream);
+void launch_dir32(const int64_t *bptr, int64_t n, uint32_t *out, hipStream_t stream);
 // dst = sum of in[0..n) (one workgroup; scan.hip).
 void launch_sum(const int64_t *in, int64_t n, unsigned long long *dst, hipStream_t stream);
 // Bin sets of the launchers: bit b selects bin b.
