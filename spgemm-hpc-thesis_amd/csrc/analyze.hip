@@ -183,17 +183,31 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
     }
 }
 
-__global__ __launch_bounds__(256) void k_analyze_reduce(const int64_t *part, int64_t nblocks, Control *c) {
-    __shared__ int64_t s_ws[4];
-    for (int f = 0; f < kAnalyzeFields; ++f) {
-        int64_t v = 0;
-        for (int64_t i = threadIdx.x; i < nblocks; i += 256) v += part[(size_t)i * kAnalyzeFields + f];
-        v = block_sum<256>(v, s_ws);
-        if (threadIdx.x == 0) {
-            if (f == 0) c->products = (unsigned long long)v;
-            else if (f == 1) c->ub_small = (unsigned long long)v;
-            else c->bin_rows[f - 2] = v;
-        }
+// Every field of every block's partials in one pass: thread i sums blocks
+// i, i + 1024, ... for all fields at once.
+constexpr int kReduceTpb = 1024;
+__global__ __launch_bounds__(kReduceTpb) void k_analyze_reduce(const int64_t *part, int64_t nblocks, Control *c) {
+    __shared__ int64_t s_f[kReduceTpb / 64][kAnalyzeFields];
+    int64_t v[kAnalyzeFields] = {};
+    for (int64_t i = threadIdx.x; i < nblocks; i += kReduceTpb) {
+#pragma unroll
+        for (int f = 0; f < kAnalyzeFields; ++f) v[f] += part[(size_t)i * kAnalyzeFields + f];
+    }
+#pragma unroll
+    for (int f = 0; f < kAnalyzeFields; ++f) v[f] = wave_sum(v[f]);
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int f = 0; f < kAnalyzeFields; ++f) s_f[threadIdx.x >> 6][f] = v[f];
+    }
+    __syncthreads();
+    const int f = threadIdx.x;
+    if (f < kAnalyzeFields) {
+        int64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kReduceTpb / 64; ++w) t += s_f[w][f];
+        if (f == 0) c->products = (unsigned long long)t;
+        else if (f == 1) c->ub_small = (unsigned long long)t;
+        else c->bin_rows[f - 2] = t;
     }
 }
 
@@ -230,7 +244,7 @@ void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream)
         hipLaunchKernelGGL(k_analyze<true>, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
     else
         hipLaunchKernelGGL(k_analyze<false>, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
-    hipLaunchKernelGGL(k_analyze_reduce, dim3(1), dim3(256), 0, stream, ws.part, blocks, ws.ctrl);
+    hipLaunchKernelGGL(k_analyze_reduce, dim3(1), dim3(kReduceTpb), 0, stream, ws.part, blocks, ws.ctrl);
 }
 
 }  // namespace spg

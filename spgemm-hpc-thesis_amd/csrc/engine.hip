@@ -148,7 +148,7 @@ spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h) {
     if (p.fused) {
         p.fo.ntiles = fused_tiles(p.rows, p.fused_rows);
         const size_t nt = (size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1);
-        SPG_HIP(ctx->tile_status.ensure(nt * 8, st), "hipMalloc(tile status)");
+        SPG_HIP(ctx->tile_status.ensure(nt * kStatusStride * 8, st), "hipMalloc(tile status)");
         SPG_HIP(ctx->tile_distinct.ensure(nt * 8, st), "hipMalloc(tile counts)");
         p.fo.rowptr = ctx->rowptr[0].as<int64_t>();
         p.fo.ccol = ctx->ccol[0].as<int32_t>();
@@ -162,7 +162,7 @@ spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h) {
 void enqueue_values(spg_context *ctx, const ProductPlan &p) {
     hipStream_t st = ctx->stream;
     if (p.fused) {
-        launch_zero_bytes(p.fo.status, (size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1) * 8, st);
+        launch_zero_bytes(p.fo.status, (size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1) * kStatusStride * 8, st);
         if (p.rows == 0) launch_zero_bytes(p.fo.rowptr, 8, st);
         launch_fused(p.op, p.ws, p.fo, p.small_b, p.fused_rows, st);
     }

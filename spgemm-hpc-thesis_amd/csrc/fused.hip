@@ -43,15 +43,17 @@ constexpr int kSpinLimit = 1 << 22;       // never reached unless a tile fails t
 #ifndef SPG_LB_PROBE
 #define SPG_LB_PROBE 1
 #endif
-// Wait between probes while the nearest earlier tile is unpublished, in units
-// of 64 clocks.  Polling faster slows the whole product: on the 2^20-row
-// 16-per-row product, 127 (the longest single wait) beat 2, 8 and 32 by
-// 0.2-0.25 ms, and two waits per probe lost it again.
+// Wait between polls of an unpublished earlier tile, in units of 64 clocks.
+// With one status per line (kStatusStride) the wait length hardly matters
+// between 2 and 16; with packed statuses polling contended for one memory
+// channel and only the longest wait (127) kept it in check.
 #ifndef SPG_LB_SLEEP
-#define SPG_LB_SLEEP 127
+#define SPG_LB_SLEEP 8
 #endif
 
-constexpr int kLbProbe = SPG_LB_PROBE;    // look-back loads per lane per probe (64 tiles each)
+// Look-back loads per lane per probe (64 tiles each).  One: every status of a
+// probe is its own line, and wider probes measured slower.
+constexpr int kLbProbe = SPG_LB_PROBE;
 
 // A wave's LDS.  The row's entry metadata is dead once its gathers are
 // issued, so it shares storage with the sorted products.
@@ -79,11 +81,15 @@ struct MetaReg {
 };
 
 // Row results held in registers until the tile's output offset is known:
-// slot j of a lane is sorted position lane + 64*j of the row.
+// slot j of a lane is sorted position lane + 64*j of the row.  Byte j of exnz
+// is the slot's stored flag (bit 7) and the count of stored entries of slot j
+// in lower lanes (bits 0-5); byte j of tot (uniform) is slot j's stored count,
+// so slot j's output position is (stored entries of slots < j) + that count.
 struct RowOut {
     int col[kFItems];
     double val[kFItems];
-    int pos[kFItems];  // output position inside the row, -1 when nothing is stored
+    uint32_t exnz;
+    uint32_t tot;
 };
 
 template <typename T>
@@ -360,12 +366,12 @@ __device__ __forceinline__ int sort_fold(FusedWave<SMALLB> &w, int P, const RowI
     uint32_t before = 0;
 #pragma unroll
     for (int j = 0; j < kFItems; ++j) {
-        const bool nz = (packed_nz >> (8 * j)) & 1u;
-        out.pos[j] = nz ? (int)(before + ((ex >> (8 * j)) & 0xffu)) : -1;
         out.col[j] = cc[j];
         out.val[j] = vv[j];
         before += (tot >> (8 * j)) & 0xffu;
     }
+    out.exnz = ex | (packed_nz << 7);
+    out.tot = tot;
     const uint32_t hs = (uint32_t)wave_last(wave_incl_sum_dpp((int)packed_head));
     distinct = (int)((hs & 0xffu) + ((hs >> 8) & 0xffu) + ((hs >> 16) & 0xffu) + (hs >> 24));
     wave_sync();  // LDS of this wave is reused by its next row
@@ -384,7 +390,7 @@ __device__ __forceinline__ void st_status(uint64_t *p, uint64_t v) {
 // total, 64 * kLbProbe tiles per probe.
 __device__ int64_t tile_lookback(uint64_t *status, int64_t t, int64_t agg, unsigned &flags) {
     const int lane = lane_id();
-    if (lane == 0) st_status(&status[t], (t == 0 ? kStIncl : kStAgg) | (uint64_t)agg);
+    if (lane == 0) st_status(&status[t * kStatusStride], (t == 0 ? kStIncl : kStAgg) | (uint64_t)agg);
     if (t == 0) return 0;
     int64_t excl = 0;
     int64_t p = t - 1;
@@ -395,7 +401,7 @@ __device__ int64_t tile_lookback(uint64_t *status, int64_t t, int64_t agg, unsig
 #pragma unroll
         for (int j = 0; j < kLbProbe; ++j) {
             const int64_t q = p - lane - 64 * j;
-            s[j] = q >= 0 ? ld_status(&status[q]) : kStIncl;  // before tile 0: total 0
+            s[j] = q >= 0 ? ld_status(&status[q * kStatusStride]) : kStIncl;  // before tile 0: total 0
         }
         // fn: distance of the nearest unpublished tile, fi: of the nearest
         // inclusive total (the smallest load index j holding one wins).
@@ -417,14 +423,21 @@ __device__ int64_t tile_lookback(uint64_t *status, int64_t t, int64_t agg, unsig
         if (fn > 0) {
             p -= fn;
         } else {
-            __builtin_amdgcn_s_sleep(SPG_LB_SLEEP);
-            if (++spins > kSpinLimit) {
+            // Tile p is unpublished: wait on its status alone (every lane
+            // loads the same line) before the next full probe.
+            bool ready = false;
+            while (!ready && ++spins <= kSpinLimit) {
+                __builtin_amdgcn_s_sleep(SPG_LB_SLEEP);
+                const uint64_t sp = ld_status(&status[p * kStatusStride]);
+                ready = (__builtin_amdgcn_readfirstlane((int)(uint32_t)(sp >> 32)) & (int)((kStIncl | kStAgg) >> 32)) != 0;
+            }
+            if (!ready) {
                 flags |= kErrInternal;
                 break;
             }
         }
     }
-    if (lane == 0) st_status(&status[t], kStIncl | (uint64_t)(excl + agg));
+    if (lane == 0) st_status(&status[t * kStatusStride], kStIncl | (uint64_t)(excl + agg));
     return excl;
 }
 
@@ -433,8 +446,17 @@ __device__ __forceinline__ int64_t lane64(int64_t v, int k) {
                      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k));
 }
 
+#ifndef SPG_FUSED_WPE
+#define SPG_FUSED_WPE 0
+#endif
+#if SPG_FUSED_WPE
+#define SPG_FUSED_ATTR __attribute__((amdgpu_waves_per_eu(SPG_FUSED_WPE)))
+#else
+#define SPG_FUSED_ATTR
+#endif
+
 template <bool SMALLB, int R>
-__global__ __launch_bounds__(kFusedTpb) void k_fused(Operands op, Workspace ws, FusedOut fo) {
+__global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op, Workspace ws, FusedOut fo) {
     constexpr int NW = kFusedTpb / 64;
     __shared__ FusedWave<SMALLB> s_w[NW];
     __shared__ int64_t s_wtot[NW];
@@ -468,8 +490,8 @@ __global__ __launch_bounds__(kFusedTpb) void k_fused(Operands op, Workspace ws, 
         fused[k] = r0 + k < op.arows && ub > 0 && ub <= kFusedCap && na <= kFusedCap;
         joint &= fused[k] || ub == 0 || r0 + k >= op.arows;
         cnt[k] = 0;
-#pragma unroll
-        for (int j = 0; j < kFItems; ++j) out[k].pos[j] = -1;
+        out[k].exnz = 0;
+        out[k].tot = 0;
         if (!fused[k] && r0 + k < op.arows && ub > 0) wave_total += lane64(snz, k);  // reserved for the numeric kernels
     }
     const int64_t E = lane64(ap, R) - lane64(ap, 0);
@@ -563,15 +585,18 @@ __global__ __launch_bounds__(kFusedTpb) void k_fused(Operands op, Workspace ws, 
         if (fused[k]) {
             int32_t *cc = fo.ccol + rp;
             double *cv = fo.cval + rp;
+            uint32_t before = 0;
 #pragma unroll
             for (int j = 0; j < kFItems; ++j) {
-                const int o = out[k].pos[j];
-                if (o >= 0) {
+                const uint32_t b = (out[k].exnz >> (8 * j)) & 0xffu;
+                if (b & 0x80u) {
+                    const uint32_t o = before + (b & 0x7fu);
                     __builtin_nontemporal_store(
-                        out[k].col[j], reinterpret_cast<int32_t *>(reinterpret_cast<char *>(cc) + (uint32_t)o * 4u));
+                        out[k].col[j], reinterpret_cast<int32_t *>(reinterpret_cast<char *>(cc) + o * 4u));
                     __builtin_nontemporal_store(
-                        out[k].val[j], reinterpret_cast<double *>(reinterpret_cast<char *>(cv) + (uint32_t)o * 8u));
+                        out[k].val[j], reinterpret_cast<double *>(reinterpret_cast<char *>(cv) + o * 8u));
                 }
+                before += (out[k].tot >> (8 * j)) & 0xffu;
             }
             rp += cnt[k];
         } else if (r0 + k < op.arows && lane64(ubl, k) > 0) {

@@ -64,12 +64,25 @@ __global__ __launch_bounds__(kScanTpb) void k_scan_down(const int64_t *in, int64
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *grand;
 }
 
+// Adds the sum of in[0, n) to *dst: eight independent loads in flight per
+// thread, one atomic per workgroup.
+constexpr int kSumUnroll = 8;
 __global__ __launch_bounds__(kScanTpb) void k_sum(const int64_t *in, int64_t n, unsigned long long *dst) {
     __shared__ int64_t s_ws[kScanTpb / 64];
+    const int64_t stride = (int64_t)gridDim.x * kScanTpb;
+    int64_t acc[kSumUnroll] = {};
+    for (int64_t i0 = (int64_t)blockIdx.x * kScanTpb + threadIdx.x; i0 < n; i0 += stride * kSumUnroll) {
+#pragma unroll
+        for (int u = 0; u < kSumUnroll; ++u) {
+            const int64_t i = i0 + u * stride;
+            acc[u] += i < n ? in[i] : 0;
+        }
+    }
     int64_t v = 0;
-    for (int64_t i = threadIdx.x; i < n; i += kScanTpb) v += in[i];
+#pragma unroll
+    for (int u = 0; u < kSumUnroll; ++u) v += acc[u];
     v = block_sum<kScanTpb>(v, s_ws);
-    if (threadIdx.x == 0) *dst = (unsigned long long)v;
+    if (threadIdx.x == 0 && v != 0) atomicAdd(dst, (unsigned long long)v);
 }
 
 // One wave per row (only when exact zeros were dropped).
@@ -107,7 +120,10 @@ void launch_scan(const int64_t *in, int64_t *out, int64_t n, int64_t *tmp, unsig
 }
 
 void launch_sum(const int64_t *in, int64_t n, unsigned long long *dst, hipStream_t stream) {
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(kScanTpb), 0, stream, in, n, dst);
+    if (n <= 0) return;
+    int64_t blocks = (n + (int64_t)kScanTpb * kSumUnroll - 1) / ((int64_t)kScanTpb * kSumUnroll);
+    if (blocks > 256) blocks = 256;
+    hipLaunchKernelGGL(k_sum, dim3((unsigned)blocks), dim3(kScanTpb), 0, stream, in, n, dst);
 }
 
 void launch_compact(int32_t rows, const int64_t *rowptr_in, const int32_t *col_in, const double *val_in,
