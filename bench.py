@@ -90,16 +90,17 @@ WORKLOADS = {
 
 
 def traffic_from_profiles(workload):
-    """HBM bytes per launch of the dominant kernel, from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs (profiles/traffic_<workload>.json)."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    """HBM bytes per launch of the dominant kernel, from the calibrated
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+    profiles/traffic_<workload>.md (its line `hbm_bytes_per_launch: N`)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
     if not os.path.exists(path):
         return None
-    try:
-        with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
-    except Exception:
-        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return int(line.split(":", 1)[1].strip().replace(",", ""))
+    return None
 
 
 def cpu_baseline(a, b, products_total, target_s=15.0):
@@ -519,8 +520,10 @@ def main():
             exact &= rows_equal(c, 0, ref, pos[r])
 
     single = bool(st["single_pass"])
-    if single and not chunked:
-        kernel = "k_fused<small B>"
+    big_rows = int(sum(st["rows_per_bin"][3:]))  # bins W3, Block, Large run after the single-pass kernel
+    if single and not chunked and big_rows == 0:
+        # the timed values phase is the single-pass kernel alone (named as rocprofv3 names it)
+        kernel = f"k_fused<{'true' if b.nnz < (1 << 29) else 'false'}, 2>"
         kernel_bytes = 32 * a.rows + 20 * a.nnz + 12 * P + 12 * nnz
     else:
         kernel = "numeric phase (single-pass and big-row kernels)" if single else "numeric phase"
