@@ -194,6 +194,7 @@ spg_status spg_accumulator_add(spg_accumulator *acc, const spg_csr_view *A, cons
     plan.rows = (int32_t)R;
     plan.cols = acc->cols;
     plan.small_b = bn < (int64_t(1) << 29);
+    plan.b_span = bn;
     spg_context *own = acc->own;
     own->path = ctx->path;
     if ((s = run_product(own, plan)) != SPG_OK) return set_err(ctx, s, own->err);

@@ -157,6 +157,7 @@ void set_operands(spg_batch *b) {
     b->plan.rows = (int32_t)b->total_rows;
     b->plan.cols = b->max_cols;
     b->plan.small_b = b->b_nnz < (int64_t(1) << 29);
+    b->plan.b_span = b->b_nnz;
 }
 
 spg_status stack_b(spg_context *ctx, spg_batch *b, int32_t n, const spg_csr_view *B, std::vector<int64_t> &b_row0) {

@@ -94,7 +94,7 @@ spg_status plan_workspace(spg_context *ctx, ProductPlan &p) {
     ws.part = ctx->part.as<int64_t>();
     // B's row_ptr as the directory (bend = bstart + 1) with 32-bit entry
     // indices: analyze gathers a 4-byte copy of it (half the cache lines).
-    if (p.small_b && p.op.ndir > 0 && p.op.bend == p.op.bstart + 1) {
+    if (p.b_span < (int64_t(1) << 32) && p.op.ndir > 0 && p.op.bend == p.op.bstart + 1) {
         SPG_HIP(ctx->dir32.ensure(((size_t)p.op.ndir + 1) * 4, st), "hipMalloc(B directory)");
         ws.bdir32 = ctx->dir32.as<uint32_t>();
     }
@@ -501,6 +501,7 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
     plan.rows = op.arows;
     plan.cols = B->cols;
     plan.small_b = small_b;
+    plan.b_span = b_span;
     if ((s = run_product(ctx, plan)) != SPG_OK) return s;
     *nnz_c = ctx->res_nnz;
     return SPG_OK;

@@ -51,7 +51,8 @@ struct ProductPlan {
     Operands op{};
     Workspace ws{};
     int32_t rows = 0, cols = 0;
-    bool small_b = false;
+    bool small_b = false;   // every B entry's byte offset fits 32 bits (entries < 2^29)
+    int64_t b_span = 0;     // B entries reachable through the directory
     bool fused = false;  // single-pass kernel for bins W1, W2
     int fused_rows = kFusedRows;
     int bin_count[8] = {};
