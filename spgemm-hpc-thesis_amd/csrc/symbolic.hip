@@ -8,8 +8,10 @@
 // of a B row are contiguous, and all of a row's column loads are in flight
 // before the first insertion.  Insertions then run in lockstep probe rounds
 // with every register slot's compare-and-swap issued back to back.
-// Large rows set bits in a per-workgroup column bitmap in HBM (window by window)
-// and count the bits they newly set.
+// Rows whose columns span fewer than 32 x the table's words count with a
+// bitmap over their column range in the table's storage instead: one LDS OR
+// per product, no probing.  Large rows set bits in a per-workgroup column
+// bitmap in HBM (window by window) and count the bits they newly set.
 #include "device_common.hpp"
 
 namespace spg {
@@ -61,6 +63,49 @@ __device__ __forceinline__ void hash_step(const Operands &op, const Workspace &w
         for (int j = 0; j < ITEMS; ++j) idx[j] = pend[j] ? sh.bst[e[j]] + t[j] : 0;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) icol[j] = op.bcol[idx[j]];
+        int mn = INT_MAX, mx = INT_MIN;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            mn = pend[j] ? min(mn, icol[j]) : mn;
+            mx = pend[j] ? max(mx, icol[j]) : mx;
+        }
+        block_minmax<TPB>(mn, mx, sh.wsi);
+        int fresh = 0;
+        if ((int64_t)mx - (int64_t)mn < (int64_t)TMAX * 32) {
+            // The row's columns fit a bitmap over [mn, mx] in the table's
+            // storage: one OR per product, and a product counts when it sets
+            // its bit.  Touched words are zeroed first and given back to the
+            // table's empty value after.
+            uint32_t wd[ITEMS], bit[ITEMS];
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                const uint32_t rel = pend[j] ? (uint32_t)(icol[j] - mn) : 0u;
+                wd[j] = rel >> 5;
+                bit[j] = 1u << (rel & 31u);
+            }
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j)
+                if (pend[j]) sh.tab[wd[j]] = 0;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                if (pend[j]) {
+                    const uint32_t old = atomicOr(reinterpret_cast<uint32_t *>(&sh.tab[wd[j]]), bit[j]);
+                    fresh += (old & bit[j]) ? 0 : 1;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j)
+                if (pend[j]) sh.tab[wd[j]] = kEmptyKey;
+            const int total = block_sum<TPB>(fresh, sh.wsi);
+            if (tid == 0) ws.snnz[cur.row] = total;
+            sym += total;
+            __syncthreads();
+            cur = nxt;
+            i += G;
+            return;
+        }
         int T = 64;
         while (T < FILL * P && T < TMAX) T <<= 1;
         T = uni(T);
@@ -72,7 +117,6 @@ __device__ __forceinline__ void hash_step(const Operands &op, const Workspace &w
         // its compare-and-swap at this thread's private dummy word with a
         // compare value that never matches.  T >= 2P > distinct columns, so
         // every item settles within T rounds.
-        int fresh = 0;
         for (int round = 0; round < T; ++round) {
             int32_t old[ITEMS];
 #pragma unroll
@@ -222,8 +266,8 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
     if (op.arows <= 0) return;
     if (bins & (1u << kBinW1)) launch_hash<64, kCapW1>(op, ws, kBinW1, cus, stream);
     if (bins & (1u << kBinW2)) launch_hash<64, kCapW2>(op, ws, kBinW2, cus, stream);
-    if (bins & (1u << kBinW3)) launch_hash<64, kCapW3>(op, ws, kBinW3, cus, stream);
-    if (bins & (1u << kBinBlock)) launch_hash<kTpbBlock, kCapBlock>(op, ws, kBinBlock, cus, stream);
+    if (bins & (1u << kBinW3)) launch_hash<kSymTpbW3, kCapW3>(op, ws, kBinW3, cus, stream);
+    if (bins & (1u << kBinBlock)) launch_hash<kSymTpbBlock, kCapBlock>(op, ws, kBinBlock, cus, stream);
     if ((bins & (1u << kBinLarge)) && ws.lblocks > 0)
         hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge>), dim3(ws.lblocks), dim3(kTpbLarge), 0, stream, op,
                            ws);
