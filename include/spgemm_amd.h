@@ -17,6 +17,7 @@
 #ifndef SPGEMM_AMD_H
 #define SPGEMM_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -274,6 +275,76 @@ spg_status spg_matrix_view(const spg_matrix *m, spg_csr_view *out);
 spg_status spg_matrix_free(spg_matrix *m);
 /* Last error of the host matrix helpers (thread-local). */
 const char *spg_matrix_last_error(void);
+
+/* ---- validation of a product and its reference files --------------------
+ * Host code, multithreaded over rows (threads <= 0: all hardware threads);
+ * every decision and diagnostic equals the serial traversal's.  Views must be
+ * SPG_HOST.  Errors: spg_validation_last_error (thread-local). */
+const char *spg_validation_last_error(void);
+
+/* One reported coordinate (the first eight in row-major order whose support
+ * differs or that are rejected), as printed by validation_difference lines. */
+typedef struct spg_full_example {
+    int32_t row, column;
+    int32_t accepted, support_mismatch;
+    int64_t contributions;
+    double actual, serial_expected;
+    long double expected, abs_error, error_bound, error_ratio;
+} spg_full_example;
+
+typedef struct spg_full_report {
+    int32_t environment_ok; /* binary64 + extended, round to nearest, gradual underflow */
+    int32_t invalid_matrix; /* an operand or C failed validNumericCsr, or shapes differ */
+    int32_t passed;         /* no rejected coordinate (never set without both checks above) */
+    int32_t reference_digits;
+    int32_t worst_row, worst_column; /* -1: no coordinate with a positive ratio */
+    int32_t n_examples, reserved_;
+    int64_t rows, reference_nnz, row_nnz_mismatches, support_mismatches, rejected_coordinates;
+    double max_abs_error;
+    long double max_reference_error, max_error_ratio, worst_actual, worst_reference, worst_bound;
+    spg_full_example examples[8];
+} spg_full_report;
+
+/* Full validation, criterion componentwise_dot_bound_v1: replaces
+ * validateFullStreaming (src/common/spgemm_common.cpp:720-851).  Expected rows
+ * come from the .sfull file `sfull_path` named by `key` (64 hex digits), or,
+ * with sfull_path NULL, are rebuilt from A and B (buildFullRow, :694-717). */
+spg_status spg_validate_full(const spg_csr_view *A, const spg_csr_view *B, const spg_csr_view *C,
+                             const char *sfull_path, const char *key, int threads, spg_full_report *rep);
+/* The reference's text lines for a report (validation_* key=value); *needed
+ * receives the size including the terminating zero. */
+spg_status spg_full_report_text(const spg_full_report *rep, char *buf, size_t cap, size_t *needed);
+
+/* Partial validation, criterion row_nnz_exact_v1: replaces
+ * buildPartialReference / comparePartialReference (:541-571).  Counts come from
+ * the .spref file `spref_path` (named by `key`) or, with NULL, are rebuilt. */
+spg_status spg_partial_counts(const spg_csr_view *A, const spg_csr_view *B, int threads, int64_t *row_nnz,
+                              int32_t *finite);
+spg_status spg_validate_partial(const spg_csr_view *A, const spg_csr_view *B, const spg_csr_view *C,
+                                const char *spref_path, const char *key, int threads, int32_t *passed,
+                                int64_t *row_nnz_mismatches);
+
+/* .spref files (encode/decodePartialReference, src/common/partial_reference.cpp:202-245). */
+spg_status spg_spref_write(const char *path, const char *key, int32_t rows, int32_t cols, const int64_t *row_nnz,
+                           int32_t finite);
+spg_status spg_spref_read(const char *path, const char *key, int32_t rows, int32_t cols, int64_t *row_nnz,
+                          int32_t *finite);
+
+/* .sfull files (src/common/full_reference.cpp:147-235): written from A and B
+ * (prepareFullReference), read row by row (FullReferenceReader); finish checks
+ * the SHA-256 trailer and must succeed before a PASS is reported. */
+typedef struct spg_sfull spg_sfull;
+spg_status spg_sfull_write(const spg_csr_view *A, const spg_csr_view *B, const char *path, const char *key,
+                           int threads);
+spg_status spg_sfull_open(const char *path, const char *key, int32_t rows, int32_t cols, spg_sfull **out);
+spg_status spg_sfull_next_row(spg_sfull *s, int64_t *ncells);
+spg_status spg_sfull_row_cells(const spg_sfull *s, int32_t *column, double *serial, long double *reference,
+                               long double *bound, int64_t *terms);
+spg_status spg_sfull_finish(spg_sfull *s);
+void spg_sfull_close(spg_sfull *s);
+
+/* SHA-256 of a byte string as 64 hex digits (the files' checksums). */
+spg_status spg_sha256_hex(const void *data, size_t n, char out[65]);
 
 #ifdef __cplusplus
 }

@@ -32,10 +32,10 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wextra", f"-I{INCLUDE}"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-Wextra", f"-I{INCLUDE}"]
 
 HIP_SOURCES = ["analyze.hip", "symbolic.hip", "numeric.hip", "fused.hip", "scan.hip", "engine.hip", "batch.hip", "accumulate.hip"]
-CXX_SOURCES = ["host_matrix.cpp", "server.cpp"]
+CXX_SOURCES = ["host_matrix.cpp", "server.cpp", "validate.cpp"]
 
 
 def _run(cmd, verbose):

@@ -96,6 +96,22 @@ int main(int argc, char **argv) {
         }
         ok &= throws<std::runtime_error>([&] { readMatrixMarket("/nonexistent/file.mtx"); }, "mm open");
 
+        // validateProduct: the device product passes Full and Partial; a
+        // changed value fails Full.
+        {
+            const ValidationResult full = validateProduct(a, b, c, ValidationMode::Full, "", "", 4);
+            const ValidationResult part = validateProduct(a, b, c, ValidationMode::Partial, "", "", 4);
+            bool vok = full.passed && part.passed && full.rejectedCoordinates == 0;
+            if (!c.values.empty()) {
+                CsrMatrix bad = c;
+                bad.values[0] = bad.values[0] * 2 + 1;
+                vok &= !validateProduct(a, b, bad, ValidationMode::Full).passed;
+            }
+            if (!vok) std::fprintf(stderr, "validation of the device product failed:\n%s", full.text.c_str());
+            ok &= vok;
+            std::printf("validate %s\n", vok ? "ok" : "FAILED");
+        }
+
         // LocalAccumulator: A's columns split into two contiguous ranges.  When
         // every row of A lists its columns in ascending order, the staged sums
         // follow the same serial order as the direct product, so finish()

@@ -19,6 +19,7 @@
 #include "spgemm_amd.h"
 
 #include <cstdint>
+#include <limits>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -205,6 +206,55 @@ inline CsrMatrix readMatrixMarket(const std::string &path) {
     c.values.assign(v.values, v.values + v.nnz);
     spg_matrix_free(m);
     return c;
+}
+
+// validateProduct (src/common/spgemm_common.cpp:865-904), multithreaded on
+// the host.  The expected rows come from a reference file (`referenceFile`,
+// .spref for Partial, .sfull for Full, named by `key`) or are rebuilt from A
+// and B when it is empty.  `text` holds the validation_* lines of Full mode.
+enum class ValidationMode { None, Partial, Full };
+struct ValidationResult {
+    ValidationMode mode = ValidationMode::None;
+    bool passed = true;
+    double maxAbsoluteError = 0;
+    std::int64_t rowNnzMismatches = 0;
+    long double maxErrorRatio = 0;
+    std::int64_t rejectedCoordinates = 0;
+    int referenceDigits = 0;
+    std::string text;
+};
+
+inline ValidationResult validateProduct(const CsrMatrix &A, const CsrMatrix &B, const CsrMatrix &C,
+                                        ValidationMode mode, const std::string &referenceFile = "",
+                                        const std::string &key = "", int threads = 0) {
+    ValidationResult r;
+    r.mode = mode;
+    if (mode == ValidationMode::None) return r;
+    const spg_csr_view a = detail::view(A), b = detail::view(B), c = detail::view(C);
+    const char *file = referenceFile.empty() ? nullptr : referenceFile.c_str();
+    if (mode == ValidationMode::Partial) {
+        std::int32_t passed = 0;
+        const spg_status s = spg_validate_partial(&a, &b, &c, file, key.c_str(), threads, &passed, &r.rowNnzMismatches);
+        if (s != SPG_OK) detail::raise(s, spg_validation_last_error());
+        r.passed = passed != 0;
+        return r;
+    }
+    spg_full_report rep;
+    const spg_status s = spg_validate_full(&a, &b, &c, file, key.c_str(), threads, &rep);
+    if (s != SPG_OK) detail::raise(s, spg_validation_last_error());
+    std::size_t need = 0;
+    spg_full_report_text(&rep, nullptr, 0, &need);
+    r.text.assign(need, '\0');
+    spg_full_report_text(&rep, &r.text[0], need, nullptr);
+    r.text.resize(need ? need - 1 : 0);
+    r.passed = rep.environment_ok && !rep.invalid_matrix && rep.passed;
+    r.maxAbsoluteError = rep.environment_ok && !rep.invalid_matrix ? rep.max_abs_error
+                                                                   : std::numeric_limits<double>::infinity();
+    r.rowNnzMismatches = rep.row_nnz_mismatches;
+    r.maxErrorRatio = rep.max_error_ratio;
+    r.rejectedCoordinates = rep.rejected_coordinates;
+    r.referenceDigits = rep.reference_digits;
+    return r;
 }
 
 }  // namespace spgemm_amd

@@ -39,6 +39,9 @@ EXPORTED = [
     "spg_server_create", "spg_server_submit", "spg_server_register", "spg_server_unregister",
     "spg_server_submit_resident", "spg_server_wait", "spg_server_result", "spg_server_release", "spg_server_stats",
     "spg_server_destroy",
+    "spg_validation_last_error", "spg_validate_full", "spg_full_report_text", "spg_partial_counts",
+    "spg_validate_partial", "spg_spref_write", "spg_spref_read", "spg_sfull_write", "spg_sfull_open",
+    "spg_sfull_next_row", "spg_sfull_row_cells", "spg_sfull_finish", "spg_sfull_close", "spg_sha256_hex",
 ]
 
 PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE = 0, 1, 2
@@ -84,6 +87,25 @@ class Stats(C.Structure):
                 "rows_per_bin": list(self.rows_per_bin)[: self.bins], "compacted": bool(self.compacted),
                 "single_pass": bool(self.single_pass),
                 "ms_total": self.ms_total, "ms_symbolic": self.ms_symbolic, "ms_numeric": self.ms_numeric}
+
+
+class FullExample(C.Structure):
+    _fields_ = [("row", C.c_int32), ("column", C.c_int32), ("accepted", C.c_int32),
+                ("support_mismatch", C.c_int32), ("contributions", C.c_int64), ("actual", C.c_double),
+                ("serial_expected", C.c_double), ("expected", C.c_longdouble), ("abs_error", C.c_longdouble),
+                ("error_bound", C.c_longdouble), ("error_ratio", C.c_longdouble)]
+
+
+class FullReport(C.Structure):
+    _fields_ = [("environment_ok", C.c_int32), ("invalid_matrix", C.c_int32), ("passed", C.c_int32),
+                ("reference_digits", C.c_int32), ("worst_row", C.c_int32), ("worst_column", C.c_int32),
+                ("n_examples", C.c_int32), ("reserved_", C.c_int32),
+                ("rows", C.c_int64), ("reference_nnz", C.c_int64), ("row_nnz_mismatches", C.c_int64),
+                ("support_mismatches", C.c_int64), ("rejected_coordinates", C.c_int64),
+                ("max_abs_error", C.c_double), ("max_reference_error", C.c_longdouble),
+                ("max_error_ratio", C.c_longdouble), ("worst_actual", C.c_longdouble),
+                ("worst_reference", C.c_longdouble), ("worst_bound", C.c_longdouble),
+                ("examples", FullExample * 8)]
 
 
 class SpgError(RuntimeError):
@@ -156,6 +178,22 @@ def lib():
         L.spg_server_destroy.argtypes = [vp]
         L.spg_multiply_rows.argtypes = [vp, P(CsrView), P(CsrView), C.c_int32, P(HaloView), C.c_int32, C.c_int32,
                                         P(C.c_int64)]
+        cp = C.c_char_p
+        L.spg_validation_last_error.restype = cp
+        L.spg_validate_full.argtypes = [P(CsrView), P(CsrView), P(CsrView), cp, cp, C.c_int, P(FullReport)]
+        L.spg_full_report_text.argtypes = [P(FullReport), vp, C.c_size_t, P(C.c_size_t)]
+        L.spg_partial_counts.argtypes = [P(CsrView), P(CsrView), C.c_int, vp, P(C.c_int32)]
+        L.spg_validate_partial.argtypes = [P(CsrView), P(CsrView), P(CsrView), cp, cp, C.c_int, P(C.c_int32),
+                                           P(C.c_int64)]
+        L.spg_spref_write.argtypes = [cp, cp, C.c_int32, C.c_int32, vp, C.c_int32]
+        L.spg_spref_read.argtypes = [cp, cp, C.c_int32, C.c_int32, vp, P(C.c_int32)]
+        L.spg_sfull_write.argtypes = [P(CsrView), P(CsrView), cp, cp, C.c_int]
+        L.spg_sfull_open.argtypes = [cp, cp, C.c_int32, C.c_int32, P(vp)]
+        L.spg_sfull_next_row.argtypes = [vp, P(C.c_int64)]
+        L.spg_sfull_row_cells.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.spg_sfull_finish.argtypes = [vp]
+        L.spg_sfull_close.argtypes = [vp]
+        L.spg_sha256_hex.argtypes = [vp, C.c_size_t, C.c_char_p]
         _lib = L
     return _lib
 
@@ -556,3 +594,111 @@ class Accumulator:
             self.close()
         except Exception:
             pass
+
+
+# ---- validation (host code; no device needed) ------------------------------
+
+def _vcheck(s: int):
+    if s != 0:
+        raise SpgError(s, lib().spg_validation_last_error().decode(errors="replace"))
+
+
+def _enc(x):
+    return None if x is None else str(x).encode()
+
+
+def sha256_hex(data: bytes) -> str:
+    out = C.create_string_buffer(65)
+    buf = C.create_string_buffer(bytes(data), len(data))
+    _vcheck(lib().spg_sha256_hex(buf, len(data), out))
+    return out.value.decode()
+
+
+def validate_full(a: Csr, b: Csr, c: Csr, sfull: str | None = None, key: str | None = None,
+                  threads: int = 0) -> dict:
+    """Full validation (componentwise_dot_bound_v1); expected rows from `sfull`
+    (a .sfull file named by `key`) or rebuilt from a and b."""
+    rep = FullReport()
+    va, vb, vc = host_view(a), host_view(b), host_view(c)
+    _vcheck(lib().spg_validate_full(C.byref(va), C.byref(vb), C.byref(vc), _enc(sfull), _enc(key), threads,
+                                    C.byref(rep)))
+    need = C.c_size_t()
+    lib().spg_full_report_text(C.byref(rep), None, 0, C.byref(need))
+    buf = C.create_string_buffer(need.value)
+    lib().spg_full_report_text(C.byref(rep), buf, need.value, None)
+    out = {f: getattr(rep, f) for f, _ in FullReport._fields_ if f not in ("examples", "reserved_")}
+    out["examples"] = [{f: getattr(rep.examples[i], f) for f, _ in FullExample._fields_}
+                       for i in range(rep.n_examples)]
+    out["text"] = buf.value.decode()
+    return out
+
+
+def partial_counts(a: Csr, b: Csr, threads: int = 0):
+    counts = np.zeros(a.rows, np.int64)
+    fin = C.c_int32()
+    va, vb = host_view(a), host_view(b)
+    _vcheck(lib().spg_partial_counts(C.byref(va), C.byref(vb), threads, counts.ctypes.data, C.byref(fin)))
+    return counts, bool(fin.value)
+
+
+def validate_partial(a: Csr, b: Csr, c: Csr, spref: str | None = None, key: str | None = None,
+                     threads: int = 0):
+    passed, mism = C.c_int32(), C.c_int64()
+    va, vb, vc = host_view(a), host_view(b), host_view(c)
+    _vcheck(lib().spg_validate_partial(C.byref(va), C.byref(vb), C.byref(vc), _enc(spref), _enc(key), threads,
+                                       C.byref(passed), C.byref(mism)))
+    return bool(passed.value), int(mism.value)
+
+
+def spref_write(path: str, key: str, rows: int, cols: int, counts, finite: bool):
+    counts = np.ascontiguousarray(counts, dtype=np.int64)
+    _vcheck(lib().spg_spref_write(path.encode(), key.encode(), rows, cols, counts.ctypes.data if rows else None,
+                                  1 if finite else 0))
+
+
+def spref_read(path: str, key: str, rows: int, cols: int):
+    counts = np.zeros(rows, np.int64)
+    fin = C.c_int32()
+    _vcheck(lib().spg_spref_read(path.encode(), key.encode(), rows, cols, counts.ctypes.data if rows else None,
+                                 C.byref(fin)))
+    return counts, bool(fin.value)
+
+
+def sfull_write(a: Csr, b: Csr, path: str, key: str, threads: int = 0):
+    va, vb = host_view(a), host_view(b)
+    _vcheck(lib().spg_sfull_write(C.byref(va), C.byref(vb), path.encode(), key.encode(), threads))
+
+
+class SfullReader:
+    """Row-by-row reader of a .sfull file; finish() checks the SHA-256 trailer."""
+
+    def __init__(self, path: str, key: str, rows: int, cols: int):
+        self._h = C.c_void_p()
+        _vcheck(lib().spg_sfull_open(path.encode(), key.encode(), rows, cols, C.byref(self._h)))
+
+    def row(self) -> dict:
+        n = C.c_int64()
+        _vcheck(lib().spg_sfull_next_row(self._h, C.byref(n)))
+        k = int(n.value)
+        col = np.zeros(k, np.int32)
+        serial = np.zeros(k, np.float64)
+        ref = np.zeros(k, np.longdouble)
+        bound = np.zeros(k, np.longdouble)
+        terms = np.zeros(k, np.int64)
+        _vcheck(lib().spg_sfull_row_cells(self._h, col.ctypes.data, serial.ctypes.data, ref.ctypes.data,
+                                          bound.ctypes.data, terms.ctypes.data))
+        return {"column": col, "serial": serial, "reference": ref, "bound": bound, "terms": terms}
+
+    def finish(self):
+        _vcheck(lib().spg_sfull_finish(self._h))
+
+    def close(self):
+        if self._h:
+            lib().spg_sfull_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

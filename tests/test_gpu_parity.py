@@ -296,7 +296,7 @@ def test_cpp_shim_matches_golden(golden, tmp_path):
         r = subprocess.run([SHIM, cases.data_path(pa), cases.data_path(pb), str(out)],
                            capture_output=True, text=True, timeout=60)
         assert r.returncode == 0, r.stdout + r.stderr
-        assert "shim ok" in r.stdout
+        assert "shim ok" in r.stdout and "validate ok" in r.stdout
         if name == "cage8_AA":
             assert "accumulate ok" in r.stdout
         c = O.read_csr(str(out))
