@@ -276,6 +276,27 @@ spg_status spg_matrix_free(spg_matrix *m);
 /* Last error of the host matrix helpers (thread-local). */
 const char *spg_matrix_last_error(void);
 
+/* ---- halo rows in HBM -------------------------------------------------------
+ * spg_multiply_begin / spg_multiply_rows also take a device view of B_local
+ * together with a device halo view (both SPG_DEVICE): the global-row
+ * directory (bRowForColumn, src/baselines/spgemm_exchange.cpp:75-87) is then
+ * built on the device, so rows received straight into HBM never cross PCIe.
+ * A library-owned halo comes from spg_halo_upload (host rows) or
+ * spg_halo_pack (the sender's side). */
+typedef struct spg_device_halo spg_device_halo;
+spg_status spg_halo_upload(spg_context *ctx, const spg_halo_view *host, spg_device_halo **out);
+/* The requested global rows of the local block B_local (a device view whose
+ * global rows start at first_row), packed in request order as the rows the
+ * two-sided exchange sends (prepareRequestedRows + the packing loop of
+ * exchangeRemoteSparseRowsTwoSided, spgemm_exchange.cpp:60-73, 217-235): slot s
+ * holds row requests[s].  A request outside the local block is
+ * SPG_EMISSING_ROW ("received a request for a non-local sparse row of B").
+ * requests_location says where `requests` lives (SPG_HOST / SPG_DEVICE). */
+spg_status spg_halo_pack(spg_context *ctx, const spg_csr_view *B_local, int32_t first_row, const int32_t *requests,
+                         int32_t n, int32_t requests_location, spg_device_halo **out);
+spg_status spg_device_halo_view(const spg_device_halo *h, spg_halo_view *out);
+spg_status spg_device_halo_free(spg_device_halo *h);
+
 /* ---- validation of a product and its reference files --------------------
  * Host code, multithreaded over rows (threads <= 0: all hardware threads);
  * every decision and diagnostic equals the serial traversal's.  Views must be

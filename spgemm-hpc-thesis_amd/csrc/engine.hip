@@ -355,7 +355,7 @@ spg_status spg_destroy(spg_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->a_ptr, &ctx->a_col, &ctx->a_val, &ctx->b_ptr, &ctx->b_col, &ctx->b_val,
                       &ctx->dir_start, &ctx->dir_end, &ctx->ebst, &ctx->elen, &ctx->ub, &ctx->bin_rows, &ctx->snnz,
-                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->tile_status, &ctx->tile_distinct, &ctx->part, &ctx->dir32, &ctx->rowptr[0],
+                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->tile_status, &ctx->tile_distinct, &ctx->part, &ctx->dir32, &ctx->halo_err, &ctx->halo_cnt, &ctx->halo_tmp, &ctx->rowptr[0],
                       &ctx->rowptr[1], &ctx->ccol[0], &ctx->ccol[1], &ctx->cval[0], &ctx->cval[1]};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t &e : ctx->ev)
@@ -440,14 +440,17 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
         op.bend = bptr + 1;
         op.bcol = bcol;
         op.bval = bval;
+    } else if (B->location == SPG_DEVICE || (halo && halo->location == SPG_DEVICE)) {
+        // Device-resident local block and halo: directory built on the device.
+        if (B->location != SPG_DEVICE || (halo && halo->location != SPG_DEVICE))
+            return set_err(ctx, SPG_EINVAL, "B_local and the halo must both be host or both device views");
+        if ((s = device_halo_operands(ctx, A, B, b_first_row, halo, op, b_span)) != SPG_OK) return s;
     } else {
         // Directory over global B rows: local block first, then halo slots
         // (bRowForColumn order, spgemm_exchange.cpp:75-87).  Entries are
         // concatenated on the device: [B_local entries | halo entries].
         if (halo && (halo->n_rows < 0 || (halo->n_rows > 0 && (!halo->global_rows || !halo->row_ptr))))
             return set_err(ctx, SPG_EINVAL, "malformed halo view");
-        if (B->location != SPG_HOST || (halo && halo->location != SPG_HOST))
-            return set_err(ctx, SPG_EINVAL, "halo products take host views");
         const int64_t hn = halo ? halo->n_rows : 0;
         const int64_t hnnz = hn ? halo->row_ptr[hn] : 0;
         b_span = B->nnz + hnnz;
@@ -461,8 +464,8 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
             const int32_t g = halo->global_rows[slot];
             if (g < 0 || g >= A->cols) return set_err(ctx, SPG_EINVAL, "halo row outside A's column range");
             if (!seen.insert(g).second) return set_err(ctx, SPG_EINVAL, "duplicate halo row");
-            if (halo->row_ptr[slot + 1] < halo->row_ptr[slot])
-                return set_err(ctx, SPG_EINVAL, "halo row_ptr is not nondecreasing");
+            if (halo->row_ptr[slot + 1] < halo->row_ptr[slot] || halo->row_ptr[slot] < 0)
+                return set_err(ctx, SPG_EINVAL, "halo row_ptr is negative or not nondecreasing");
             if (g >= b_first_row && g < b_first_row + B->rows) continue;  // local rows take precedence
             ds[(size_t)g] = B->nnz + halo->row_ptr[slot];
             de[(size_t)g] = B->nnz + halo->row_ptr[slot + 1];

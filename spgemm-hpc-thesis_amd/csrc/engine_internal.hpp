@@ -76,6 +76,7 @@ struct spg_context {
     spg::DevBuf a_ptr, a_col, a_val, b_ptr, b_col, b_val, dir_start, dir_end;
     // workspace
     spg::DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc, tile_status, tile_distinct, part, dir32;
+    spg::DevBuf halo_err, halo_cnt, halo_tmp;  // device halo directory and packing
     int path = SPG_PATH_AUTO;
     int fused_rows = spg::kFusedRows;
     int64_t lbits_width = 0;
@@ -93,6 +94,11 @@ namespace spg {
 spg_status set_err(spg_context *c, spg_status s, const std::string &msg);
 spg_status hip_err(spg_context *c, hipError_t e, const char *what);
 spg_status check_host_csr(spg_context *ctx, const spg_csr_view *m, const char *name);
+// Operands of a product whose local B block and halo rows are device views:
+// entries [B_local | halo] and the global-row directory built on the device
+// (halo.hip).  b_span receives the number of B entries.
+spg_status device_halo_operands(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int32_t b_first_row,
+                                const spg_halo_view *halo, Operands &op, int64_t &b_span);
 spg_status read_control(spg_context *ctx);
 spg_status flag_error(spg_context *ctx, unsigned long long flags);
 

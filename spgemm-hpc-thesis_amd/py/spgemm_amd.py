@@ -39,6 +39,7 @@ EXPORTED = [
     "spg_server_create", "spg_server_submit", "spg_server_register", "spg_server_unregister",
     "spg_server_submit_resident", "spg_server_wait", "spg_server_result", "spg_server_release", "spg_server_stats",
     "spg_server_destroy",
+    "spg_halo_upload", "spg_halo_pack", "spg_device_halo_view", "spg_device_halo_free",
     "spg_validation_last_error", "spg_validate_full", "spg_full_report_text", "spg_partial_counts",
     "spg_validate_partial", "spg_spref_write", "spg_spref_read", "spg_sfull_write", "spg_sfull_open",
     "spg_sfull_next_row", "spg_sfull_row_cells", "spg_sfull_finish", "spg_sfull_close", "spg_sha256_hex",
@@ -178,6 +179,10 @@ def lib():
         L.spg_server_destroy.argtypes = [vp]
         L.spg_multiply_rows.argtypes = [vp, P(CsrView), P(CsrView), C.c_int32, P(HaloView), C.c_int32, C.c_int32,
                                         P(C.c_int64)]
+        L.spg_halo_upload.argtypes = [vp, P(HaloView), P(vp)]
+        L.spg_halo_pack.argtypes = [vp, P(CsrView), C.c_int32, vp, C.c_int32, C.c_int32, P(vp)]
+        L.spg_device_halo_view.argtypes = [vp, P(HaloView)]
+        L.spg_device_halo_free.argtypes = [vp]
         cp = C.c_char_p
         L.spg_validation_last_error.restype = cp
         L.spg_validate_full.argtypes = [P(CsrView), P(CsrView), P(CsrView), cp, cp, C.c_int, P(FullReport)]
@@ -350,6 +355,13 @@ class Engine:
     def upload(self, m: Csr) -> DeviceMatrix:
         return DeviceMatrix(self, m)
 
+    def multiply_halo_device(self, a: DeviceMatrix, b_local: DeviceMatrix, b_first_row: int,
+                             halo: "DeviceHalo | None") -> int:
+        """Product with a device-resident local B block and device halo rows
+        (directory built on the device); the result stays on the device."""
+        hv = halo.view() if halo is not None else None
+        return self.begin(a.view(), b_local.view(), b_first_row, hv)
+
     def synchronize(self):
         self._check(self._L.spg_synchronize(self._h))
 
@@ -387,6 +399,65 @@ class Engine:
     def result_host(self) -> Csr:
         v = self.result_view()
         return self.finish(v.rows, v.cols, v.nnz)
+
+
+class DeviceHalo:
+    """Halo rows in HBM owned by the library: uploaded host rows
+    (spg_halo_upload) or requested rows packed from a device B block
+    (spg_halo_pack)."""
+
+    def __init__(self, engine: "Engine", handle):
+        self._e, self._L, self._h = engine, engine._L, handle
+
+    @classmethod
+    def upload(cls, engine: "Engine", global_rows, row_ptr, col, val) -> "DeviceHalo":
+        g, rp = np.ascontiguousarray(global_rows, np.int32), np.ascontiguousarray(row_ptr, np.int64)
+        cl, vl = np.ascontiguousarray(col, np.int32), np.ascontiguousarray(val, np.float64)
+        hv = HaloView()
+        hv.n_rows = len(g)
+        hv.global_rows = g.ctypes.data if len(g) else None
+        hv.row_ptr = rp.ctypes.data
+        hv.col_idx = cl.ctypes.data if len(cl) else None
+        hv.values = vl.ctypes.data if len(vl) else None
+        hv.location = SPG_HOST
+        h = C.c_void_p()
+        engine._check(engine._L.spg_halo_upload(engine._h, C.byref(hv), C.byref(h)))
+        return cls(engine, h)
+
+    @classmethod
+    def pack(cls, engine: "Engine", b_local: "DeviceMatrix", first_row: int, requests) -> "DeviceHalo":
+        req = np.ascontiguousarray(requests, np.int32)
+        h = C.c_void_p()
+        bv = b_local.view()
+        engine._check(engine._L.spg_halo_pack(engine._h, C.byref(bv), int(first_row),
+                                              req.ctypes.data if len(req) else None, len(req), SPG_HOST,
+                                              C.byref(h)))
+        return cls(engine, h)
+
+    def view(self) -> HaloView:
+        v = HaloView()
+        self._L.spg_device_halo_view(self._h, C.byref(v))
+        return v
+
+    def host(self):
+        """(global_rows, row_ptr, col, val) copied back to the host."""
+        import ctypes
+        v = self.view()
+        n = int(v.n_rows)
+        hip = ctypes.CDLL("libamdhip64.so")
+        def get(ptr, count, dtype):
+            out = np.zeros(max(count, 1), dtype)
+            if count:
+                hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(count * out.itemsize), 2)
+            return out[:count]
+        rp = get(v.row_ptr, n + 1, np.int64)
+        nnz = int(rp[-1]) if n else 0
+        return get(v.global_rows, n, np.int32), rp, get(v.col_idx, nnz, np.int32), get(v.values, nnz, np.float64)
+
+    def free(self):
+        if self._h:
+            self._L.spg_device_halo_free(self._h)
+            self._h = None
 
 
 class Batch:

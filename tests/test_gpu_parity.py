@@ -301,3 +301,67 @@ def test_cpp_shim_matches_golden(golden, tmp_path):
             assert "accumulate ok" in r.stdout
         c = O.read_csr(str(out))
         assert O.csr_digest(c) == golden["products"][name]["digest"], name
+
+
+def test_device_halo_resolves_like_the_full_product(engine, golden):
+    # bRowForColumn with the local block and the halo rows in HBM: the
+    # directory is built on the device; the sender's packed rows
+    # (spgemm_exchange.cpp:217-235) are the same rows, and give the same product
+    for name in ("uniform_rect", "ragged_dups_unsorted", "heavy_rows"):
+        _, a, b = next(x for x in cases.synthetic_cases() if x[0] == name)
+        print("case", name, flush=True)
+        first, count = b.rows // 3, b.rows // 2
+        local, halo = split_halo(b, first, count, seed=1)
+        da, dl, dbf = engine.upload(to_s(a)), engine.upload(local), engine.upload(to_s(b))
+        up = S.DeviceHalo.upload(engine, *halo)
+        engine.multiply_halo_device(da, dl, first, up)
+        assert digest(engine.result_host()) == golden["products"][name]["digest"], name
+        packed = S.DeviceHalo.pack(engine, dbf, 0, halo[0])
+        g, rp, col, val = packed.host()
+        assert g.tolist() == halo[0].tolist() and rp.tolist() == halo[1].tolist()
+        assert col.tolist() == halo[2].tolist() and val.view(np.uint64).tolist() == np.asarray(halo[3]).view(np.uint64).tolist()
+        engine.multiply_halo_device(da, dl, first, packed)
+        assert digest(engine.result_host()) == golden["products"][name]["digest"], name
+        for h in (up, packed):
+            h.free()
+        for m in (da, dl, dbf):
+            m.free()
+
+
+def test_device_halo_errors(engine):
+    _, a, b = next(x for x in cases.synthetic_cases() if x[0] == "uniform_rect")
+    print("case device halo errors", flush=True)
+    first, count = 0, 10
+    local, halo = split_halo(b, first, count, seed=3)
+    da, dl, dbf = engine.upload(to_s(a)), engine.upload(local), engine.upload(to_s(b))
+    g, rp, col, val = halo
+    # duplicate halo row, row outside A's columns
+    dup = np.array(g), rp
+    dup[0][1] = dup[0][0]
+    for rows, code in ((dup[0], "SPG_EINVAL"), (np.where(np.arange(len(g)) == 0, a.cols + 5, g), "SPG_EINVAL")):
+        h = S.DeviceHalo.upload(engine, rows, rp, col, val)
+        with pytest.raises(S.SpgError) as e:
+            engine.multiply_halo_device(da, dl, first, h)
+        assert e.value.name == code
+        h.free()
+    # a referenced row in neither the block nor the halo
+    referenced = int(a.col[np.argmax(a.col >= count)])
+    keep = [i for i in range(len(g)) if g[i] != referenced]
+    sub_rp = [0]
+    sub_col, sub_val = [], []
+    for i in keep:
+        sub_col.append(col[rp[i]:rp[i + 1]]); sub_val.append(val[rp[i]:rp[i + 1]])
+        sub_rp.append(sub_rp[-1] + int(rp[i + 1] - rp[i]))
+    h = S.DeviceHalo.upload(engine, g[keep], sub_rp, np.concatenate(sub_col), np.concatenate(sub_val))
+    with pytest.raises(S.SpgError) as e:
+        engine.multiply_halo_device(da, dl, first, h)
+    assert e.value.name == "SPG_EMISSING_ROW"
+    h.free()
+    # a request for a row the sender does not own
+    with pytest.raises(S.SpgError) as e:
+        S.DeviceHalo.pack(engine, dl, first, [0, count + 3])
+    assert e.value.name == "SPG_EMISSING_ROW"
+    # the engine stays usable
+    engine.multiply_halo_device(da, dbf, 0, None)
+    for m in (da, dl, dbf):
+        m.free()
