@@ -273,6 +273,11 @@ typedef struct spg_matrix spg_matrix;
 spg_status spg_matrix_read_mm(const char *path, spg_matrix **out);
 spg_status spg_matrix_view(const spg_matrix *m, spg_csr_view *out);
 spg_status spg_matrix_free(spg_matrix *m);
+/* Binary CSR ("SPGCSR01": int32 rows, int32 cols, int64 nnz, row_ptr, col,
+ * val; little endian): written from a host view, read back at disk speed.
+ * No counterpart in the reference (it re-parses MatrixMarket every run). */
+spg_status spg_matrix_write_csr(const spg_csr_view *m, const char *path);
+spg_status spg_matrix_read_csr(const char *path, spg_matrix **out);
 /* Last error of the host matrix helpers (thread-local). */
 const char *spg_matrix_last_error(void);
 

@@ -39,6 +39,7 @@ EXPORTED = [
     "spg_server_create", "spg_server_submit", "spg_server_register", "spg_server_unregister",
     "spg_server_submit_resident", "spg_server_wait", "spg_server_result", "spg_server_release", "spg_server_stats",
     "spg_server_destroy",
+    "spg_matrix_write_csr", "spg_matrix_read_csr",
     "spg_halo_upload", "spg_halo_pack", "spg_device_halo_view", "spg_device_halo_free",
     "spg_validation_last_error", "spg_validate_full", "spg_full_report_text", "spg_partial_counts",
     "spg_validate_partial", "spg_spref_write", "spg_spref_read", "spg_sfull_write", "spg_sfull_open",
@@ -179,6 +180,8 @@ def lib():
         L.spg_server_destroy.argtypes = [vp]
         L.spg_multiply_rows.argtypes = [vp, P(CsrView), P(CsrView), C.c_int32, P(HaloView), C.c_int32, C.c_int32,
                                         P(C.c_int64)]
+        L.spg_matrix_write_csr.argtypes = [P(CsrView), C.c_char_p]
+        L.spg_matrix_read_csr.argtypes = [C.c_char_p, P(vp)]
         L.spg_halo_upload.argtypes = [vp, P(HaloView), P(vp)]
         L.spg_halo_pack.argtypes = [vp, P(CsrView), C.c_int32, vp, C.c_int32, C.c_int32, P(vp)]
         L.spg_device_halo_view.argtypes = [vp, P(HaloView)]
@@ -218,10 +221,27 @@ def host_view(m: Csr) -> CsrView:
 
 
 def read_mm(path: str) -> Csr:
-    """readMatrixMarket through the engine's host reader."""
+    """readMatrixMarket through the engine's host reader (parallel parse)."""
+    return _read_host_matrix(lib().spg_matrix_read_mm, path)
+
+
+def read_csr(path: str) -> Csr:
+    """A binary CSR file written by write_csr."""
+    return _read_host_matrix(lib().spg_matrix_read_csr, path)
+
+
+def write_csr(m: Csr, path: str):
+    L = lib()
+    v = host_view(m)
+    s = L.spg_matrix_write_csr(C.byref(v), path.encode())
+    if s != 0:
+        raise SpgError(s, L.spg_matrix_last_error().decode())
+
+
+def _read_host_matrix(fn, path: str) -> Csr:
     L = lib()
     h = C.c_void_p()
-    s = L.spg_matrix_read_mm(path.encode(), C.byref(h))
+    s = fn(path.encode(), C.byref(h))
     if s != 0:
         raise SpgError(s, L.spg_matrix_last_error().decode())
     try:

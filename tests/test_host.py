@@ -183,3 +183,72 @@ def test_products_and_bytes_model():
     assert W.products(a, b) == 100 * 4 * 7
     assert W.row_products(a, b).sum() == 2800
     assert W.streaming_bytes(a, 200, 2800, 10) == 8 * 101 + 12 * 400 + 16 * 400 + 12 * 2800 + 8 * 101 + 120
+
+
+def _big_mm(path, symmetric, bad_at=None, extra=0, missing=0, seed=5):
+    """A MatrixMarket file large enough for many reader threads: comments and
+    blank lines between entries, D exponents, '+' signs and duplicates."""
+    rng = np.random.default_rng(seed)
+    n, ents = 4000, 400_000
+    r = rng.integers(1, n + 1, ents)
+    c = rng.integers(1, n + 1, ents)
+    if symmetric:
+        r, c = np.maximum(r, c), np.minimum(r, c)
+    v = rng.standard_normal(ents)
+    kind = "symmetric" if symmetric else "general"
+    lines = [f"%%MatrixMarket matrix coordinate real {kind}", "% generated", f"{n} {n} {ents - missing}"]
+    for i in range(ents - missing):
+        if i % 9973 == 0:
+            lines.append("% a comment between entries")
+        if i % 7919 == 0:
+            lines.append("   ")
+        val = f"{v[i]:.17g}"
+        if i % 5 == 0:
+            val = val.replace("e", "D")
+        if i % 11 == 0 and not val.startswith("-"):
+            val = "+" + val
+        lines.append(f"{r[i]} {c[i]} {val}" if i != bad_at else f"{r[i]} {c[i]} {val} 7")
+    for i in range(extra):  # undeclared
+        lines.append(f"1 1 {i}.5")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_parallel_mm_reader_matches_oracle(tmp_path, symmetric):
+    path = str(tmp_path / "big.mtx")
+    _big_mm(path, symmetric)
+    assert os.path.getsize(path) > 9 << 20  # several reader chunks
+    got = S.read_mm(path)
+    want = O.mm_read(path)
+    assert O.csr_digest(O.Csr(got.rows, got.cols, got.row_ptr, got.col, got.val)) == O.csr_digest(want)
+    # binary CSR round trip
+    bpath = str(tmp_path / "big.csr")
+    S.write_csr(got, bpath)
+    back = S.read_csr(bpath)
+    assert back.row_ptr.tolist() == got.row_ptr.tolist() and back.col.tolist() == got.col.tolist()
+    assert back.val.view(np.uint64).tolist() == got.val.view(np.uint64).tolist()
+
+
+def test_parallel_mm_reader_errors_in_late_chunks(tmp_path):
+    cases_ = [("late_bad", dict(bad_at=390_000), "invalid Matrix Market entry"),
+              ("extra", dict(extra=3), "more Matrix Market entries than declared"),
+              ("missing", dict(missing=2), "unexpected end of Matrix Market entries")]
+    for name, kw, msg in cases_:
+        path = str(tmp_path / f"{name}.mtx")
+        _big_mm(path, False, **kw)
+        if name == "missing":  # declare two more than present
+            txt = open(path).read().replace("\n4000 4000 399998\n", "\n4000 4000 400000\n", 1)
+            open(path, "w").write(txt)
+        with pytest.raises(S.SpgError) as e:
+            S.read_mm(path)
+        assert msg in str(e.value), name
+        with pytest.raises(O.OracleError) as e2:
+            O.mm_read(path)
+    bad = str(tmp_path / "trunc.csr")
+    good = S.Csr.make(2, 2, [0, 1, 2], [0, 1], [1.0, 2.0])
+    S.write_csr(good, bad)
+    raw = open(bad, "rb").read()
+    open(bad, "wb").write(raw[:-3])
+    with pytest.raises(S.SpgError):
+        S.read_csr(bad)
