@@ -272,6 +272,19 @@ __device__ __forceinline__ void st_bits(uint32_t *p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Dynamic row assignment for the large-row kernels: the workgroup takes the
+// next list entry from a counter in the control block (zeroed with it).  Rows
+// differ by orders of magnitude in work, and the heaviest come first in row
+// order on power-law matrices, so taking rows on demand balances the tail.
+// No workgroup ever waits on another.
+__device__ __forceinline__ int next_large_row(int *counter, int *s_slot) {
+    if (threadIdx.x == 0) *s_slot = atomicAdd(counter, 1);
+    __syncthreads();
+    const int i = *s_slot;
+    __syncthreads();  // every thread has read the slot before it is reused
+    return i;
+}
+
 // Rows of one bin: list and count (filled by the analyze pass).
 __device__ __forceinline__ const RowRec *bin_list(const Workspace &ws, int bin) {
     return ws.bin_recs + (size_t)bin * (size_t)ws.list_stride;

@@ -74,6 +74,8 @@ struct Control {
     unsigned long long distinct;    // distinct coordinates of the single-pass rows
     int bin_count[8];               // rows listed per bin (list lengths)
     long long bin_rows[8];          // rows per bin (statistics; listed or not)
+    int large_next[2];              // next large row to take: symbolic, numeric kernel
+    int pad_[2];
 };
 
 // Per-block partial sums of the analyze pass, reduced by one small kernel

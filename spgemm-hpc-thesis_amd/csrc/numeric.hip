@@ -245,6 +245,18 @@ __global__ __launch_bounds__(TPB) void k_numeric_esc(Operands op, Workspace ws, 
     if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
 }
 
+// The large-row accumulator is read and written at agent scope (through L2):
+// a column's running value may have been written by another thread of the
+// workgroup in an earlier chunk or row, and must not come from a stale L1 line.
+__device__ __forceinline__ double ld_acc(const double *p) {
+    return __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<const uint64_t *>(p), __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void st_acc(double *p, double v) {
+    __hip_atomic_store(reinterpret_cast<uint64_t *>(p), (uint64_t)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int TPB, int CAP>
 __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Workspace &ws, int64_t e0, int nbt,
                                                     int64_t *s_bst, double *s_av, int64_t *s_off, int64_t *s_ws64) {
@@ -296,7 +308,8 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
     double *acc = ws.lacc + (size_t)blockIdx.x * (size_t)W;
     unsigned flags = 0;
     unsigned long long stored = 0;
-    for (int i = (int)blockIdx.x; i < count; i += (int)gridDim.x) {
+    __shared__ int s_next;
+    for (int i; (i = next_large_row(&ws.ctrl->large_next[1], &s_next)) < count;) {
         const int row = list[i].row;
         const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
         const int64_t cbase = crowptr[row];
@@ -369,45 +382,63 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                     }
                     if (tid == 0) s_cnt[nb] = V;
                     __syncthreads();
+                    uint64_t key[ITEMS];
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        key[j] = ((uint64_t)(uint32_t)icol[j] << 32) | (uint32_t)(tid + j * TPB);
+                        if (ok[j]) s_key[s_cnt[bk[j]] + pos[j]] = key[j];
+                    }
+                    __syncthreads();
+                    // Rank inside the bucket by (column, serial index): the
+                    // item's sorted position.  Keys are distinct.
+                    int x[ITEMS];
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        x[j] = -1;
+                        if (ok[j]) {
+                            const int lo = s_cnt[bk[j]], hi = s_cnt[bk[j] + 1];
+                            int rank = 0;
+                            for (int y = lo; y < hi; ++y) rank += s_key[y] < key[j];
+                            x[j] = lo + rank;
+                        }
+                    }
+                    __syncthreads();  // every bucket read before the sorted placement
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) {
                         if (ok[j]) {
-                            const int s = tid + j * TPB;
-                            const int d = s_cnt[bk[j]] + pos[j];
-                            s_key[d] = ((uint64_t)(uint32_t)icol[j] << 32) | (uint32_t)s;
-                            s_val[d] = ival[j];
+                            s_key[x[j]] = key[j];
+                            s_val[x[j]] = ival[j];
                         }
                     }
                     __syncthreads();
-                    for (int q = 0; q < nb; q += TPB) {
-                        const int b = q + tid;
-                        const int lo = s_cnt[b];
-                        const int hi = s_cnt[b + 1];
-                        for (int x = lo + 1; x < hi; ++x) {
-                            const uint64_t k = s_key[x];
-                            const double v = s_val[x];
-                            int y = x - 1;
-                            while (y >= lo && s_key[y] > k) {
-                                s_key[y + 1] = s_key[y];
-                                s_val[y + 1] = s_val[y];
-                                --y;
-                            }
-                            s_key[y + 1] = k;
-                            s_val[y + 1] = v;
-                        }
-                        for (int x = lo; x < hi;) {
-                            const uint32_t c = (uint32_t)(s_key[x] >> 32);
-                            const uint32_t rel = (uint32_t)((int64_t)(int32_t)c - w0);
-                            const uint32_t bit = 1u << (rel & 31u);
-                            const bool present = (ld_bits(&bits[rel >> 5]) & bit) != 0u;
-                            double v = present ? acc[rel] : 0.0;
-                            while (x < hi && (uint32_t)(s_key[x] >> 32) == c) {
-                                v = v + s_val[x];
-                                ++x;
-                            }
-                            acc[rel] = v;
-                            if (!present) atomicOr(&bits[rel >> 5], bit);
-                        }
+                    // Each column's first sorted position continues that
+                    // column's running sum (the accumulator, when an earlier
+                    // chunk touched it, else its first product) with its
+                    // products in serial order.  All heads' accumulator and
+                    // presence loads are issued before any fold.
+                    bool head[ITEMS];
+                    uint32_t hcol[ITEMS], hword[ITEMS];
+                    uint64_t hacc[ITEMS];
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        const int y = tid + j * TPB;
+                        head[j] = y < V && (y == 0 || (s_key[y - 1] >> 32) != (s_key[y] >> 32));
+                        hcol[j] = head[j] ? (uint32_t)(s_key[y] >> 32) : 0u;
+                        const uint32_t rel = head[j] ? (uint32_t)((int64_t)(int32_t)hcol[j] - w0) : 0u;
+                        hword[j] = head[j] ? ld_bits(&bits[rel >> 5]) : 0u;
+                        hacc[j] = head[j] ? (uint64_t)__double_as_longlong(ld_acc(&acc[rel])) : 0ull;
+                    }
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        if (!head[j]) continue;
+                        const int y0 = tid + j * TPB;
+                        const uint32_t rel = (uint32_t)((int64_t)(int32_t)hcol[j] - w0);
+                        const uint32_t bit = 1u << (rel & 31u);
+                        const bool present = (hword[j] & bit) != 0u;
+                        double v = present ? __longlong_as_double((long long)hacc[j]) + s_val[y0] : s_val[y0];
+                        for (int y = y0 + 1; y < V && (uint32_t)(s_key[y] >> 32) == hcol[j]; ++y) v = v + s_val[y];
+                        st_acc(&acc[rel], v);
+                        if (!present) atomicOr(&bits[rel >> 5], bit);
                     }
                     __syncthreads();  // chunk folded; LDS and accumulator state consistent
                 }
@@ -423,7 +454,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                 while (m) {
                     const int b = __ffs((int)m) - 1;
                     m &= m - 1u;
-                    if (op.keep_zeros || acc[q * 32 + b] != 0.0) ++mine;
+                    if (op.keep_zeros || ld_acc(&acc[q * 32 + b]) != 0.0) ++mine;
                 }
             }
             int64_t wtotal;
@@ -433,7 +464,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                 while (m) {
                     const int b = __ffs((int)m) - 1;
                     m &= m - 1u;
-                    const double v = acc[q * 32 + b];
+                    const double v = ld_acc(&acc[q * 32 + b]);
                     if (op.keep_zeros || v != 0.0) {
                         if (o < lim) {
                             ccol[cbase + o] = (int32_t)(w0 + q * 32 + b);

@@ -20,6 +20,7 @@ namespace {
 using namespace dev;
 
 constexpr int32_t kEmptyKey = INT_MAX;  // never a valid column (col < cols <= INT_MAX)
+constexpr int kLargeUnroll = 4;         // products in flight per thread on the large-row path
 
 __device__ __forceinline__ uint32_t hash_col(int32_t c) { return (uint32_t)c * 2654435761u; }
 
@@ -219,7 +220,8 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
     uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
     unsigned flags = 0;
     unsigned long long sym = 0;
-    for (int i = (int)blockIdx.x; i < count; i += (int)gridDim.x) {
+    __shared__ int s_next;
+    for (int i; (i = next_large_row(&ws.ctrl->large_next[0], &s_next)) < count;) {
         const int row = list[i].row;
         const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
         int64_t cnt = 0;
@@ -228,15 +230,30 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
             for (int64_t e0 = a0; e0 < a1; e0 += CAP) {
                 const int nbt = (int)min((int64_t)CAP, a1 - e0);
                 const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, nullptr, s_off, s_ws64, false);
-                for (int64_t s = tid; s < Pb; s += TPB) {
-                    const int e = upper_entry(s_off, nbt, s);
-                    const int64_t c = op.bcol[s_bst[e] + (s - s_off[e])];
-                    if (c >= w0 && c < wend) {
-                        const uint32_t rel = (uint32_t)(c - w0);
-                        const uint32_t bit = 1u << (rel & 31u);
-                        const uint32_t old = atomicOr(&bits[rel >> 5], bit);
-                        if (!(old & bit)) ++cnt;
+                // kLargeUnroll products per thread per step: every column load,
+                // then every bit claim, in flight together.
+                for (int64_t s0 = tid; s0 < Pb; s0 += (int64_t)TPB * kLargeUnroll) {
+                    int64_t c[kLargeUnroll];
+#pragma unroll
+                    for (int u = 0; u < kLargeUnroll; ++u) {
+                        const int64_t s = s0 + (int64_t)u * TPB;
+                        c[u] = -1;
+                        if (s < Pb) {
+                            const int e = upper_entry(s_off, nbt, s);
+                            c[u] = op.bcol[s_bst[e] + (s - s_off[e])];
+                        }
                     }
+                    uint32_t old[kLargeUnroll], bit[kLargeUnroll];
+                    bool in[kLargeUnroll];
+#pragma unroll
+                    for (int u = 0; u < kLargeUnroll; ++u) {
+                        in[u] = c[u] >= w0 && c[u] < wend;
+                        const uint32_t rel = in[u] ? (uint32_t)(c[u] - w0) : 0u;
+                        bit[u] = 1u << (rel & 31u);
+                        old[u] = in[u] ? atomicOr(&bits[rel >> 5], bit[u]) : bit[u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLargeUnroll; ++u) cnt += in[u] && !(old[u] & bit[u]);
                 }
                 __syncthreads();  // batch metadata is overwritten next
             }
