@@ -8,9 +8,10 @@
 // of a B row are contiguous, and all of a row's column loads are in flight
 // before the first insertion.  Insertions then run in lockstep probe rounds
 // with every register slot's compare-and-swap issued back to back.
-// Rows whose columns span fewer than 32 x the table's words count with a
-// bitmap over their column range in the table's storage instead: one LDS OR
-// per product, no probing.  Large rows set bits in a per-workgroup column
+// Rows whose columns span fewer than 32 x the table's words (at least 65536
+// columns for rows of up to 512 products) count with a bitmap over their
+// column range in the table's storage instead: one LDS OR per product, no
+// probing.  Large rows set bits in a per-workgroup column
 // bitmap in HBM (window by window) and count the bits they newly set.
 #include "device_common.hpp"
 
@@ -28,7 +29,11 @@ template <int TPB, int CAP>
 struct HashLds {
     static constexpr int FILL = CAP <= 256 ? 4 : 2;  // table size / product bound
     static constexpr int TMAX = FILL * CAP;
-    int32_t tab[TMAX + TPB];  // table, then one dummy word per thread
+    // Words of the bitmap path: at least 2048 (columns spanning < 65536) from
+    // 512-product rows on, so rows of products with up to 64 K columns never probe.
+    static constexpr int BMW = CAP >= 512 && TMAX < 2048 ? 2048 : TMAX;
+    static constexpr int TABW = (BMW > TMAX + TPB ? BMW : TMAX + TPB);
+    int32_t tab[TABW];  // table (dummy word per thread at TMAX + tid), or the bitmap
     int64_t bst[CAP];
     int off[CAP + 1];
     int wsi[2 * (TPB / 64) + 2];
@@ -45,6 +50,7 @@ __device__ __forceinline__ void hash_step(const Operands &op, const Workspace &w
     constexpr int ITEMS = CAP / TPB;
     constexpr int FILL = HashLds<TPB, CAP>::FILL;
     constexpr int TMAX = HashLds<TPB, CAP>::TMAX;
+    constexpr int BMW = HashLds<TPB, CAP>::BMW;
     const int tid = threadIdx.x;
     int P, L;
     meta_to_lds<TPB, false>(op, ws, cur, pcur, cur.na <= TPB, sh.bst, nullptr, sh.off, sh.wsi, P, L);
@@ -72,7 +78,7 @@ __device__ __forceinline__ void hash_step(const Operands &op, const Workspace &w
         }
         block_minmax<TPB>(mn, mx, sh.wsi);
         int fresh = 0;
-        if ((int64_t)mx - (int64_t)mn < (int64_t)TMAX * 32) {
+        if ((int64_t)mx - (int64_t)mn < (int64_t)BMW * 32) {
             // The row's columns fit a bitmap over [mn, mx] in the table's
             // storage: one OR per product, and a product counts when it sets
             // its bit.  Touched words are zeroed first and given back to the
@@ -151,7 +157,7 @@ template <int TPB, int CAP>
 __global__ __launch_bounds__(TPB) void k_symbolic_hash(Operands op, Workspace ws, int binid) {
     __shared__ HashLds<TPB, CAP> sh;
     const int tid = threadIdx.x;
-    for (int q = tid; q < HashLds<TPB, CAP>::TMAX + TPB; q += TPB) sh.tab[q] = kEmptyKey;
+    for (int q = tid; q < HashLds<TPB, CAP>::TABW; q += TPB) sh.tab[q] = kEmptyKey;
     __syncthreads();
     const RowRec *list = bin_list(ws, binid);
     const int count = ws.ctrl->bin_count[binid];
