@@ -200,8 +200,8 @@ def run_batched(args, wl):
     log("bench: autotune")
     c1 = w1.autotune(2)
     c2 = w2.autotune(2)
-    w1.set_graph(True)
-    w2.set_graph(True)
+    w1.set_graph(not args.no_graph)
+    w2.set_graph(not args.no_graph)
 
     def batch_step():
         n1 = w1.run()
@@ -287,7 +287,7 @@ def run_batched(args, wl):
                "sample": f"first {nsample} jobs, serial-order C restatement of spgemm() with OpenMP static rows"}
     out = {
         "metric": f"batched SpGEMM jobs/s ({njobs} independent jobs, 1/3 Galerkin R.A.P), one batched "
-                  "pipeline replayed from a HIP graph",
+                  + ("pipeline launched eagerly" if args.no_graph else "pipeline replayed from a HIP graph"),
         "value": round(njobs / batch_s, 2),
         "unit": "jobs/s",
         "n_gpus": 1,
@@ -438,6 +438,9 @@ def main():
     ap.add_argument("--workload", default="uniform_2p20x16", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--jobs", type=int, default=0, help="batched workload: number of jobs (default 4096)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched workload: launch the pipeline eagerly instead of replaying a HIP graph "
+                         "(for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this engine computes one rank's local product; "
