@@ -28,6 +28,7 @@ __global__ void k_zero_words(uint32_t *p, size_t n) {
 }
 
 constexpr int kAnalyzeTpb = 256;
+constexpr int kAnChunks = 2;  // 64-entry chunks in flight per wave (1 and 4 measured no faster)
 
 __global__ void k_dir32(const int64_t *bptr, int64_t n, uint32_t *out) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -50,16 +51,16 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
     s_ub[lane] = 0ull;
     wave_sync();
     unsigned flags = 0;
-    // Two chunks of 64 entries per step; both chunks' column loads and B-row
-    // bound gathers are issued (unconditionally, at clamped indices) before
-    // either chunk is reduced, so a wave has four loads in flight per lane.
+    // kAnChunks chunks of 64 entries per step; every chunk's column loads and
+    // B-row bound gathers are issued (unconditionally, at clamped indices)
+    // before any chunk is reduced.
     if (op.nnz_a <= 0 && E1 > E0) flags |= kErrInternal;  // row_ptr inconsistent with nnz(A)
-    for (int64_t base = E0; op.nnz_a > 0 && base < E1; base += 128) {
-        int64_t ev[2], sv[2], lv[2];
-        bool inv[2];
-        int32_t kv[2];
+    for (int64_t base = E0; op.nnz_a > 0 && base < E1; base += 64 * kAnChunks) {
+        int64_t ev[kAnChunks], sv[kAnChunks], lv[kAnChunks];
+        bool inv[kAnChunks];
+        int32_t kv[kAnChunks];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < kAnChunks; ++h) {
             ev[h] = base + 64 * h + lane;
             inv[h] = ev[h] < E1;
             if (inv[h] && ev[h] >= op.nnz_a) {  // row_ptr inconsistent with nnz(A)
@@ -69,9 +70,9 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
             const int64_t ec = max((int64_t)0, min(ev[h], min(E1, op.nnz_a) - 1));
             kv[h] = __builtin_nontemporal_load(&op.acol[ec]);
         }
-        int64_t bs[2], be[2];
+        int64_t bs[kAnChunks], be[kAnChunks];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < kAnChunks; ++h) {
             const bool ok = kv[h] >= 0 && kv[h] < op.ndir;
             const int32_t kk = ok ? kv[h] : 0;
             if (DIR32) {
@@ -85,7 +86,7 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
             sv[h] = ok ? bs[h] : -1;
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < kAnChunks; ++h) {
             int64_t s0 = 0, len = 0;
             if (inv[h] && kv[h] >= 0 && kv[h] < op.ndir) {
                 if (sv[h] < 0) {
@@ -106,7 +107,7 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
             }
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < kAnChunks; ++h) {
             // Row (lane) holding entry e = base + 64h + lane: the largest row
             // l <= last whose first entry a0_l <= e.  Rows that start inside
             // the chunk mark their start position; a prefix maximum over the
