@@ -184,10 +184,23 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
     }
 }
 
+// Large-row work classes: two per power of two of P_row (its leading bit
+// and the next one), so rows of one class differ by less than 1.5x.
+constexpr int kLargeClasses = 128;
+__device__ __forceinline__ int large_class(int64_t ub) {
+    const int lg = 63 - __clzll((long long)max(ub, (int64_t)2));
+    return 2 * lg + (int)((ub >> (lg - 1)) & 1);
+}
+
 // Every field of every block's partials in one pass: thread i sums blocks
-// i, i + 1024, ... for all fields at once.
+// i, i + 1024, ... for all fields at once.  Then the large rows are listed
+// again in descending work class (heaviest first) in the storage of the
+// empty bin, which lists no rows: the large-row kernels take rows in list
+// order, so the heaviest start first and the lightest fill the tail.  The
+// order inside a class is arbitrary; no result depends on it.
 constexpr int kReduceTpb = 1024;
-__global__ __launch_bounds__(kReduceTpb) void k_analyze_reduce(const int64_t *part, int64_t nblocks, Control *c) {
+__global__ __launch_bounds__(kReduceTpb) void k_analyze_reduce(const int64_t *part, int64_t nblocks, Workspace ws) {
+    Control *c = ws.ctrl;
     __shared__ int64_t s_f[kReduceTpb / 64][kAnalyzeFields];
     int64_t v[kAnalyzeFields] = {};
     for (int64_t i = threadIdx.x; i < nblocks; i += kReduceTpb) {
@@ -209,6 +222,28 @@ __global__ __launch_bounds__(kReduceTpb) void k_analyze_reduce(const int64_t *pa
         if (f == 0) c->products = (unsigned long long)t;
         else if (f == 1) c->ub_small = (unsigned long long)t;
         else c->bin_rows[f - 2] = t;
+    }
+    const int n = c->bin_count[kBinLarge];
+    if (n <= 0) return;
+    __shared__ int s_cls[kLargeClasses];
+    const RowRec *in = ws.bin_recs + (size_t)kBinLarge * (size_t)ws.list_stride;
+    RowRec *out = ws.bin_recs + (size_t)kBinEmpty * (size_t)ws.list_stride;
+    if (threadIdx.x < kLargeClasses) s_cls[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kReduceTpb) atomicAdd(&s_cls[large_class(ws.ub[in[i].row])], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {  // class starts, heaviest class first
+        int run = 0;
+        for (int k = kLargeClasses - 1; k >= 0; --k) {
+            const int cnt = s_cls[k];
+            s_cls[k] = run;
+            run += cnt;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kReduceTpb) {
+        const RowRec r = in[i];
+        out[atomicAdd(&s_cls[large_class(ws.ub[r.row])], 1)] = r;
     }
 }
 
@@ -245,7 +280,7 @@ void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream)
         hipLaunchKernelGGL(k_analyze<true>, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
     else
         hipLaunchKernelGGL(k_analyze<false>, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
-    hipLaunchKernelGGL(k_analyze_reduce, dim3(1), dim3(kReduceTpb), 0, stream, ws.part, blocks, ws.ctrl);
+    hipLaunchKernelGGL(k_analyze_reduce, dim3(1), dim3(kReduceTpb), 0, stream, ws.part, blocks, ws);
 }
 
 }  // namespace spg

@@ -289,6 +289,9 @@ __device__ __forceinline__ int next_large_row(int *counter, int *s_slot) {
 __device__ __forceinline__ const RowRec *bin_list(const Workspace &ws, int bin) {
     return ws.bin_recs + (size_t)bin * (size_t)ws.list_stride;
 }
+// The large rows heaviest first (bin_count[kBinLarge] records), ordered by
+// the analyze reduction into the storage of the empty bin.
+__device__ __forceinline__ const RowRec *large_list(const Workspace &ws) { return bin_list(ws, kBinEmpty); }
 
 // Record i (block-uniform i) is fetched as one dword per lane (lanes 0..3 of
 // each wave), i.e. with a vector load whose latency is tracked apart from LDS

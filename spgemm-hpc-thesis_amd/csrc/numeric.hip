@@ -288,10 +288,33 @@ __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Wo
     return Pb;
 }
 
+// Products s = c0 + tid + j*TPB of the current A-entry batch: B column, B
+// value and the batch entry e (its A value is s_av[e]); column -1 past Pb.
+template <int TPB, int CAP, int ITEMS>
+__device__ __forceinline__ void load_chunk(const Operands &op, const int64_t *s_bst, const int64_t *s_off, int nbt,
+                                           int64_t Pb, int64_t c0, int (&col)[ITEMS], double (&bv)[ITEMS],
+                                           int (&ent)[ITEMS]) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const int64_t g = c0 + threadIdx.x + j * TPB;
+        col[j] = -1;
+        bv[j] = 0.0;
+        ent[j] = 0;
+        if (g < Pb) {
+            const int e = upper_entry(s_off, nbt, g);
+            const int64_t idx = s_bst[e] + (g - s_off[e]);
+            col[j] = op.bcol[idx];
+            bv[j] = op.bval[idx];
+            ent[j] = e;
+        }
+    }
+}
+
 template <int TPB, int CAP>
 __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws, const int64_t *crowptr,
                                                        int32_t *ccol, double *cval) {
     constexpr int ITEMS = CAP / TPB;
+    constexpr int kExtract = 8;  // accumulator loads in flight per thread during extraction
     __shared__ int64_t s_bst[CAP];
     __shared__ double s_av[CAP];
     __shared__ int64_t s_off[CAP + 1];
@@ -301,7 +324,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
     __shared__ int s_wsi[2 * (TPB / 64) + 2];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
     const int tid = threadIdx.x;
-    const RowRec *list = bin_list(ws, kBinLarge);
+    const RowRec *list = large_list(ws);
     const int count = ws.ctrl->bin_count[kBinLarge];
     const int64_t W = ws.lwidth;
     uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
@@ -317,38 +340,39 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
         int64_t outn = 0;
         for (int64_t w0 = 0; w0 < op.bcols; w0 += W) {
             const int64_t wend = min(w0 + W, (int64_t)op.bcols);
+            // This thread's net change of the number of the window's columns
+            // whose running sum is exactly zero.
+            int zc = 0;
             for (int64_t e0 = a0; e0 < a1; e0 += CAP) {
                 const int nbt = (int)min((int64_t)CAP, a1 - e0);
                 const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, s_av, s_off, s_ws64);
+                // Each chunk's products are loaded one chunk ahead, so their
+                // latency overlaps the previous chunk's accumulator round trip.
+                int pcol[ITEMS], pent[ITEMS];
+                double pbv[ITEMS];
+                load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, 0, pcol, pbv, pent);
                 for (int64_t c0 = 0; c0 < Pb; c0 += CAP) {
-                    const int Pc = (int)min((int64_t)CAP, Pb - c0);
                     int icol[ITEMS];
                     double ival[ITEMS];
                     bool ok[ITEMS];
                     int cmn = INT_MAX, cmx = INT_MIN, nvalid = 0;
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) {
-                        const int s = tid + j * TPB;
-                        icol[j] = 0;
-                        ival[j] = 0.0;
-                        ok[j] = false;
-                        if (s < Pc) {
-                            const int64_t g = c0 + s;
-                            const int e = upper_entry(s_off, nbt, g);
-                            const int64_t idx = s_bst[e] + (g - s_off[e]);
-                            const int c = op.bcol[idx];
-                            if (c >= w0 && c < wend) {
-                                icol[j] = c;
-                                ival[j] = s_av[e] * op.bval[idx];
-                                ok[j] = true;
-                                cmn = min(cmn, c);
-                                cmx = max(cmx, c);
-                                ++nvalid;
-                            }
+                        const int c = pcol[j];
+                        ok[j] = c >= w0 && c < wend;  // -1 past the batch
+                        icol[j] = ok[j] ? c : 0;
+                        ival[j] = ok[j] ? s_av[pent[j]] * pbv[j] : 0.0;
+                        if (ok[j]) {
+                            cmn = min(cmn, c);
+                            cmx = max(cmx, c);
+                            ++nvalid;
                         }
                     }
                     const int V = block_sum<TPB>(nvalid, s_wsi);
-                    if (V == 0) continue;  // uniform across the block
+                    if (V == 0) {  // uniform across the block
+                        load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, pcol, pbv, pent);
+                        continue;
+                    }
                     block_minmax<TPB>(cmn, cmx, s_wsi);
                     int nb = TPB;
                     while (nb < V && nb < CAP) nb <<= 1;
@@ -415,7 +439,8 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                     // column's running sum (the accumulator, when an earlier
                     // chunk touched it, else its first product) with its
                     // products in serial order.  All heads' accumulator and
-                    // presence loads are issued before any fold.
+                    // presence loads are issued before any fold, then the
+                    // next chunk's products.
                     bool head[ITEMS];
                     uint32_t hcol[ITEMS], hword[ITEMS];
                     uint64_t hacc[ITEMS];
@@ -428,6 +453,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                         hword[j] = head[j] ? ld_bits(&bits[rel >> 5]) : 0u;
                         hacc[j] = head[j] ? (uint64_t)__double_as_longlong(ld_acc(&acc[rel])) : 0ull;
                     }
+                    load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, pcol, pbv, pent);
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) {
                         if (!head[j]) continue;
@@ -435,45 +461,75 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                         const uint32_t rel = (uint32_t)((int64_t)(int32_t)hcol[j] - w0);
                         const uint32_t bit = 1u << (rel & 31u);
                         const bool present = (hword[j] & bit) != 0u;
-                        double v = present ? __longlong_as_double((long long)hacc[j]) + s_val[y0] : s_val[y0];
+                        const double old = __longlong_as_double((long long)hacc[j]);
+                        double v = present ? old + s_val[y0] : s_val[y0];
                         for (int y = y0 + 1; y < V && (uint32_t)(s_key[y] >> 32) == hcol[j]; ++y) v = v + s_val[y];
                         st_acc(&acc[rel], v);
                         if (!present) atomicOr(&bits[rel >> 5], bit);
+                        zc += (int)(v == 0.0) - (int)(present && old == 0.0);
                     }
                     __syncthreads();  // chunk folded; LDS and accumulator state consistent
                 }
             }
-            // Extraction of the window in ascending column order, zeros dropped.
+            // Extraction of the window in ascending column order, zeros
+            // dropped.  When no column of the window sums to exactly zero (or
+            // zeros are kept) the entries before a word are the set bits
+            // before it, and the counting pass reads the bitmap only.
             const int64_t nwords = (wend - w0 + 31) / 32;
             const int64_t wpt = (nwords + TPB - 1) / TPB;
-            const int64_t q0 = (int64_t)tid * wpt;
+            const int64_t q0 = min((int64_t)tid * wpt, nwords);
             const int64_t q1 = min(q0 + wpt, nwords);
+            const bool all_stored = op.keep_zeros || block_sum<TPB>(zc, s_wsi) == 0;
             int64_t mine = 0;
-            for (int64_t q = q0; q < q1; ++q) {
-                uint32_t m = ld_bits(&bits[q]);
-                while (m) {
-                    const int b = __ffs((int)m) - 1;
-                    m &= m - 1u;
-                    if (op.keep_zeros || ld_acc(&acc[q * 32 + b]) != 0.0) ++mine;
+            if (all_stored) {
+                for (int64_t q = q0; q < q1; q += kExtract) {
+                    uint32_t wd[kExtract];
+#pragma unroll
+                    for (int u = 0; u < kExtract; ++u) wd[u] = q + u < q1 ? ld_bits(&bits[q + u]) : 0u;
+#pragma unroll
+                    for (int u = 0; u < kExtract; ++u) mine += __popc(wd[u]);
+                }
+            } else {
+                for (int64_t q = q0; q < q1; ++q) {
+                    uint32_t m = ld_bits(&bits[q]);
+                    while (m) {
+                        const int b = __ffs((int)m) - 1;
+                        m &= m - 1u;
+                        if (ld_acc(&acc[q * 32 + b]) != 0.0) ++mine;
+                    }
                 }
             }
             int64_t wtotal;
             int64_t o = outn + block_excl_scan<TPB>(mine, s_ws64, wtotal);
+            uint32_t m = q0 < q1 ? ld_bits(&bits[q0]) : 0u;
             for (int64_t q = q0; q < q1; ++q) {
-                uint32_t m = ld_bits(&bits[q]);
+                const uint32_t nxt = q + 1 < q1 ? ld_bits(&bits[q + 1]) : 0u;
                 while (m) {
-                    const int b = __ffs((int)m) - 1;
-                    m &= m - 1u;
-                    const double v = ld_acc(&acc[q * 32 + b]);
-                    if (op.keep_zeros || v != 0.0) {
-                        if (o < lim) {
-                            ccol[cbase + o] = (int32_t)(w0 + q * 32 + b);
-                            cval[cbase + o] = v;
+                    int b[kExtract];
+                    double v[kExtract];
+#pragma unroll
+                    for (int u = 0; u < kExtract; ++u) {
+                        b[u] = -1;
+                        if (m) {
+                            b[u] = __ffs((int)m) - 1;
+                            m &= m - 1u;
                         }
-                        ++o;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kExtract; ++u) v[u] = b[u] >= 0 ? ld_acc(&acc[q * 32 + b[u]]) : 0.0;
+#pragma unroll
+                    for (int u = 0; u < kExtract; ++u) {
+                        if (b[u] >= 0 && (op.keep_zeros || v[u] != 0.0)) {
+                            if (o < lim) {
+                                ccol[cbase + o] = (int32_t)(w0 + q * 32 + b[u]);
+                                cval[cbase + o] = v[u];
+                            }
+                            ++o;
+                        }
                     }
                 }
                 st_bits(&bits[q], 0u);
+                m = nxt;
             }
             outn += wtotal;
             __threadfence();

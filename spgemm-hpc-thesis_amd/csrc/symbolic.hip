@@ -220,7 +220,7 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
     __shared__ int64_t s_off[CAP + 1];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
     const int tid = threadIdx.x;
-    const RowRec *list = bin_list(ws, kBinLarge);
+    const RowRec *list = large_list(ws);
     const int count = ws.ctrl->bin_count[kBinLarge];
     const int64_t W = ws.lwidth;
     uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);

@@ -53,3 +53,25 @@ def test_accumulate_finish_twice_and_more_stages(engine):
         acc.add_product(S.Csr.make(3, 2, [0, 0, 0, 0], [], []), b)  # wrong row count
     assert e.value.name == "SPG_EINVAL"
     acc.close()
+
+
+def test_accumulate_large_rows_with_cancellation(engine):
+    # Two stages of rows with 24,000 products each into 64 columns, values
+    # +-1 (the large-row path with exact-zero running sums), against the
+    # oracle's restatement of LocalAccumulator.
+    rng = np.random.Generator(np.random.PCG64(33))
+
+    def pm1(rows, cols, per_row):
+        rp = np.arange(rows + 1, dtype=np.int64) * per_row
+        return O.Csr.make(rows, cols, rp, rng.integers(0, cols, size=rows * per_row),
+                          rng.choice(np.array([-1.0, 1.0]), size=rows * per_row))
+
+    stages = [(pm1(30, 4000, 3000), pm1(4000, 64, 8)), (pm1(30, 2000, 3000), pm1(2000, 64, 8))]
+    acc = S.Accumulator(engine, 30, 64)
+    for a, b in stages:
+        acc.add_product(to_s(a), to_s(b))
+    c = acc.finish()
+    acc.close()
+    ref = O.accumulate(30, 64, stages)
+    assert c.nnz == ref.nnz
+    assert O.csr_digest(O.Csr.make(c.rows, c.cols, c.row_ptr, c.col, c.val)) == O.csr_digest(ref)

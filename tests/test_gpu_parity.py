@@ -278,6 +278,41 @@ def test_rmat_s13_AA(engine):
     assert digest(c) == O.csr_digest(O.product(to_o(a), to_o(a)))
 
 
+def pm1_csr(rng, rows, cols, per_row):
+    rp = np.arange(rows + 1, dtype=np.int64) * per_row
+    col = rng.integers(0, cols, size=rows * per_row)
+    val = rng.choice(np.array([-1.0, 1.0]), size=rows * per_row)
+    return O.Csr.make(rows, cols, rp, col, val)
+
+
+def test_large_rows_exact_zero_sums(engine):
+    # 40 rows of 24,000 products into 64 columns with values +-1: running sums
+    # cross zero between chunks and some end at exactly zero, so the large-row
+    # extraction must drop them; whole product bit-exact against the oracle.
+    rng = np.random.Generator(np.random.PCG64(31))
+    a, b = pm1_csr(rng, 40, 4000, 3000), pm1_csr(rng, 4000, 64, 8)
+    c = engine.multiply(to_s(a), to_s(b))
+    assert engine.stats()["rows_per_bin"][5] == 40
+    ref = O.product(a, b)
+    assert ref.nnz < 40 * 64  # some coordinates cancel
+    assert digest(c) == O.csr_digest(ref)
+
+
+def test_large_rows_heaviest_first_order(engine):
+    # Large rows of very different work, heaviest last in row order: the
+    # engine reorders them by work; every row is still the oracle's.
+    rng = np.random.Generator(np.random.PCG64(32))
+    lens = np.concatenate([np.full(60, 300), [4000, 9000, 20000]])
+    rp = np.zeros(len(lens) + 1, dtype=np.int64)
+    np.cumsum(lens, out=rp[1:])
+    a = O.Csr.make(len(lens), 5000, rp, rng.integers(0, 5000, size=int(rp[-1])), 2.0 * rng.random(int(rp[-1])) - 1.0)
+    b = O.Csr.make(5000, 30000, np.arange(5001, dtype=np.int64) * 12, rng.integers(0, 30000, size=60000),
+                   2.0 * rng.random(60000) - 1.0)
+    c = engine.multiply(to_s(a), to_s(b))
+    assert engine.stats()["rows_per_bin"][5] == len(lens)
+    assert digest(c) == O.csr_digest(O.product(a, b))
+
+
 SHIM = os.path.join(os.path.dirname(S.LIB_PATH), "shim_test")
 
 
