@@ -36,6 +36,9 @@ constexpr int kTpbBlock = 256;
 constexpr int kSymTpbW3 = SPG_SYM_TPB_W3, kSymTpbBlock = SPG_SYM_TPB_BLOCK;
 constexpr int kCapLarge = 1024;  // products per chunk on the large-row path
 constexpr int kTpbLarge = 256;
+// Column windows of up to this many columns keep the large-row presence
+// bitmap in LDS (32 KB) instead of HBM.
+constexpr int64_t kLdsBitsCols = int64_t(1) << 18;
 // Single-pass path: rows with at most kFusedCap products and A entries are
 // computed by one wave each, kFusedRows consecutive rows per wave, in tiles of
 // (kFusedTpb / 64) * kFusedRows rows (these are exactly the rows of bins W1, W2).

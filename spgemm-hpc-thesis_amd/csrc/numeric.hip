@@ -310,10 +310,15 @@ __device__ __forceinline__ void load_chunk(const Operands &op, const int64_t *s_
     }
 }
 
-template <int TPB, int CAP>
+// LDSB: the window's presence bitmap lives in dynamic LDS (lwidth / 8
+// bytes), so a head knows whether its column is present before touching HBM
+// and loads the running sum of present columns only; else it is the
+// workgroup's bitmap in HBM, loaded together with the running sum.
+template <int TPB, int CAP, bool LDSB>
 __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws, const int64_t *crowptr,
                                                        int32_t *ccol, double *cval) {
     constexpr int ITEMS = CAP / TPB;
+    extern __shared__ uint32_t s_lbits[];
     constexpr int kExtract = 8;  // accumulator loads in flight per thread during extraction
     __shared__ int64_t s_bst[CAP];
     __shared__ double s_av[CAP];
@@ -331,6 +336,9 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
     double *acc = ws.lacc + (size_t)blockIdx.x * (size_t)W;
     unsigned flags = 0;
     unsigned long long stored = 0;
+    if (LDSB) {
+        for (int64_t q = tid; q < W / 32; q += TPB) s_lbits[q] = 0u;
+    }
     __shared__ int s_next;
     for (int i; (i = next_large_row(&ws.ctrl->large_next[1], &s_next)) < count;) {
         const int row = list[i].row;
@@ -438,9 +446,9 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                     // Each column's first sorted position continues that
                     // column's running sum (the accumulator, when an earlier
                     // chunk touched it, else its first product) with its
-                    // products in serial order.  All heads' accumulator and
-                    // presence loads are issued before any fold, then the
-                    // next chunk's products.
+                    // products in serial order.  All heads' accumulator loads
+                    // are issued before any fold, then the next chunk's
+                    // products.
                     bool head[ITEMS];
                     uint32_t hcol[ITEMS], hword[ITEMS];
                     uint64_t hacc[ITEMS];
@@ -450,8 +458,14 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                         head[j] = y < V && (y == 0 || (s_key[y - 1] >> 32) != (s_key[y] >> 32));
                         hcol[j] = head[j] ? (uint32_t)(s_key[y] >> 32) : 0u;
                         const uint32_t rel = head[j] ? (uint32_t)((int64_t)(int32_t)hcol[j] - w0) : 0u;
-                        hword[j] = head[j] ? ld_bits(&bits[rel >> 5]) : 0u;
-                        hacc[j] = head[j] ? (uint64_t)__double_as_longlong(ld_acc(&acc[rel])) : 0ull;
+                        if (LDSB) {
+                            hword[j] = head[j] ? s_lbits[rel >> 5] : 0u;
+                            const bool present = (hword[j] >> (rel & 31u)) & 1u;
+                            hacc[j] = present ? (uint64_t)__double_as_longlong(ld_acc(&acc[rel])) : 0ull;
+                        } else {
+                            hword[j] = head[j] ? ld_bits(&bits[rel >> 5]) : 0u;
+                            hacc[j] = head[j] ? (uint64_t)__double_as_longlong(ld_acc(&acc[rel])) : 0ull;
+                        }
                     }
                     load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, pcol, pbv, pent);
 #pragma unroll
@@ -465,7 +479,10 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                         double v = present ? old + s_val[y0] : s_val[y0];
                         for (int y = y0 + 1; y < V && (uint32_t)(s_key[y] >> 32) == hcol[j]; ++y) v = v + s_val[y];
                         st_acc(&acc[rel], v);
-                        if (!present) atomicOr(&bits[rel >> 5], bit);
+                        if (!present) {
+                            if (LDSB) atomicOr(&s_lbits[rel >> 5], bit);
+                            else atomicOr(&bits[rel >> 5], bit);
+                        }
                         zc += (int)(v == 0.0) - (int)(present && old == 0.0);
                     }
                     __syncthreads();  // chunk folded; LDS and accumulator state consistent
@@ -485,13 +502,14 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                 for (int64_t q = q0; q < q1; q += kExtract) {
                     uint32_t wd[kExtract];
 #pragma unroll
-                    for (int u = 0; u < kExtract; ++u) wd[u] = q + u < q1 ? ld_bits(&bits[q + u]) : 0u;
+                    for (int u = 0; u < kExtract; ++u)
+                        wd[u] = q + u < q1 ? (LDSB ? s_lbits[q + u] : ld_bits(&bits[q + u])) : 0u;
 #pragma unroll
                     for (int u = 0; u < kExtract; ++u) mine += __popc(wd[u]);
                 }
             } else {
                 for (int64_t q = q0; q < q1; ++q) {
-                    uint32_t m = ld_bits(&bits[q]);
+                    uint32_t m = LDSB ? s_lbits[q] : ld_bits(&bits[q]);
                     while (m) {
                         const int b = __ffs((int)m) - 1;
                         m &= m - 1u;
@@ -501,9 +519,9 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
             }
             int64_t wtotal;
             int64_t o = outn + block_excl_scan<TPB>(mine, s_ws64, wtotal);
-            uint32_t m = q0 < q1 ? ld_bits(&bits[q0]) : 0u;
+            uint32_t m = q0 < q1 ? (LDSB ? s_lbits[q0] : ld_bits(&bits[q0])) : 0u;
             for (int64_t q = q0; q < q1; ++q) {
-                const uint32_t nxt = q + 1 < q1 ? ld_bits(&bits[q + 1]) : 0u;
+                const uint32_t nxt = q + 1 < q1 ? (LDSB ? s_lbits[q + 1] : ld_bits(&bits[q + 1])) : 0u;
                 while (m) {
                     int b[kExtract];
                     double v[kExtract];
@@ -528,7 +546,8 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                         }
                     }
                 }
-                st_bits(&bits[q], 0u);
+                if (LDSB) s_lbits[q] = 0u;
+                else st_bits(&bits[q], 0u);
                 m = nxt;
             }
             outn += wtotal;
@@ -568,8 +587,12 @@ void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_coun
         launch_esc<kTpbBlock, kCapBlock>(op, ws, kBinBlock, bin_counts[kBinBlock], cus, crowptr, ccol, cval, stream);
     if ((bins & (1u << kBinLarge)) && bin_counts[kBinLarge] > 0) {
         const int g = bin_counts[kBinLarge] < ws.lblocks ? bin_counts[kBinLarge] : ws.lblocks;
-        hipLaunchKernelGGL((k_large_numeric<kTpbLarge, kCapLarge>), dim3(g), dim3(kTpbLarge), 0, stream, op, ws,
-                           crowptr, ccol, cval);
+        if (ws.lwidth <= kLdsBitsCols)
+            hipLaunchKernelGGL((k_large_numeric<kTpbLarge, kCapLarge, true>), dim3(g), dim3(kTpbLarge),
+                               (unsigned)(ws.lwidth / 8), stream, op, ws, crowptr, ccol, cval);
+        else
+            hipLaunchKernelGGL((k_large_numeric<kTpbLarge, kCapLarge, false>), dim3(g), dim3(kTpbLarge), 0, stream, op,
+                               ws, crowptr, ccol, cval);
     }
 }
 

@@ -11,8 +11,9 @@
 // Rows whose columns span fewer than 32 x the table's words (at least 65536
 // columns for rows of up to 512 products) count with a bitmap over their
 // column range in the table's storage instead: one LDS OR per product, no
-// probing.  Large rows set bits in a per-workgroup column
-// bitmap in HBM (window by window) and count the bits they newly set.
+// probing.  Large rows set bits in a per-workgroup column bitmap (in LDS for
+// windows of up to 2^18 columns, else in HBM), window by window, and count
+// the bits they newly set.
 #include "device_common.hpp"
 
 namespace spg {
@@ -214,8 +215,12 @@ __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Wo
     return Pb;
 }
 
-template <int TPB, int CAP>
+// LDSB: the window's bitmap lives in dynamic LDS (lwidth / 8 bytes) and the
+// claims are LDS atomics; else it is the workgroup's bitmap in HBM.
+template <int TPB, int CAP, bool LDSB>
 __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace ws) {
+    constexpr int UNROLL = LDSB ? 2 * kLargeUnroll : kLargeUnroll;
+    extern __shared__ uint32_t s_lbits[];
     __shared__ int64_t s_bst[CAP];
     __shared__ int64_t s_off[CAP + 1];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
@@ -226,6 +231,9 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
     uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
     unsigned flags = 0;
     unsigned long long sym = 0;
+    if (LDSB) {
+        for (int64_t q = tid; q < W / 32; q += TPB) s_lbits[q] = 0u;
+    }
     __shared__ int s_next;
     for (int i; (i = next_large_row(&ws.ctrl->large_next[0], &s_next)) < count;) {
         const int row = list[i].row;
@@ -236,12 +244,12 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
             for (int64_t e0 = a0; e0 < a1; e0 += CAP) {
                 const int nbt = (int)min((int64_t)CAP, a1 - e0);
                 const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, nullptr, s_off, s_ws64, false);
-                // kLargeUnroll products per thread per step: every column load,
+                // UNROLL products per thread per step: every column load,
                 // then every bit claim, in flight together.
-                for (int64_t s0 = tid; s0 < Pb; s0 += (int64_t)TPB * kLargeUnroll) {
-                    int64_t c[kLargeUnroll];
+                for (int64_t s0 = tid; s0 < Pb; s0 += (int64_t)TPB * UNROLL) {
+                    int64_t c[UNROLL];
 #pragma unroll
-                    for (int u = 0; u < kLargeUnroll; ++u) {
+                    for (int u = 0; u < UNROLL; ++u) {
                         const int64_t s = s0 + (int64_t)u * TPB;
                         c[u] = -1;
                         if (s < Pb) {
@@ -249,22 +257,26 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                             c[u] = op.bcol[s_bst[e] + (s - s_off[e])];
                         }
                     }
-                    uint32_t old[kLargeUnroll], bit[kLargeUnroll];
-                    bool in[kLargeUnroll];
+                    uint32_t old[UNROLL], bit[UNROLL];
+                    bool in[UNROLL];
 #pragma unroll
-                    for (int u = 0; u < kLargeUnroll; ++u) {
+                    for (int u = 0; u < UNROLL; ++u) {
                         in[u] = c[u] >= w0 && c[u] < wend;
                         const uint32_t rel = in[u] ? (uint32_t)(c[u] - w0) : 0u;
                         bit[u] = 1u << (rel & 31u);
-                        old[u] = in[u] ? atomicOr(&bits[rel >> 5], bit[u]) : bit[u];
+                        uint32_t *wp = LDSB ? &s_lbits[rel >> 5] : &bits[rel >> 5];
+                        old[u] = in[u] ? atomicOr(wp, bit[u]) : bit[u];
                     }
 #pragma unroll
-                    for (int u = 0; u < kLargeUnroll; ++u) cnt += in[u] && !(old[u] & bit[u]);
+                    for (int u = 0; u < UNROLL; ++u) cnt += in[u] && !(old[u] & bit[u]);
                 }
                 __syncthreads();  // batch metadata is overwritten next
             }
             const int64_t nwords = (wend - w0 + 31) / 32;
-            for (int64_t q = tid; q < nwords; q += TPB) st_bits(&bits[q], 0u);
+            for (int64_t q = tid; q < nwords; q += TPB) {
+                if (LDSB) s_lbits[q] = 0u;
+                else st_bits(&bits[q], 0u);
+            }
             __threadfence();
             __syncthreads();
         }
@@ -292,8 +304,12 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
     if (bins & (1u << kBinW3)) launch_hash<kSymTpbW3, kCapW3>(op, ws, kBinW3, cus, stream);
     if (bins & (1u << kBinBlock)) launch_hash<kSymTpbBlock, kCapBlock>(op, ws, kBinBlock, cus, stream);
     if ((bins & (1u << kBinLarge)) && ws.lblocks > 0)
-        hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge>), dim3(ws.lblocks), dim3(kTpbLarge), 0, stream, op,
-                           ws);
+        if (ws.lwidth <= kLdsBitsCols)
+            hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, true>), dim3(ws.lblocks), dim3(kTpbLarge),
+                               (unsigned)(ws.lwidth / 8), stream, op, ws);
+        else
+            hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, false>), dim3(ws.lblocks), dim3(kTpbLarge), 0,
+                               stream, op, ws);
 }
 
 }  // namespace spg
