@@ -34,11 +34,16 @@ constexpr int kTpbBlock = 256;
 #define SPG_SYM_TPB_BLOCK 256
 #endif
 constexpr int kSymTpbW3 = SPG_SYM_TPB_W3, kSymTpbBlock = SPG_SYM_TPB_BLOCK;
-constexpr int kCapLarge = 1024;  // products per chunk on the large-row path
-constexpr int kTpbLarge = 256;
-// Column windows of up to this many columns keep the large-row presence
-// bitmap in LDS (32 KB) instead of HBM.
+// Large-row path: column windows of up to kLdsBitsCols columns keep the
+// presence bitmap in LDS (32 KB), wider windows in HBM.  Products per chunk
+// and workgroup size per case (measured: 1024/256 is 11 % faster than
+// 2048/512 on R-MAT x B(65536 columns), 2048/512 13 % faster on R-MAT A.A with
+// its 2^20-column windows).
 constexpr int64_t kLdsBitsCols = int64_t(1) << 18;
+constexpr int kCapLarge = 1024;
+constexpr int kTpbLarge = 256;
+constexpr int kCapLargeHbm = 2048;
+constexpr int kTpbLargeHbm = 512;
 // Single-pass path: rows with at most kFusedCap products and A entries are
 // computed by one wave each, kFusedRows consecutive rows per wave, in tiles of
 // (kFusedTpb / 64) * kFusedRows rows (these are exactly the rows of bins W1, W2).

@@ -303,13 +303,14 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
     if (bins & (1u << kBinW2)) launch_hash<64, kCapW2>(op, ws, kBinW2, cus, stream);
     if (bins & (1u << kBinW3)) launch_hash<kSymTpbW3, kCapW3>(op, ws, kBinW3, cus, stream);
     if (bins & (1u << kBinBlock)) launch_hash<kSymTpbBlock, kCapBlock>(op, ws, kBinBlock, cus, stream);
-    if ((bins & (1u << kBinLarge)) && ws.lblocks > 0)
+    if ((bins & (1u << kBinLarge)) && ws.lblocks > 0) {
         if (ws.lwidth <= kLdsBitsCols)
             hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, true>), dim3(ws.lblocks), dim3(kTpbLarge),
                                (unsigned)(ws.lwidth / 8), stream, op, ws);
         else
-            hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, false>), dim3(ws.lblocks), dim3(kTpbLarge), 0,
-                               stream, op, ws);
+            hipLaunchKernelGGL((k_large_symbolic<kTpbLargeHbm, kCapLargeHbm, false>), dim3(ws.lblocks),
+                               dim3(kTpbLargeHbm), 0, stream, op, ws);
+    }
 }
 
 }  // namespace spg
