@@ -42,6 +42,10 @@ constexpr int kSymTpbW3 = SPG_SYM_TPB_W3, kSymTpbBlock = SPG_SYM_TPB_BLOCK;
 constexpr int64_t kLdsBitsCols = int64_t(1) << 18;
 constexpr int kCapLarge = 1024;
 constexpr int kTpbLarge = 256;
+// The count kernel also keeps windows of up to 2^20 columns (128 KB) in LDS,
+// one 1024-thread workgroup per CU.
+constexpr int64_t kLdsBitsColsWide = int64_t(1) << 20;
+constexpr int kTpbLargeWide = 1024;
 constexpr int kCapLargeHbm = 2048;
 constexpr int kTpbLargeHbm = 512;
 // Single-pass path: rows with at most kFusedCap products and A entries are
@@ -164,6 +168,9 @@ constexpr unsigned kAllBins = kSmallBins | kBigBins;
 // Symbolic phase for the selected bins (symbolic.hip); kernels read their row
 // counts from the control block, so no host synchronisation is needed before it.
 void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream);
+// Per-process kernel attributes (dynamic LDS limits); called when a context
+// is created, with the device's LDS bytes per workgroup.
+hipError_t prepare_kernels(size_t lds_per_block);
 // Single-pass kernel over all rows (fused.hip): computes the rows of bins W1,
 // W2 and reserves the symbolic counts of the others.  small_b: every B entry
 // index i has 8*i < 2^32.

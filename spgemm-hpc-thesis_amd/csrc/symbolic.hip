@@ -16,6 +16,8 @@
 // the bits they newly set.
 #include "device_common.hpp"
 
+#include <atomic>
+
 namespace spg {
 namespace {
 
@@ -295,7 +297,31 @@ void launch_hash(const Operands &op, const Workspace &ws, int bin, int cus, hipS
     hipLaunchKernelGGL(k, dim3(g), dim3(TPB), 0, stream, op, ws, bin);
 }
 
+// Set once per process by prepare_kernels: the device gives a workgroup
+// enough LDS for the wide-window count.
+std::atomic<bool> g_wide_lds{false};
+
 }  // namespace
+
+hipError_t prepare_kernels(size_t lds_per_block) {
+    // The wide-window count keeps a bitmap of up to 2^20 columns (128 KB) in
+    // dynamic LDS beside its static arrays; without that much LDS per
+    // workgroup it is not used.
+    const size_t dyn = (size_t)(kLdsBitsColsWide / 8);
+    const size_t stat = sizeof(int64_t) * (2 * kCapLarge + 1) + sizeof(int64_t) * (kTpbLargeWide / 64 + 1) + 64;
+    bool ok = lds_per_block >= dyn + stat;
+    if (ok) {
+        const hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&k_large_symbolic<kTpbLargeWide, kCapLarge, true>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ok = false;
+        }
+    }
+    g_wide_lds.store(ok, std::memory_order_relaxed);
+    return hipSuccess;
+}
 
 void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream) {
     if (op.arows <= 0) return;
@@ -307,6 +333,9 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
         if (ws.lwidth <= kLdsBitsCols)
             hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, true>), dim3(ws.lblocks), dim3(kTpbLarge),
                                (unsigned)(ws.lwidth / 8), stream, op, ws);
+        else if (ws.lwidth <= kLdsBitsColsWide && g_wide_lds.load(std::memory_order_relaxed))
+            hipLaunchKernelGGL((k_large_symbolic<kTpbLargeWide, kCapLarge, true>), dim3(ws.lblocks),
+                               dim3(kTpbLargeWide), (unsigned)(ws.lwidth / 8), stream, op, ws);
         else
             hipLaunchKernelGGL((k_large_symbolic<kTpbLargeHbm, kCapLargeHbm, false>), dim3(ws.lblocks),
                                dim3(kTpbLargeHbm), 0, stream, op, ws);

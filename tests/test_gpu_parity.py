@@ -285,17 +285,18 @@ def pm1_csr(rng, rows, cols, per_row):
     return O.Csr.make(rows, cols, rp, col, val)
 
 
-@pytest.mark.parametrize("spread", [1, 4099])
-def test_large_rows_exact_zero_sums(engine, spread):
+@pytest.mark.parametrize("spread,cols", [(1, 64), (4099, 300000), (16411, 1200000)])
+def test_large_rows_exact_zero_sums(engine, spread, cols):
     # 40 rows of 24,000 products into 64 distinct columns with values +-1:
     # running sums cross zero between chunks and some end at exactly zero, so
     # the large-row extraction must drop them; whole product bit-exact against
-    # the oracle.  Spread 4099 places the 64 columns up to 258,237 of 300,000
-    # (more than 2^18 columns: the presence bitmap is in HBM, not LDS).
+    # the oracle.  The 64 columns are spread over 64, 300,000 and 1,200,000
+    # columns: the presence bitmaps are in LDS; in HBM for the values and in
+    # LDS (wide window) for the count; in HBM for both.
     rng = np.random.Generator(np.random.PCG64(31))
     a, b = pm1_csr(rng, 40, 4000, 3000), pm1_csr(rng, 4000, 64, 8)
     if spread > 1:
-        b = O.Csr.make(b.rows, 300000, b.row_ptr, b.col.astype(np.int64) * spread, b.val)
+        b = O.Csr.make(b.rows, cols, b.row_ptr, b.col.astype(np.int64) * spread, b.val)
     print("case spread", spread, "cols", b.cols, flush=True)
     c = engine.multiply(to_s(a), to_s(b))
     assert engine.stats()["rows_per_bin"][5] == 40
