@@ -35,39 +35,38 @@ __global__ void k_dir32(const int64_t *bptr, int64_t n, uint32_t *out) {
         out[i] = (uint32_t)bptr[i];
 }
 
-// One wave: rows tile*64 .. tile*64+63.  DIR32: B row k is
-// [bdir32[k], bdir32[k+1]) (B's own row_ptr, 32-bit copy).
+// Row tiles of 64 rows, one per wave.  A tile with more than kAnHeavy
+// entries (R-MAT's first rows hold hundreds of thousands) is walked by all
+// waves of its workgroup together, each taking a contiguous share of its
+// entries, so no single wave carries it.
+constexpr int64_t kAnHeavy = 16384;
+
+// Entries [S, T) of a row tile (lane l holds the tile's row l, starting at
+// entry a0; rows 0..last exist).  DIR32: B row k is [bdir32[k], bdir32[k+1])
+// (B's own row_ptr, 32-bit copy).  Writes each entry's (B row start, length)
+// and adds the lengths into s_ub[row] (LDS atomics: several waves may walk
+// one tile).  s_mark is this wave's scratch.
 template <bool DIR32>
-__device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace &ws, int64_t tile,
-                                             unsigned long long *s_ub, int *s_mark, int64_t *s_part) {
+__device__ __forceinline__ unsigned tile_entries(const Operands &op, const Workspace &ws, int last, int64_t a0,
+                                                 int64_t S, int64_t T, unsigned long long *s_ub, int *s_mark) {
     const int lane = lane_id();
-    const int64_t r = tile * 64 + lane;
-    const bool valid = r < op.arows;
-    const int last = (int)min((int64_t)63, (int64_t)op.arows - 1 - tile * 64);
-    const int64_t a0 = valid ? op.aptr[r] : 0;
-    const int64_t a1 = valid ? op.aptr[r + 1] : 0;
-    const int64_t E0 = __shfl(a0, 0, 64);
-    const int64_t E1 = __shfl(a1, last, 64);
-    s_ub[lane] = 0ull;
-    wave_sync();
     unsigned flags = 0;
     // kAnChunks chunks of 64 entries per step; every chunk's column loads and
     // B-row bound gathers are issued (unconditionally, at clamped indices)
     // before any chunk is reduced.
-    if (op.nnz_a <= 0 && E1 > E0) flags |= kErrInternal;  // row_ptr inconsistent with nnz(A)
-    for (int64_t base = E0; op.nnz_a > 0 && base < E1; base += 64 * kAnChunks) {
+    for (int64_t base = S; op.nnz_a > 0 && base < T; base += 64 * kAnChunks) {
         int64_t ev[kAnChunks], sv[kAnChunks], lv[kAnChunks];
         bool inv[kAnChunks];
         int32_t kv[kAnChunks];
 #pragma unroll
         for (int h = 0; h < kAnChunks; ++h) {
             ev[h] = base + 64 * h + lane;
-            inv[h] = ev[h] < E1;
+            inv[h] = ev[h] < T;
             if (inv[h] && ev[h] >= op.nnz_a) {  // row_ptr inconsistent with nnz(A)
                 flags |= kErrInternal;
                 inv[h] = false;
             }
-            const int64_t ec = max((int64_t)0, min(ev[h], min(E1, op.nnz_a) - 1));
+            const int64_t ec = max((int64_t)0, min(ev[h], min(T, op.nnz_a) - 1));
             kv[h] = __builtin_nontemporal_load(&op.acol[ec]);
         }
         int64_t bs[kAnChunks], be[kAnChunks];
@@ -123,7 +122,16 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
             if (inv[h]) atomicAdd(&s_ub[row], (unsigned long long)lv[h]);
         }
     }
-    wave_sync();
+    return flags;
+}
+
+// Classification of a tile's rows from their summed bounds: ub, bins, bin
+// lists and the wave's partial sums.
+__device__ __forceinline__ void tile_finish(const Operands &op, const Workspace &ws, int64_t tile, int64_t a0,
+                                            int64_t a1, const unsigned long long *s_ub, int64_t *s_part) {
+    const int lane = lane_id();
+    const int64_t r = tile * 64 + lane;
+    const bool valid = r < op.arows;
     const int64_t ub = (int64_t)s_ub[lane];
     int b = -1;
     if (valid) {
@@ -136,13 +144,9 @@ __device__ __forceinline__ void analyze_tile(const Operands &op, const Workspace
     }
     const int64_t tot = wave_sum(valid ? ub : (int64_t)0);
     const int64_t small = wave_sum((b == kBinW1 || b == kBinW2) ? ub : (int64_t)0);
-    unsigned long long fl = flags;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) fl |= __shfl_xor(fl, d, 64);
     if (lane == 0) {
         s_part[0] = tot;
         s_part[1] = small;
-        if (fl) atomicOr(&ws.ctrl->flags, fl);
     }
 #pragma unroll
     for (int bb = 0; bb < kNumBins; ++bb) {
@@ -170,11 +174,47 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
     __shared__ unsigned long long s_ub[NW][64];
     __shared__ int s_mark[NW][64];
     __shared__ int64_t s_part[NW][kAnalyzeFields];
+    __shared__ int64_t s_span[NW][2];  // entry range of each wave's tile when heavy, else empty
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
     const int64_t tile = (int64_t)blockIdx.x * NW + w;
+    const bool has = tile * 64 < op.arows;  // uniform per wave
     if (lane < kAnalyzeFields) s_part[w][lane] = 0;
-    if (tile * 64 < op.arows) analyze_tile<DIR32>(op, ws, tile, s_ub[w], s_mark[w], s_part[w]);  // uniform per wave
+    const int64_t r = tile * 64 + lane;
+    const bool valid = r < op.arows;
+    const int last = has ? (int)min((int64_t)63, (int64_t)op.arows - 1 - tile * 64) : 0;
+    const int64_t a0 = valid ? op.aptr[r] : 0;
+    const int64_t a1 = valid ? op.aptr[r + 1] : 0;
+    const int64_t E0 = __shfl(a0, 0, 64);
+    const int64_t E1 = __shfl(a1, last, 64);
+    s_ub[w][lane] = 0ull;
+    unsigned flags = 0;
+    if (has && op.nnz_a <= 0 && E1 > E0) flags |= kErrInternal;  // row_ptr inconsistent with nnz(A)
+    const bool heavy = has && E1 - E0 > kAnHeavy;
+    if (lane == 0) {
+        s_span[w][0] = heavy ? E0 : 0;
+        s_span[w][1] = heavy ? E1 : 0;
+    }
+    wave_sync();
+    if (has && !heavy) flags |= tile_entries<DIR32>(op, ws, last, a0, E0, E1, s_ub[w], s_mark[w]);
+    __syncthreads();
+    for (int h = 0; h < NW; ++h) {  // heavy tiles of the workgroup, walked by every wave
+        const int64_t hE0 = s_span[h][0], hE1 = s_span[h][1];
+        if (hE1 <= hE0) continue;  // uniform across the workgroup
+        const int64_t htile = (int64_t)blockIdx.x * NW + h;
+        const int64_t hr = htile * 64 + lane;
+        const int hlast = (int)min((int64_t)63, (int64_t)op.arows - 1 - htile * 64);
+        const int64_t ha0 = hr < op.arows ? op.aptr[hr] : 0;
+        const int64_t per = ((hE1 - hE0 + NW - 1) / NW + 63) / 64 * 64;
+        const int64_t S = hE0 + (int64_t)w * per, T = min(S + per, hE1);
+        if (S < T) flags |= tile_entries<DIR32>(op, ws, hlast, ha0, S, T, s_ub[h], s_mark[w]);
+    }
+    __syncthreads();
+    if (has) tile_finish(op, ws, tile, a0, a1, s_ub[w], s_part[w]);
+    unsigned long long fl = flags;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) fl |= __shfl_xor(fl, d, 64);
+    if (lane == 0 && fl) atomicOr(&ws.ctrl->flags, fl);
     __syncthreads();
     if (threadIdx.x < kAnalyzeFields) {
         int64_t v = 0;

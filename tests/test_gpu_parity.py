@@ -305,6 +305,28 @@ def test_large_rows_exact_zero_sums(engine, spread, cols):
     assert digest(c) == O.csr_digest(ref)
 
 
+def test_heavy_row_tiles_in_analyze(engine):
+    # Rows 0..63 hold 400 entries each (25,600 entries: one tile walked by all
+    # waves of its workgroup), the other rows 5; whole product bit-exact, and
+    # a column outside B's rows in the last quarter of the heavy tile's
+    # entries is reported.
+    rng = np.random.Generator(np.random.PCG64(34))
+    lens = np.where(np.arange(300) < 64, 400, 5)
+    rp = np.zeros(301, dtype=np.int64)
+    np.cumsum(lens, out=rp[1:])
+    col = rng.integers(0, 3000, size=int(rp[-1]))
+    a = O.Csr.make(300, 3000, rp, col, 2.0 * rng.random(int(rp[-1])) - 1.0)
+    b = O.Csr.make(3000, 5000, np.arange(3001, dtype=np.int64) * 7, rng.integers(0, 5000, size=21000),
+                   2.0 * rng.random(21000) - 1.0)
+    c = engine.multiply(to_s(a), to_s(b))
+    assert digest(c) == O.csr_digest(O.product(a, b))
+    bad = col.copy()
+    bad[int(rp[64]) - 3] = 3000 + 11
+    with pytest.raises(S.SpgError) as e:
+        engine.multiply(S.Csr.make(300, 3000, rp, bad, a.val), to_s(b))
+    assert e.value.name == "SPG_ERANGE"
+
+
 def test_large_rows_heaviest_first_order(engine):
     # Large rows of very different work, heaviest last in row order: the
     # engine reorders them by work; every row is still the oracle's.
