@@ -46,6 +46,9 @@ constexpr int kTpbLarge = 256;
 // one 1024-thread workgroup per CU.
 constexpr int64_t kLdsBitsColsWide = int64_t(1) << 20;
 constexpr int kTpbLargeWide = 1024;
+// The value kernel fits such a bitmap beside 512-product chunk arrays.
+constexpr int kCapLargeWideNum = 512;
+constexpr int kTpbLargeWideNum = 512;
 constexpr int kCapLargeHbm = 2048;
 constexpr int kTpbLargeHbm = 512;
 // Single-pass path: rows with at most kFusedCap products and A entries are
@@ -171,6 +174,7 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
 // Per-process kernel attributes (dynamic LDS limits); called when a context
 // is created, with the device's LDS bytes per workgroup.
 hipError_t prepare_kernels(size_t lds_per_block);
+bool prepare_numeric_kernels(size_t lds_per_block);  // numeric.hip; false: wide LDS path unavailable
 // Single-pass kernel over all rows (fused.hip): computes the rows of bins W1,
 // W2 and reserves the symbolic counts of the others.  small_b: every B entry
 // index i has 8*i < 2^32.

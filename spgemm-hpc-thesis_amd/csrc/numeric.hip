@@ -22,6 +22,8 @@
 // windows), so a column's running sum continues where the previous chunk left it.
 #include "device_common.hpp"
 
+#include <atomic>
+
 namespace spg {
 namespace {
 
@@ -575,7 +577,26 @@ void launch_esc(const Operands &op, const Workspace &ws, int bin, int rows, int 
     hipLaunchKernelGGL(k, dim3(g), dim3(TPB), 0, stream, op, ws, bin, crowptr, ccol, cval);
 }
 
+// Set once per process by prepare_numeric_kernels.
+std::atomic<bool> g_num_wide{false};
+
 }  // namespace
+
+bool prepare_numeric_kernels(size_t lds_per_block) {
+    const size_t dyn = (size_t)(kLdsBitsColsWide / 8);
+    constexpr int C = kCapLargeWideNum;
+    const size_t stat = (size_t)C * (8 + 8 + 8 + 8 + 8 + 4) + 64 * 8 + 64;
+    if (lds_per_block < dyn + stat) return false;
+    const hipError_t e =
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_large_numeric<kTpbLargeWideNum, kCapLargeWideNum, true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    g_num_wide.store(true, std::memory_order_relaxed);
+    return true;
+}
 
 void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, unsigned bins, int cus,
                     const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream) {
@@ -592,6 +613,9 @@ void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_coun
         if (ws.lwidth <= kLdsBitsCols)
             hipLaunchKernelGGL((k_large_numeric<kTpbLarge, kCapLarge, true>), dim3(g), dim3(kTpbLarge),
                                (unsigned)(ws.lwidth / 8), stream, op, ws, crowptr, ccol, cval);
+        else if (ws.lwidth <= kLdsBitsColsWide && g_num_wide.load(std::memory_order_relaxed))
+            hipLaunchKernelGGL((k_large_numeric<kTpbLargeWideNum, kCapLargeWideNum, true>), dim3(g),
+                               dim3(kTpbLargeWideNum), (unsigned)(ws.lwidth / 8), stream, op, ws, crowptr, ccol, cval);
         else
             hipLaunchKernelGGL((k_large_numeric<kTpbLargeHbm, kCapLargeHbm, false>), dim3(g), dim3(kTpbLargeHbm), 0,
                                stream, op, ws, crowptr, ccol, cval);

@@ -320,6 +320,7 @@ hipError_t prepare_kernels(size_t lds_per_block) {
         }
     }
     g_wide_lds.store(ok, std::memory_order_relaxed);
+    (void)prepare_numeric_kernels(lds_per_block);
     return hipSuccess;
 }
 
