@@ -323,7 +323,8 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                                                        int32_t *ccol, double *cval) {
     constexpr int ITEMS = CAP / TPB;
     extern __shared__ uint32_t s_lbits[];
-    constexpr int kExtract = 8;  // accumulator loads in flight per thread during extraction
+    constexpr int kExtract = 8;      // accumulator loads in flight per thread during extraction
+    constexpr int kExtractLds = 16;  // the same with the bitmap in LDS
     __shared__ int64_t s_bst[CAP];
     __shared__ double s_av[CAP];
     __shared__ int64_t s_off[CAP + 1];
@@ -523,9 +524,49 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
             }
             int64_t wtotal;
             int64_t o = outn + block_excl_scan<TPB>(mine, s_ws64, wtotal);
-            uint32_t m = q0 < q1 ? (LDSB ? s_lbits[q0] : ld_bits(&bits[q0])) : 0u;
+            if (LDSB) {
+                // Set bits are taken across the thread's words in column
+                // order, kExtractLds sums loaded per round trip.
+                int64_t q = q0;
+                uint32_t m = q0 < q1 ? s_lbits[q0] : 0u;
+                for (;;) {
+                    int rel[kExtractLds];
+                    double v[kExtractLds];
+#pragma unroll
+                    for (int u = 0; u < kExtractLds; ++u) {
+                        while (m == 0u && q + 1 < q1) {
+                            s_lbits[q] = 0u;
+                            m = s_lbits[++q];
+                        }
+                        rel[u] = -1;
+                        if (m) {
+                            rel[u] = (int)((q - q0) * 32) + __ffs((int)m) - 1;
+                            m &= m - 1u;
+                        }
+                    }
+                    if (rel[0] < 0) break;
+#pragma unroll
+                    for (int u = 0; u < kExtractLds; ++u)
+                        v[u] = rel[u] >= 0 ? ld_acc(&acc[q0 * 32 + rel[u]]) : 0.0;
+#pragma unroll
+                    for (int u = 0; u < kExtractLds; ++u) {
+                        if (rel[u] >= 0 && (op.keep_zeros || v[u] != 0.0)) {
+                            if (o < lim) {
+                                ccol[cbase + o] = (int32_t)(w0 + q0 * 32 + rel[u]);
+                                cval[cbase + o] = v[u];
+                            }
+                            ++o;
+                        }
+                    }
+                }
+                if (q < q1) s_lbits[q] = 0u;
+                outn += wtotal;
+                __syncthreads();
+                continue;
+            }
+            uint32_t m = q0 < q1 ? ld_bits(&bits[q0]) : 0u;
             for (int64_t q = q0; q < q1; ++q) {
-                const uint32_t nxt = q + 1 < q1 ? (LDSB ? s_lbits[q + 1] : ld_bits(&bits[q + 1])) : 0u;
+                const uint32_t nxt = q + 1 < q1 ? ld_bits(&bits[q + 1]) : 0u;
                 while (m) {
                     int b[kExtract];
                     double v[kExtract];
@@ -550,8 +591,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                         }
                     }
                 }
-                if (LDSB) s_lbits[q] = 0u;
-                else st_bits(&bits[q], 0u);
+                st_bits(&bits[q], 0u);
                 m = nxt;
             }
             outn += wtotal;
