@@ -75,6 +75,25 @@ __device__ __forceinline__ T block_excl_scan(T v, T *ws, T &total) {
     }
 }
 
+// Exclusive scan of s[0, n) in place, n a multiple of TPB: each thread sums
+// n / TPB consecutive counters, one block scan orders the threads.  Returns
+// the total; the written offsets are visible after the caller's next barrier.
+template <int TPB, typename T>
+__device__ __forceinline__ T block_scan_runs(T *s, int n, T *ws) {
+    const int per = n / TPB;
+    T *mine = s + threadIdx.x * per;
+    T loc = 0;
+    for (int k = 0; k < per; ++k) loc += mine[k];
+    T total;
+    T run = block_excl_scan<TPB>(loc, ws, total);
+    for (int k = 0; k < per; ++k) {
+        const T c = mine[k];
+        mine[k] = run;
+        run += c;
+    }
+    return total;
+}
+
 template <int TPB, typename T>
 __device__ __forceinline__ T block_sum(T v, T *ws) {
     T total;

@@ -98,17 +98,7 @@ __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws,
         }
     }
     __syncthreads();
-    {
-        int run = 0;
-        for (int q = 0; q < nb; q += TPB) {
-            const int b = q + tid;
-            const int c = s_c[b];
-            int tot;
-            const int ex = block_excl_scan<TPB>(c, s_wsi, tot);
-            s_c[b] = run + ex;
-            run += tot;
-        }
-    }
+    block_scan_runs<TPB>(s_c, nb, s_wsi);
     if (tid == 0) s_c[nb] = P;
     __syncthreads();
 #pragma unroll
@@ -406,17 +396,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                         }
                     }
                     __syncthreads();
-                    {
-                        int run = 0;
-                        for (int q = 0; q < nb; q += TPB) {
-                            const int b = q + tid;
-                            const int c = s_cnt[b];
-                            int tot;
-                            const int ex = block_excl_scan<TPB>(c, s_wsi, tot);
-                            s_cnt[b] = run + ex;
-                            run += tot;
-                        }
-                    }
+                    block_scan_runs<TPB>(s_cnt, nb, s_wsi);
                     if (tid == 0) s_cnt[nb] = V;
                     __syncthreads();
                     uint64_t key[ITEMS];
