@@ -229,6 +229,7 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
     const int tid = threadIdx.x;
     const RowRec *list = large_list(ws);
     const int count = ws.ctrl->bin_count[kBinLarge];
+    if (count == 0) return;  // launched before the host knows the bins; no row takes a ticket
     const int64_t W = ws.lwidth;
     uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
     unsigned flags = 0;
