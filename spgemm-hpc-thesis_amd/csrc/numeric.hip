@@ -318,8 +318,8 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
     __shared__ int64_t s_bst[CAP];
     __shared__ double s_av[CAP];
     __shared__ int64_t s_off[CAP + 1];
-    __shared__ uint64_t s_key[CAP];
-    __shared__ double s_val[CAP];
+    __shared__ uint64_t s_key[2][CAP];  // sorted chunk keys, two chunks in flight
+    __shared__ double s_val[2][CAP];
     __shared__ int s_cnt[CAP + 1];
     __shared__ int s_wsi[2 * (TPB / 64) + 2];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
@@ -346,11 +346,50 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
             // This thread's net change of the number of the window's columns
             // whose running sum is exactly zero.
             int zc = 0;
+            // The chunk whose heads' running sums are in flight.  It is folded
+            // after the next chunk has been bucketed, ranked and placed in the
+            // other buffer, so the accumulator round trip overlaps that work.
+            int pend_v = 0;  // its valid products (0: none pending)
+            int pbuf = 0;    // its sorted keys and values: s_key[pbuf], s_val[pbuf]
+            int cbuf = 0;    // buffer of the next chunk
+            bool head[ITEMS];
+            uint32_t hcol[ITEMS], hword[ITEMS];
+            uint64_t hacc[ITEMS];
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                head[j] = false;
+                hcol[j] = hword[j] = 0u;
+                hacc[j] = 0ull;
+            }
+            // Each head continues its column's running sum (the accumulator,
+            // when an earlier chunk touched it, else its first product) with
+            // the column's products in serial order.
+            auto fold_pending = [&]() {
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    if (!head[j]) continue;
+                    const int y0 = tid + j * TPB;
+                    const uint32_t rel = (uint32_t)((int64_t)(int32_t)hcol[j] - w0);
+                    const uint32_t bit = 1u << (rel & 31u);
+                    const bool present = (hword[j] & bit) != 0u;
+                    const double old = __longlong_as_double((long long)hacc[j]);
+                    double v = present ? old + s_val[pbuf][y0] : s_val[pbuf][y0];
+                    for (int y = y0 + 1; y < pend_v && (uint32_t)(s_key[pbuf][y] >> 32) == hcol[j]; ++y)
+                        v = v + s_val[pbuf][y];
+                    st_acc(&acc[rel], v);
+                    if (!present) {
+                        if (LDSB) atomicOr(&s_lbits[rel >> 5], bit);
+                        else atomicOr(&bits[rel >> 5], bit);
+                    }
+                    zc += (int)(v == 0.0) - (int)(present && old == 0.0);
+                    head[j] = false;
+                }
+                pend_v = 0;
+            };
             for (int64_t e0 = a0; e0 < a1; e0 += CAP) {
                 const int nbt = (int)min((int64_t)CAP, a1 - e0);
                 const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, s_av, s_off, s_ws64);
-                // Each chunk's products are loaded one chunk ahead, so their
-                // latency overlaps the previous chunk's accumulator round trip.
+                // Each chunk's products are loaded one chunk ahead.
                 int pcol[ITEMS], pent[ITEMS];
                 double pbv[ITEMS];
                 load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, 0, pcol, pbv, pent);
@@ -371,11 +410,11 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                             ++nvalid;
                         }
                     }
+                    load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, pcol, pbv, pent);
                     const int V = block_sum<TPB>(nvalid, s_wsi);
-                    if (V == 0) {  // uniform across the block
-                        load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, pcol, pbv, pent);
-                        continue;
-                    }
+                    if (V == 0) continue;  // uniform across the block; a pending chunk stays pending
+                    uint64_t *const sk = s_key[cbuf];
+                    double *const sv = s_val[cbuf];
                     block_minmax<TPB>(cmn, cmx, s_wsi);
                     int nb = TPB;
                     while (nb < V && nb < CAP) nb <<= 1;
@@ -403,7 +442,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) {
                         key[j] = ((uint64_t)(uint32_t)icol[j] << 32) | (uint32_t)(tid + j * TPB);
-                        if (ok[j]) s_key[s_cnt[bk[j]] + pos[j]] = key[j];
+                        if (ok[j]) sk[s_cnt[bk[j]] + pos[j]] = key[j];
                     }
                     __syncthreads();
                     // Rank inside the bucket by (column, serial index): the
@@ -415,7 +454,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                         if (ok[j]) {
                             const int lo = s_cnt[bk[j]], hi = s_cnt[bk[j] + 1];
                             int rank = 0;
-                            for (int y = lo; y < hi; ++y) rank += s_key[y] < key[j];
+                            for (int y = lo; y < hi; ++y) rank += sk[y] < key[j];
                             x[j] = lo + rank;
                         }
                     }
@@ -423,25 +462,20 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) {
                         if (ok[j]) {
-                            s_key[x[j]] = key[j];
-                            s_val[x[j]] = ival[j];
+                            sk[x[j]] = key[j];
+                            sv[x[j]] = ival[j];
                         }
                     }
-                    __syncthreads();
-                    // Each column's first sorted position continues that
-                    // column's running sum (the accumulator, when an earlier
-                    // chunk touched it, else its first product) with its
-                    // products in serial order.  All heads' accumulator loads
-                    // are issued before any fold, then the next chunk's
-                    // products.
-                    bool head[ITEMS];
-                    uint32_t hcol[ITEMS], hword[ITEMS];
-                    uint64_t hacc[ITEMS];
+                    fold_pending();   // the previous chunk's sums have had this chunk's sort to arrive
+                    __syncthreads();  // placement done; folded sums and presence bits visible
+                    // This chunk's heads (each column's first sorted
+                    // position): presence and running-sum loads are issued
+                    // now and used when the next chunk has been sorted.
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) {
                         const int y = tid + j * TPB;
-                        head[j] = y < V && (y == 0 || (s_key[y - 1] >> 32) != (s_key[y] >> 32));
-                        hcol[j] = head[j] ? (uint32_t)(s_key[y] >> 32) : 0u;
+                        head[j] = y < V && (y == 0 || (sk[y - 1] >> 32) != (sk[y] >> 32));
+                        hcol[j] = head[j] ? (uint32_t)(sk[y] >> 32) : 0u;
                         const uint32_t rel = head[j] ? (uint32_t)((int64_t)(int32_t)hcol[j] - w0) : 0u;
                         if (LDSB) {
                             hword[j] = head[j] ? s_lbits[rel >> 5] : 0u;
@@ -452,27 +486,13 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                             hacc[j] = head[j] ? (uint64_t)__double_as_longlong(ld_acc(&acc[rel])) : 0ull;
                         }
                     }
-                    load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, pcol, pbv, pent);
-#pragma unroll
-                    for (int j = 0; j < ITEMS; ++j) {
-                        if (!head[j]) continue;
-                        const int y0 = tid + j * TPB;
-                        const uint32_t rel = (uint32_t)((int64_t)(int32_t)hcol[j] - w0);
-                        const uint32_t bit = 1u << (rel & 31u);
-                        const bool present = (hword[j] & bit) != 0u;
-                        const double old = __longlong_as_double((long long)hacc[j]);
-                        double v = present ? old + s_val[y0] : s_val[y0];
-                        for (int y = y0 + 1; y < V && (uint32_t)(s_key[y] >> 32) == hcol[j]; ++y) v = v + s_val[y];
-                        st_acc(&acc[rel], v);
-                        if (!present) {
-                            if (LDSB) atomicOr(&s_lbits[rel >> 5], bit);
-                            else atomicOr(&bits[rel >> 5], bit);
-                        }
-                        zc += (int)(v == 0.0) - (int)(present && old == 0.0);
-                    }
-                    __syncthreads();  // chunk folded; LDS and accumulator state consistent
+                    pend_v = V;
+                    pbuf = cbuf;
+                    cbuf ^= 1;
                 }
             }
+            fold_pending();
+            __syncthreads();  // every sum and presence bit of the window in place
             // Extraction of the window in ascending column order, zeros
             // dropped.  When no column of the window sums to exactly zero (or
             // zeros are kept) the entries before a word are the set bits
@@ -605,7 +625,8 @@ std::atomic<bool> g_num_wide{false};
 bool prepare_numeric_kernels(size_t lds_per_block) {
     const size_t dyn = (size_t)(kLdsBitsColsWide / 8);
     constexpr int C = kCapLargeWideNum;
-    const size_t stat = (size_t)C * (8 + 8 + 8 + 8 + 8 + 4) + 64 * 8 + 64;
+    // static arrays: batch metadata 3 x 8 B, two chunk buffers 2 x 16 B, bucket counts 4 B per product
+    const size_t stat = (size_t)C * (3 * 8 + 2 * 16 + 4) + 64 * 8 + 64;
     if (lds_per_block < dyn + stat) return false;
     const hipError_t e =
         hipFuncSetAttribute(reinterpret_cast<const void *>(&k_large_numeric<kTpbLargeWideNum, kCapLargeWideNum, true>),
