@@ -248,9 +248,10 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                 const int nbt = (int)min((int64_t)CAP, a1 - e0);
                 const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, nullptr, s_off, s_ws64, false);
                 // UNROLL products per thread per step: every column load,
-                // then every bit claim, in flight together.
-                for (int64_t s0 = tid; s0 < Pb; s0 += (int64_t)TPB * UNROLL) {
-                    int64_t c[UNROLL];
+                // then every bit claim, in flight together.  With the bitmap
+                // in LDS the next step's column loads are issued before this
+                // step's claims (LDS atomics do not wait on them).
+                auto load_step = [&](int64_t s0, int (&c)[UNROLL]) {
 #pragma unroll
                     for (int u = 0; u < UNROLL; ++u) {
                         const int64_t s = s0 + (int64_t)u * TPB;
@@ -259,6 +260,19 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                             const int e = upper_entry(s_off, nbt, s);
                             c[u] = op.bcol[s_bst[e] + (s - s_off[e])];
                         }
+                    }
+                };
+                constexpr int64_t kStep = (int64_t)TPB * UNROLL;
+                int cn[UNROLL];
+                if (LDSB) load_step(tid, cn);
+                for (int64_t s0 = tid; s0 < Pb; s0 += kStep) {
+                    int c[UNROLL];
+                    if (LDSB) {
+#pragma unroll
+                        for (int u = 0; u < UNROLL; ++u) c[u] = cn[u];
+                        load_step(s0 + kStep, cn);
+                    } else {
+                        load_step(s0, c);
                     }
                     uint32_t old[UNROLL], bit[UNROLL];
                     bool in[UNROLL];
