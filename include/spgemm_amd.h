@@ -113,9 +113,12 @@ spg_status spg_context_bind(spg_context *ctx);
  *                         device memory, else two-phase
  *   SPG_PATH_SINGLE_PASS  always single pass
  *   SPG_PATH_TWO_PHASE    count every row first, then compute
+ *   SPG_PATH_SINGLE_PASS_DEALT  single pass over 32-row tiles whose small rows
+ *                         are dealt to 16 waves: for products whose rows are
+ *                         mostly bigger than 256 products
  * Results are identical on every path.  No counterpart in the reference (its
  * spgemm() has one CPU pipeline); exposed so callers and tests can pin a path. */
-enum { SPG_PATH_AUTO = 0, SPG_PATH_SINGLE_PASS = 1, SPG_PATH_TWO_PHASE = 2 };
+enum { SPG_PATH_AUTO = 0, SPG_PATH_SINGLE_PASS = 1, SPG_PATH_TWO_PHASE = 2, SPG_PATH_SINGLE_PASS_DEALT = 3 };
 spg_status spg_set_path(spg_context *ctx, int path);
 /* Record HIP events around the phases of each multiply (spg_stats.ms_*). */
 spg_status spg_set_timing(spg_context *ctx, int enabled);
@@ -202,9 +205,10 @@ spg_status spg_batch_run(spg_batch *b, int64_t *nnz_total);
 /* After a first run, replay the planned pipeline from a captured HIP graph
  * (no host synchronisation inside the pipeline). */
 spg_status spg_batch_set_graph(spg_batch *b, int enabled);
-/* Time the pipeline variants (single pass with 2 or 1 rows per wave,
- * two-phase) over `repeats` runs each and keep the fastest; *choice receives
- * its index. */
+/* Time the pipeline variants over `repeats` runs each and keep the fastest;
+ * *choice receives its index: 0 single pass with 2 rows per wave, 1 with 1
+ * row per wave, 2 two-phase, 3 single pass over 32-row tiles whose
+ * single-pass rows are dealt to 16 waves. */
 spg_status spg_batch_autotune(spg_batch *b, int repeats, int *choice);
 /* Results of the last run: entry count of product i, and a host copy of C_i
  * (row_ptr: rows_i + 1 entries starting at 0). */

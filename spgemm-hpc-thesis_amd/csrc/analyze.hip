@@ -133,9 +133,8 @@ __device__ __forceinline__ void tile_finish(const Operands &op, const Workspace 
     const int64_t r = tile * 64 + lane;
     const bool valid = r < op.arows;
     const int64_t ub = (int64_t)s_ub[lane];
-    int b = -1;
+    const int b = valid ? classify(ub, a1 - a0) : -1;
     if (valid) {
-        b = classify(ub, a1 - a0);
         ws.ub[r] = ub;
         if (b == kBinEmpty) {
             ws.snnz[r] = 0;

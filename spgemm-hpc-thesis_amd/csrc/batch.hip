@@ -15,7 +15,8 @@
 // for operands of the same structure; the pipeline can then be captured in a
 // HIP graph and replayed without host synchronisation inside.  An autotuner
 // times the pipeline variants (single pass with one or two rows per wave,
-// two-phase) on the batch and keeps the fastest.
+// two-phase, single pass over dealt 32-row tiles) on the batch and keeps the
+// fastest.
 //
 // A chained batch takes its A operands from the results of another batch (the
 // second product of a Galerkin triple (R.A).P): its arena rows are refreshed on
@@ -523,13 +524,17 @@ spg_status spg_batch_autotune(spg_batch *b, int repeats, int *choice) {
     struct Variant {
         int path, rows;
     };
-    const Variant variants[] = {{SPG_PATH_SINGLE_PASS, 2}, {SPG_PATH_SINGLE_PASS, 1}, {SPG_PATH_TWO_PHASE, 2}};
+    const Variant variants[] = {{SPG_PATH_SINGLE_PASS, 2},
+                                {SPG_PATH_SINGLE_PASS, 1},
+                                {SPG_PATH_TWO_PHASE, 2},
+                                {SPG_PATH_SINGLE_PASS, kFusedDealRows}};
+    constexpr int kVariants = (int)(sizeof(variants) / sizeof(variants[0]));
     const bool was_graph = b->use_graph;
     b->use_graph = false;
     b->drop_graph();
     float best = 0;
     int best_i = 0;
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < kVariants; ++i) {
         ctx->path = variants[i].path;
         ctx->fused_rows = variants[i].rows;
         int64_t nnz = 0;

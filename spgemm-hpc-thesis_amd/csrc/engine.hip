@@ -387,8 +387,9 @@ spg_status spg_context_bind(spg_context *ctx) {
 
 spg_status spg_set_path(spg_context *ctx, int path) {
     if (!ctx) return SPG_EINVAL;
-    if (path < SPG_PATH_AUTO || path > SPG_PATH_TWO_PHASE) return set_err(ctx, SPG_EINVAL, "unknown path");
-    ctx->path = path;
+    if (path < SPG_PATH_AUTO || path > SPG_PATH_SINGLE_PASS_DEALT) return set_err(ctx, SPG_EINVAL, "unknown path");
+    ctx->path = path == SPG_PATH_SINGLE_PASS_DEALT ? SPG_PATH_SINGLE_PASS : path;
+    ctx->fused_rows = path == SPG_PATH_SINGLE_PASS_DEALT ? kFusedDealRows : kFusedRows;
     return SPG_OK;
 }
 

@@ -143,6 +143,40 @@ __device__ __forceinline__ void stage_meta(const Operands &op, const Workspace &
     wave_sync();
 }
 
+// The same over two entry ranges staged one after the other: [aA, aA+nA)
+// at positions [0, nA), then [aB, ...) up to E positions in all.
+template <bool SMALLB>
+__device__ __forceinline__ MetaReg<SMALLB> load_meta2(const Operands &op, const Workspace &ws, int64_t aA, int nA,
+                                                      int64_t aB, int E, int c0) {
+    using bst_t = typename FusedWave<SMALLB>::bst_t;
+    const int p = min(c0 + lane_id(), max(E - 1, 0));
+    const int64_t e = p < nA ? aA + p : aB + (p - nA);
+    MetaReg<SMALLB> m;
+    m.bst = (bst_t)__builtin_nontemporal_load(&ws.ebst[e]);
+    m.len = __builtin_nontemporal_load(&ws.elen[e]);
+    m.av = __builtin_nontemporal_load(&op.aval[e]);
+    return m;
+}
+template <bool SMALLB>
+__device__ __forceinline__ void stage_meta2(const Operands &op, const Workspace &ws, FusedWave<SMALLB> &w, int64_t aA,
+                                            int nA, int64_t aB, int E, const MetaReg<SMALLB> &m0) {
+    const int lane = lane_id();
+    int carry = 0;
+    for (int c0 = 0; c0 < E; c0 += 64) {
+        const MetaReg<SMALLB> m = c0 == 0 ? m0 : load_meta2<SMALLB>(op, ws, aA, nA, aB, E, c0);
+        const int e = c0 + lane;
+        const int len = e < E ? m.len : 0;
+        const int incl = wave_incl_sum_dpp(len);
+        w.bst[e] = m.bst;
+        w.av[e] = m.av;
+        w.off[e] = carry + incl - len;
+        carry += wave_last(incl);
+    }
+    wave_sync();
+    if (lane == 0) w.off[E] = carry;
+    wave_sync();
+}
+
 // Expansion of the row whose entries are staged at [m, m+na) with P products:
 // slot j of each lane gets one product's column, serial index and values; the
 // B gathers are issued here and consumed later.
@@ -613,10 +647,178 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op,
     if (lane == 0 && flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
 }
 
+// Dealt tiles: 32 consecutive rows per 1024-thread workgroup, whose
+// single-pass rows are dealt to the 16 waves in row order, two at most per
+// wave.  Every other row's count (reserved, or empty) is folded into the
+// tile's total, so the look-back and the row offsets are those of
+// consecutive-row tiles.
+template <bool SMALLB>
+__global__ __launch_bounds__(kFusedDealTpb) void k_fused_dealt(Operands op, Workspace ws, FusedOut fo) {
+    constexpr int NW = kFusedDealTpb / 64;
+    constexpr int R = 2;                // single-pass rows per wave at most
+    constexpr int TR = kFusedDealTile;  // rows per tile
+    static_assert(TR <= 64 && NW * R >= TR, "every row of a tile fits its waves");
+    __shared__ FusedWave<SMALLB> s_w[NW];
+    __shared__ int64_t s_rcnt[TR];  // entries of each tile row
+    __shared__ int64_t s_rp[TR];    // output offset of each tile row
+    __shared__ int64_t s_dist[NW];
+    const int wv = threadIdx.x >> 6;
+    const int lane = lane_id();
+    FusedWave<SMALLB> &w = s_w[wv];
+    unsigned flags = 0;
+    int64_t distinct_sum = 0;
+
+    const int64_t t = (int64_t)blockIdx.x;  // tiles in workgroup order, as k_fused
+    const int64_t tr0 = t * (int64_t)TR;
+    // The tile's rows on lanes 0..TR-1: row offsets, work bounds, counts.
+    const bool lrow_ok = lane < TR && tr0 + lane < op.arows;
+    const int64_t rq = min(tr0 + lane, (int64_t)op.arows - 1);
+    const int64_t ap = lane < TR ? op.aptr[min(tr0 + lane, (int64_t)op.arows)] : 0;
+    const int64_t ap1 = lane < TR ? op.aptr[min(tr0 + lane + 1, (int64_t)op.arows)] : 0;
+    const int64_t ubl = lrow_ok ? ws.ub[rq] : 0;
+    const int64_t snz = lrow_ok ? ws.snnz[rq] : 0;
+    // Single-pass rows by the bins' rule (bins W1, W2).
+    const uint64_t fmask = __ballot(lrow_ok && ubl > 0 && ubl <= kFusedCap && ap1 - ap <= kFusedCap);
+    const int nf = __popcll(fmask);  // <= TR <= NW * R
+    // Ranks to waves in row order: with at most NW rows one each, else the
+    // first nf - NW waves take two adjacent ranks.
+    int first, nmine;
+    if (nf <= NW) {
+        first = wv;
+        nmine = wv < nf ? 1 : 0;
+    } else {
+        const int extra = nf - NW;
+        first = wv + min(wv, extra);
+        nmine = wv < extra ? 2 : 1;
+    }
+    int lrow[R];
+    bool fused[R];
+    int64_t a0[R];
+    int na[R], ubk[R], cnt[R];
+    RowOut out[R];
+    {
+        uint64_t m = fmask;
+        for (int i = 0; i < first; ++i) m &= m - 1ull;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            fused[k] = k < nmine;
+            lrow[k] = fused[k] ? uni(__ffsll((long long)m) - 1) : 0;
+            if (fused[k]) m &= m - 1ull;
+            a0[k] = lane64(ap, lrow[k]);
+            na[k] = (int)(lane64(ap1, lrow[k]) - a0[k]);
+            ubk[k] = (int)lane64(ubl, lrow[k]);
+            cnt[k] = 0;
+            out[k].exnz = 0;
+            out[k].tot = 0;
+        }
+    }
+    // The wave's rows (adjacent or not) with at most kFusedCap entries
+    // together are staged one after the other and gathered at once: one
+    // memory round trip for both.
+    const int nB = nmine == 2 ? na[1] : 0;
+    const int E = na[0] + nB;
+    const bool joint = fused[0] && E <= kFusedCap;
+    MetaReg<SMALLB> meta0;
+    if (fused[0]) {
+        if (joint) meta0 = load_meta2<SMALLB>(op, ws, a0[0], na[0], a0[1], E, 0);
+        else meta0 = load_meta<SMALLB>(op, ws, a0[0], na[0], 0);
+    }
+    if (joint) {
+        stage_meta2<SMALLB>(op, ws, w, a0[0], na[0], a0[1], E, meta0);
+        RowItems it[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (fused[k]) expand_row<SMALLB>(op, w, k == 0 ? 0 : na[0], na[k], ubk[k], it[k], flags);
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (fused[k]) form_products(it[k]);
+        wave_sync();  // metadata reads are complete before the sort reuses its storage
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (fused[k]) {
+                int dist = 0;
+                cnt[k] = sort_fold<SMALLB>(w, ubk[k], it[k], out[k], dist, op.keep_zeros);
+                distinct_sum += dist;
+            }
+        }
+    } else if (fused[0]) {
+        MetaReg<SMALLB> meta[R];
+        meta[0] = meta0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (k + 1 < R && fused[k + 1]) meta[k + 1] = load_meta<SMALLB>(op, ws, a0[k + 1], na[k + 1], 0);
+            if (fused[k]) {
+                stage_meta<SMALLB>(op, ws, w, a0[k], na[k], meta[k]);
+                RowItems it;
+                expand_row<SMALLB>(op, w, 0, na[k], ubk[k], it, flags);
+                form_products(it);
+                wave_sync();
+                int dist = 0;
+                cnt[k] = sort_fold<SMALLB>(w, ubk[k], it, out[k], dist, op.keep_zeros);
+                distinct_sum += dist;
+            }
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (fused[k]) s_rcnt[lrow[k]] = cnt[k];
+        s_dist[wv] = distinct_sum;
+    }
+    if (wv == 0 && lane < TR && !((fmask >> lane) & 1ull)) s_rcnt[lane] = ubl > 0 ? snz : 0;  // reserved or empty
+    __syncthreads();
+    if (wv == 0) {
+        const int64_t c = lane < TR ? s_rcnt[lane] : 0;
+        const int64_t inc = wave_incl_scan(c);
+        const int64_t agg = lane64(inc, TR - 1);
+        int64_t dist = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) dist += s_dist[i];
+        if (lane == 0) fo.distinct[t] = dist;
+        const int64_t excl = tile_lookback(fo.status, t, agg, flags);
+        const int64_t rp = excl + inc - c;
+        if (lane < TR) {
+            s_rp[lane] = rp;
+            if (lrow_ok) {
+                fo.rowptr[tr0 + lane] = rp;
+                if ((fmask >> lane) & 1ull) ws.annz[tr0 + lane] = c;
+            }
+        }
+        if (lane == 0 && t == fo.ntiles - 1) {
+            fo.rowptr[op.arows] = excl + agg;
+            ws.ctrl->reserved = (unsigned long long)(excl + agg);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (fused[k]) {
+            const int64_t rp = uni64(s_rp[lrow[k]]);
+            int32_t *cc = fo.ccol + rp;
+            double *cv = fo.cval + rp;
+            uint32_t before = 0;
+#pragma unroll
+            for (int j = 0; j < kFItems; ++j) {
+                const uint32_t b = (out[k].exnz >> (8 * j)) & 0xffu;
+                if (b & 0x80u) {
+                    const uint32_t o = before + (b & 0x7fu);
+                    __builtin_nontemporal_store(
+                        out[k].col[j], reinterpret_cast<int32_t *>(reinterpret_cast<char *>(cc) + o * 4u));
+                    __builtin_nontemporal_store(
+                        out[k].val[j], reinterpret_cast<double *>(reinterpret_cast<char *>(cv) + o * 8u));
+                }
+                before += (out[k].tot >> (8 * j)) & 0xffu;
+            }
+        }
+    }
+    if (lane == 0 && flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
+}
+
 }  // namespace
 
 int64_t fused_tiles(int32_t rows, int rows_per_wave) {
-    const int64_t tile = (int64_t)(kFusedTpb / 64) * rows_per_wave;
+    const int64_t tile =
+        rows_per_wave == kFusedDealRows ? (int64_t)kFusedDealTile : (int64_t)(kFusedTpb / 64) * rows_per_wave;
     return ((int64_t)rows + tile - 1) / tile;
 }
 
@@ -624,7 +826,11 @@ void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, b
                   hipStream_t stream) {
     if (op.arows <= 0) return;
     const dim3 grid((unsigned)fo.ntiles), block(kFusedTpb);
-    if (rows_per_wave == 1) {
+    if (rows_per_wave == kFusedDealRows) {
+        const dim3 dblock(kFusedDealTpb);
+        if (small_b) hipLaunchKernelGGL((k_fused_dealt<true>), grid, dblock, 0, stream, op, ws, fo);
+        else hipLaunchKernelGGL((k_fused_dealt<false>), grid, dblock, 0, stream, op, ws, fo);
+    } else if (rows_per_wave == 1) {
         if (small_b) hipLaunchKernelGGL((k_fused<true, 1>), grid, block, 0, stream, op, ws, fo);
         else hipLaunchKernelGGL((k_fused<false, 1>), grid, block, 0, stream, op, ws, fo);
     } else {

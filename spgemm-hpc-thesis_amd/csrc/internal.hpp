@@ -60,6 +60,14 @@ constexpr int kFusedCap = kCapW2;
 #endif
 constexpr int kFusedTpb = SPG_FUSED_TPB;
 constexpr int kFusedRows = 2;  // default rows per wave of a tile (1 or 2)
+// Dealt tiles (rows_per_wave == kFusedDealRows, a mode id): a tile is 32
+// consecutive rows on a 1024-thread workgroup, and its single-pass rows are
+// dealt to the 16 waves in row order, two at most per wave.  Where most rows
+// are counted ahead (bins W3 and up), a tile's waves then hold about one row
+// each, and a wave's two rows are staged together even when not adjacent.
+constexpr int kFusedDealRows = 4;
+constexpr int kFusedDealTpb = 1024;
+constexpr int kFusedDealTile = 32;
 // Tile t's look-back status is word t * kStatusStride of the status array:
 // one status per 64-byte line.  Packed statuses put the few lines that every
 // waiting tile polls on one memory channel; spreading them took the 2^20-row
@@ -178,7 +186,8 @@ bool prepare_numeric_kernels(size_t lds_per_block);  // numeric.hip; false: wide
 // Single-pass kernel over all rows (fused.hip): computes the rows of bins W1,
 // W2 and reserves the symbolic counts of the others.  small_b: every B entry
 // index i has 8*i < 2^32.
-// rows_per_wave: 1 or 2 (rows of a tile per wave; the tile has kFusedTpb/64 waves).
+// rows_per_wave: 1 or 2 (rows of a tile per wave; the tile has kFusedTpb/64
+// waves), or kFusedDealRows (dealt 32-row tiles).
 int64_t fused_tiles(int32_t rows, int rows_per_wave);
 void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b, int rows_per_wave,
                   hipStream_t stream);

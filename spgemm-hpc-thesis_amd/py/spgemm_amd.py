@@ -46,7 +46,7 @@ EXPORTED = [
     "spg_sfull_next_row", "spg_sfull_row_cells", "spg_sfull_finish", "spg_sfull_close", "spg_sha256_hex",
 ]
 
-PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE = 0, 1, 2
+PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE, PATH_SINGLE_PASS_DEALT = 0, 1, 2, 3
 
 
 @dataclass
@@ -328,7 +328,7 @@ class Engine:
         self._check(self._L.spg_set_timing(self._h, 1 if on else 0))
 
     def set_path(self, path: int):
-        """PATH_AUTO, PATH_SINGLE_PASS or PATH_TWO_PHASE (results are identical)."""
+        """PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE or PATH_SINGLE_PASS_DEALT (results are identical)."""
         self._check(self._L.spg_set_path(self._h, int(path)))
         self._path = int(path)
 

@@ -39,7 +39,8 @@ def all_cases():
     return out
 
 
-PATHS = [("auto", S.PATH_AUTO), ("single_pass", S.PATH_SINGLE_PASS), ("two_phase", S.PATH_TWO_PHASE)]
+PATHS = [("auto", S.PATH_AUTO), ("single_pass", S.PATH_SINGLE_PASS), ("two_phase", S.PATH_TWO_PHASE),
+         ("single_pass_dealt", S.PATH_SINGLE_PASS_DEALT)]
 
 
 @pytest.fixture(params=PATHS, ids=[p[0] for p in PATHS])
