@@ -116,9 +116,18 @@ spg_status spg_context_bind(spg_context *ctx);
  *   SPG_PATH_SINGLE_PASS_DEALT  single pass over 32-row tiles whose small rows
  *                         are dealt to 16 waves: for products whose rows are
  *                         mostly bigger than 256 products
+ *   SPG_PATH_SINGLE_PASS_PACKED  single pass over tiles of 16 small rows each
+ *                         (per 256-row block, in row order), the other rows'
+ *                         counts folded into the tiles: the same use
  * Results are identical on every path.  No counterpart in the reference (its
  * spgemm() has one CPU pipeline); exposed so callers and tests can pin a path. */
-enum { SPG_PATH_AUTO = 0, SPG_PATH_SINGLE_PASS = 1, SPG_PATH_TWO_PHASE = 2, SPG_PATH_SINGLE_PASS_DEALT = 3 };
+enum {
+    SPG_PATH_AUTO = 0,
+    SPG_PATH_SINGLE_PASS = 1,
+    SPG_PATH_TWO_PHASE = 2,
+    SPG_PATH_SINGLE_PASS_DEALT = 3,
+    SPG_PATH_SINGLE_PASS_PACKED = 4
+};
 spg_status spg_set_path(spg_context *ctx, int path);
 /* Record HIP events around the phases of each multiply (spg_stats.ms_*). */
 spg_status spg_set_timing(spg_context *ctx, int enabled);
@@ -208,7 +217,8 @@ spg_status spg_batch_set_graph(spg_batch *b, int enabled);
 /* Time the pipeline variants over `repeats` runs each and keep the fastest;
  * *choice receives its index: 0 single pass with 2 rows per wave, 1 with 1
  * row per wave, 2 two-phase, 3 single pass over 32-row tiles whose
- * single-pass rows are dealt to 16 waves. */
+ * single-pass rows are dealt to 16 waves, 4 single pass over packed tiles of
+ * 16 single-pass rows. */
 spg_status spg_batch_autotune(spg_batch *b, int repeats, int *choice);
 /* Results of the last run: entry count of product i, and a host copy of C_i
  * (row_ptr: rows_i + 1 entries starting at 0). */

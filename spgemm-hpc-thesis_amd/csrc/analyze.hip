@@ -28,6 +28,7 @@ __global__ void k_zero_words(uint32_t *p, size_t n) {
 }
 
 constexpr int kAnalyzeTpb = 256;
+static_assert(kAnalyzeTpb / 64 * 64 == kAnalyzeRows, "packed single-pass tiles take an analyze block's rows");
 constexpr int kAnChunks = 2;  // 64-entry chunks in flight per wave (1 and 4 measured no faster)
 
 __global__ void k_dir32(const int64_t *bptr, int64_t n, uint32_t *out) {

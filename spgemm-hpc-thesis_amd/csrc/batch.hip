@@ -15,8 +15,8 @@
 // for operands of the same structure; the pipeline can then be captured in a
 // HIP graph and replayed without host synchronisation inside.  An autotuner
 // times the pipeline variants (single pass with one or two rows per wave,
-// two-phase, single pass over dealt 32-row tiles) on the batch and keeps the
-// fastest.
+// two-phase, single pass over dealt 32-row tiles, single pass over packed
+// tiles of small rows) on the batch and keeps the fastest.
 //
 // A chained batch takes its A operands from the results of another batch (the
 // second product of a Galerkin triple (R.A).P): its arena rows are refreshed on
@@ -527,7 +527,8 @@ spg_status spg_batch_autotune(spg_batch *b, int repeats, int *choice) {
     const Variant variants[] = {{SPG_PATH_SINGLE_PASS, 2},
                                 {SPG_PATH_SINGLE_PASS, 1},
                                 {SPG_PATH_TWO_PHASE, 2},
-                                {SPG_PATH_SINGLE_PASS, kFusedDealRows}};
+                                {SPG_PATH_SINGLE_PASS, kFusedDealRows},
+                                {SPG_PATH_SINGLE_PASS, kFusedPackRows}};
     constexpr int kVariants = (int)(sizeof(variants) / sizeof(variants[0]));
     const bool was_graph = b->use_graph;
     b->use_graph = false;

@@ -146,7 +146,23 @@ spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h) {
     }
     p.fo = FusedOut{};
     if (p.fused) {
-        p.fo.ntiles = fused_tiles(p.rows, p.fused_rows);
+        if (p.fused_rows == kFusedPackRows) {
+            // Packed tiles: at most one per block plus one per kFusedPackCap
+            // single-pass rows; the device counts them exactly.
+            p.pack_blocks = analyze_blocks(p.rows);
+            const int64_t nf = (int64_t)h.bin_rows[kBinW1] + (int64_t)h.bin_rows[kBinW2];
+            p.fo.ntiles = p.rows > 0 ? p.pack_blocks + (nf + kFusedPackCap - 1) / kFusedPackCap : 0;
+            const size_t nb = (size_t)(p.pack_blocks > 0 ? p.pack_blocks : 1);
+            SPG_HIP(ctx->pack_cnt.ensure(nb * 8, st), "hipMalloc(packed tile counts)");
+            SPG_HIP(ctx->pack_off.ensure(nb * 8, st), "hipMalloc(packed tile offsets)");
+            SPG_HIP(ctx->pack_map.ensure((size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1) * 8, st),
+                    "hipMalloc(packed tile map)");
+            p.pack_cnt = ctx->pack_cnt.as<int64_t>();
+            p.pack_off = ctx->pack_off.as<int64_t>();
+            p.fo.tmap = ctx->pack_map.as<int64_t>();
+        } else {
+            p.fo.ntiles = fused_tiles(p.rows, p.fused_rows);
+        }
         const size_t nt = (size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1);
         SPG_HIP(ctx->tile_status.ensure(nt * kStatusStride * 8, st), "hipMalloc(tile status)");
         SPG_HIP(ctx->tile_distinct.ensure(nt * 8, st), "hipMalloc(tile counts)");
@@ -162,6 +178,9 @@ spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h) {
 void enqueue_values(spg_context *ctx, const ProductPlan &p) {
     hipStream_t st = ctx->stream;
     if (p.fused) {
+        if (p.fused_rows == kFusedPackRows)
+            launch_pack_tiles(p.ws.part, p.pack_blocks, p.pack_cnt, p.pack_off, p.fo.tmap, p.ws.scan_tmp, p.ws.ctrl,
+                              st);
         launch_zero_bytes(p.fo.status, (size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1) * kStatusStride * 8, st);
         if (p.rows == 0) launch_zero_bytes(p.fo.rowptr, 8, st);
         launch_fused(p.op, p.ws, p.fo, p.small_b, p.fused_rows, st);
@@ -356,7 +375,7 @@ spg_status spg_destroy(spg_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->a_ptr, &ctx->a_col, &ctx->a_val, &ctx->b_ptr, &ctx->b_col, &ctx->b_val,
                       &ctx->dir_start, &ctx->dir_end, &ctx->ebst, &ctx->elen, &ctx->ub, &ctx->bin_rows, &ctx->snnz,
-                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->tile_status, &ctx->tile_distinct, &ctx->part, &ctx->dir32, &ctx->halo_err, &ctx->halo_cnt, &ctx->halo_tmp, &ctx->rowptr[0],
+                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->tile_status, &ctx->tile_distinct, &ctx->part, &ctx->dir32, &ctx->halo_err, &ctx->halo_cnt, &ctx->halo_tmp, &ctx->pack_cnt, &ctx->pack_off, &ctx->pack_map, &ctx->rowptr[0],
                       &ctx->rowptr[1], &ctx->ccol[0], &ctx->ccol[1], &ctx->cval[0], &ctx->cval[1]};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t &e : ctx->ev)
@@ -387,9 +406,10 @@ spg_status spg_context_bind(spg_context *ctx) {
 
 spg_status spg_set_path(spg_context *ctx, int path) {
     if (!ctx) return SPG_EINVAL;
-    if (path < SPG_PATH_AUTO || path > SPG_PATH_SINGLE_PASS_DEALT) return set_err(ctx, SPG_EINVAL, "unknown path");
-    ctx->path = path == SPG_PATH_SINGLE_PASS_DEALT ? SPG_PATH_SINGLE_PASS : path;
-    ctx->fused_rows = path == SPG_PATH_SINGLE_PASS_DEALT ? kFusedDealRows : kFusedRows;
+    if (path < SPG_PATH_AUTO || path > SPG_PATH_SINGLE_PASS_PACKED) return set_err(ctx, SPG_EINVAL, "unknown path");
+    const bool dealt = path == SPG_PATH_SINGLE_PASS_DEALT, packed = path == SPG_PATH_SINGLE_PASS_PACKED;
+    ctx->path = dealt || packed ? SPG_PATH_SINGLE_PASS : path;
+    ctx->fused_rows = dealt ? kFusedDealRows : packed ? kFusedPackRows : kFusedRows;
     return SPG_OK;
 }
 

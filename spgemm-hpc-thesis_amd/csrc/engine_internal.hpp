@@ -58,6 +58,9 @@ struct ProductPlan {
     int bin_count[8] = {};
     int64_t capacity = 0;  // C entries allocated
     FusedOut fo{};
+    int64_t pack_blocks = 0;             // packed tiles: analyze blocks
+    int64_t *pack_cnt = nullptr;         // packed tiles: tiles per block, then their offsets
+    int64_t *pack_off = nullptr;
 };
 
 }  // namespace spg
@@ -77,6 +80,7 @@ struct spg_context {
     // workspace
     spg::DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc, tile_status, tile_distinct, part, dir32;
     spg::DevBuf halo_err, halo_cnt, halo_tmp;  // device halo directory and packing
+    spg::DevBuf pack_cnt, pack_off, pack_map;  // packed single-pass tiles
     int path = SPG_PATH_AUTO;
     int fused_rows = spg::kFusedRows;
     int64_t lbits_width = 0;

@@ -814,7 +814,238 @@ __global__ __launch_bounds__(kFusedDealTpb) void k_fused_dealt(Operands op, Work
     if (lane == 0 && flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
 }
 
+// Packed tiles per analyze block: max(1, ceil(single-pass rows / cap)).
+__global__ void k_pack_counts(const int64_t *part, int64_t nblocks, int64_t *tcnt) {
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblocks;
+         b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = part[b * kAnalyzeFields + 2 + kBinW1] + part[b * kAnalyzeFields + 2 + kBinW2];
+        tcnt[b] = max((int64_t)1, (f + kFusedPackCap - 1) / kFusedPackCap);
+    }
+}
+
+// Tile map: tile toff[b] + j is tile j of block b.
+__global__ void k_pack_map(const int64_t *tcnt, const int64_t *toff, int64_t nblocks, int64_t *tmap) {
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblocks;
+         b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = toff[b];
+        for (int64_t j = 0; j < tcnt[b]; ++j) tmap[o + j] = (b << 5) | j;
+    }
+}
+
+// Packed tiles: tile j of an analyze block computes the block's single-pass
+// rows of rank 16j .. 16j+15 (in row order), dealt two per wave, and owns
+// the block's rows from the one after the previous tile's last single-pass
+// row up to its own last one (the block's last tile: up to the block's end).
+// The counts of the other rows in that range (reserved: counted ahead; empty:
+// zero) join the tile's total, and the tile writes the row offsets of every
+// row it owns.  Tiles run in map order, which is row order, so the look-back
+// is that of consecutive-row tiles.
+template <bool SMALLB>
+__global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused_packed(Operands op, Workspace ws, FusedOut fo) {
+    constexpr int NW = kFusedTpb / 64;
+    constexpr int R = 2;
+    constexpr int BR = kAnalyzeRows;  // rows per block
+    static_assert(NW * R == kFusedPackCap && BR == 256 && NW >= BR / 64, "tile shape");
+    __shared__ FusedWave<SMALLB> s_w[NW];
+    __shared__ int64_t s_rc[BR];  // count of each owned row of the block (0 for others)
+    __shared__ int64_t s_rp[BR];  // output offset of each owned row
+    __shared__ uint64_t s_fm[BR / 64];
+    __shared__ int64_t s_dist[NW];
+    const int wv = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int tid = threadIdx.x;
+    FusedWave<SMALLB> &w = s_w[wv];
+    unsigned flags = 0;
+    int64_t distinct_sum = 0;
+
+    const int64_t t = (int64_t)blockIdx.x;  // tiles in workgroup order
+    const int64_t ntiles = (int64_t)ws.ctrl->packed_tiles;
+    if (t >= ntiles) {  // launched for an upper bound of the tile count
+        if (tid == 0) fo.distinct[t] = 0;
+        return;
+    }
+    const int64_t code = fo.tmap[t];
+    const int64_t br0 = (code >> 5) * BR;
+    const int jt = (int)(code & 31);
+    // The block's rows: thread tid < BR holds row br0 + tid.
+    const int64_t r = br0 + tid;
+    const bool valid = tid < BR && r < op.arows;
+    const int64_t ubr = valid ? ws.ub[r] : 0;
+    const int64_t nar = valid ? op.aptr[r + 1] - op.aptr[r] : 0;
+    const bool sp = valid && ubr > 0 && ubr <= kFusedCap && nar <= kFusedCap;
+    const int64_t snr = valid && ubr > 0 && !sp ? ws.snnz[r] : 0;
+    {
+        const uint64_t m = __ballot(sp);
+        if (wv < BR / 64 && lane == 0) s_fm[wv] = m;
+    }
+    __syncthreads();
+    uint64_t fm[BR / 64];
+    int nfb = 0;
+#pragma unroll
+    for (int i = 0; i < BR / 64; ++i) {
+        fm[i] = s_fm[i];
+        nfb += __popcll(fm[i]);
+    }
+    // Row (offset in the block) of single-pass rank q, -1 if none.
+    auto rank_row = [&](int q) -> int {
+        if (q < 0 || q >= nfb) return -1;
+#pragma unroll
+        for (int i = 0; i < BR / 64; ++i) {
+            const int c = __popcll(fm[i]);
+            if (q < c) {
+                uint64_t m = fm[i];
+                for (int k = 0; k < q; ++k) m &= m - 1ull;
+                return i * 64 + __ffsll((long long)m) - 1;
+            }
+            q -= c;
+        }
+        return -1;
+    };
+    const int q0 = jt * kFusedPackCap;
+    const bool last_tile = q0 + kFusedPackCap >= nfb;
+    const int lo = jt == 0 ? 0 : rank_row(q0 - 1) + 1;
+    const int hi = last_tile ? BR - 1 : rank_row(q0 + kFusedPackCap - 1);
+    // This wave's ranks: q0 + 2wv, q0 + 2wv + 1.
+    int brow[R];
+    bool fused[R];
+    int64_t a0[R];
+    int na[R], ubk[R], cnt[R];
+    RowOut out[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        brow[k] = rank_row(q0 + R * wv + k);
+        fused[k] = brow[k] >= 0 && q0 + R * wv + k < q0 + kFusedPackCap;
+        const int64_t rr = br0 + (fused[k] ? brow[k] : 0);
+        a0[k] = fused[k] ? op.aptr[rr] : 0;
+        na[k] = fused[k] ? (int)(op.aptr[rr + 1] - a0[k]) : 0;
+        ubk[k] = fused[k] ? (int)ws.ub[rr] : 0;
+        cnt[k] = 0;
+        out[k].exnz = 0;
+        out[k].tot = 0;
+    }
+    const int nB = fused[1] ? na[1] : 0;
+    const int E = na[0] + nB;
+    const bool joint = fused[0] && E <= kFusedCap;
+    MetaReg<SMALLB> meta0;
+    if (fused[0]) {
+        if (joint) meta0 = load_meta2<SMALLB>(op, ws, a0[0], na[0], a0[1], E, 0);
+        else meta0 = load_meta<SMALLB>(op, ws, a0[0], na[0], 0);
+    }
+    if (joint) {
+        stage_meta2<SMALLB>(op, ws, w, a0[0], na[0], a0[1], E, meta0);
+        RowItems it[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (fused[k]) expand_row<SMALLB>(op, w, k == 0 ? 0 : na[0], na[k], ubk[k], it[k], flags);
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (fused[k]) form_products(it[k]);
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (fused[k]) {
+                int dist = 0;
+                cnt[k] = sort_fold<SMALLB>(w, ubk[k], it[k], out[k], dist, op.keep_zeros);
+                distinct_sum += dist;
+            }
+        }
+    } else if (fused[0]) {
+        MetaReg<SMALLB> meta[R];
+        meta[0] = meta0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (k + 1 < R && fused[k + 1]) meta[k + 1] = load_meta<SMALLB>(op, ws, a0[k + 1], na[k + 1], 0);
+            if (fused[k]) {
+                stage_meta<SMALLB>(op, ws, w, a0[k], na[k], meta[k]);
+                RowItems it;
+                expand_row<SMALLB>(op, w, 0, na[k], ubk[k], it, flags);
+                form_products(it);
+                wave_sync();
+                int dist = 0;
+                cnt[k] = sort_fold<SMALLB>(w, ubk[k], it, out[k], dist, op.keep_zeros);
+                distinct_sum += dist;
+            }
+        }
+    }
+    // Counts of the owned rows: the single-pass rows of this tile from their
+    // waves, the others from the count kernels (or zero).
+    if (tid < BR) s_rc[tid] = (tid >= lo && tid <= hi && !sp) ? snr : 0;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (fused[k]) s_rc[brow[k]] = cnt[k];
+        s_dist[wv] = distinct_sum;
+    }
+    __syncthreads();
+    if (wv == 0) {
+        // Scan of the block's BR counts, 4 consecutive rows per lane.
+        int64_t c[BR / 64];
+        int64_t loc = 0;
+#pragma unroll
+        for (int k = 0; k < BR / 64; ++k) {
+            c[k] = s_rc[lane * (BR / 64) + k];
+            loc += c[k];
+        }
+        const int64_t inc = wave_incl_scan(loc);
+        const int64_t agg = lane64(inc, 63);
+        int64_t dist = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) dist += s_dist[i];
+        if (lane == 0) fo.distinct[t] = dist;
+        const int64_t excl = tile_lookback(fo.status, t, agg, flags);
+        int64_t run = excl + inc - loc;
+#pragma unroll
+        for (int k = 0; k < BR / 64; ++k) {
+            const int o = lane * (BR / 64) + k;
+            s_rp[o] = run;
+            const int64_t rr = br0 + o;
+            if (o >= lo && o <= hi && rr < op.arows) {
+                fo.rowptr[rr] = run;
+                if ((fm[o >> 6] >> (o & 63)) & 1ull) ws.annz[rr] = c[k];
+            }
+            run += c[k];
+        }
+        if (lane == 0 && t == ntiles - 1) {
+            fo.rowptr[op.arows] = excl + agg;
+            ws.ctrl->reserved = (unsigned long long)(excl + agg);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (fused[k]) {
+            const int64_t rp = uni64(s_rp[brow[k]]);
+            int32_t *cc = fo.ccol + rp;
+            double *cv = fo.cval + rp;
+            uint32_t before = 0;
+#pragma unroll
+            for (int j = 0; j < kFItems; ++j) {
+                const uint32_t b = (out[k].exnz >> (8 * j)) & 0xffu;
+                if (b & 0x80u) {
+                    const uint32_t o = before + (b & 0x7fu);
+                    __builtin_nontemporal_store(
+                        out[k].col[j], reinterpret_cast<int32_t *>(reinterpret_cast<char *>(cc) + o * 4u));
+                    __builtin_nontemporal_store(
+                        out[k].val[j], reinterpret_cast<double *>(reinterpret_cast<char *>(cv) + o * 8u));
+                }
+                before += (out[k].tot >> (8 * j)) & 0xffu;
+            }
+        }
+    }
+    if (lane == 0 && flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
+}
+
 }  // namespace
+
+void launch_pack_tiles(const int64_t *part, int64_t nblocks, int64_t *tcnt, int64_t *toff, int64_t *tmap,
+                       int64_t *tmp, Control *ctrl, hipStream_t stream) {
+    if (nblocks <= 0) return;
+    const unsigned g = (unsigned)min((nblocks + 255) / 256, (int64_t)4096);
+    hipLaunchKernelGGL(k_pack_counts, dim3(g), dim3(256), 0, stream, part, nblocks, tcnt);
+    launch_scan(tcnt, toff, nblocks, tmp, &ctrl->packed_tiles, stream);
+    hipLaunchKernelGGL(k_pack_map, dim3(g), dim3(256), 0, stream, tcnt, toff, nblocks, tmap);
+}
 
 int64_t fused_tiles(int32_t rows, int rows_per_wave) {
     const int64_t tile =
@@ -826,7 +1057,11 @@ void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, b
                   hipStream_t stream) {
     if (op.arows <= 0) return;
     const dim3 grid((unsigned)fo.ntiles), block(kFusedTpb);
-    if (rows_per_wave == kFusedDealRows) {
+    if (rows_per_wave == kFusedPackRows) {
+        if (fo.ntiles <= 0) return;
+        if (small_b) hipLaunchKernelGGL((k_fused_packed<true>), grid, block, 0, stream, op, ws, fo);
+        else hipLaunchKernelGGL((k_fused_packed<false>), grid, block, 0, stream, op, ws, fo);
+    } else if (rows_per_wave == kFusedDealRows) {
         const dim3 dblock(kFusedDealTpb);
         if (small_b) hipLaunchKernelGGL((k_fused_dealt<true>), grid, dblock, 0, stream, op, ws, fo);
         else hipLaunchKernelGGL((k_fused_dealt<false>), grid, dblock, 0, stream, op, ws, fo);

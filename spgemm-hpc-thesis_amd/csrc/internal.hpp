@@ -68,6 +68,15 @@ constexpr int kFusedRows = 2;  // default rows per wave of a tile (1 or 2)
 constexpr int kFusedDealRows = 4;
 constexpr int kFusedDealTpb = 1024;
 constexpr int kFusedDealTile = 32;
+// Packed tiles (rows_per_wave == kFusedPackRows, a mode id): the single-pass
+// rows of each analyze block (kAnalyzeRows consecutive rows) are cut into
+// tiles of kFusedPackCap in row order, at least one tile per block; a tile
+// also carries the counts of the block's other rows up to its last
+// single-pass row (the block's last tile: to the block's end).  Tiles then
+// hold single-pass rows only, however few the block has.
+constexpr int kFusedPackRows = 8;
+constexpr int kFusedPackCap = 16;
+constexpr int kAnalyzeRows = 256;  // rows per analyze block (4 waves of 64)
 // Tile t's look-back status is word t * kStatusStride of the status array:
 // one status per 64-byte line.  Packed statuses put the few lines that every
 // waiting tile polls on one memory channel; spreading them took the 2^20-row
@@ -99,6 +108,7 @@ struct Control {
     long long bin_rows[8];          // rows per bin (statistics; listed or not)
     int large_next[2];              // next large row to take: symbolic, numeric kernel
     int pad_[2];
+    unsigned long long packed_tiles;  // tiles of the packed single-pass kernel
 };
 
 // Per-block partial sums of the analyze pass, reduced by one small kernel
@@ -113,7 +123,8 @@ struct FusedOut {
     double *cval;
     uint64_t *status;   // [ntiles] look-back words, zero before the launch
     int64_t *distinct;  // [ntiles] distinct columns of the tile's single-pass rows
-    int64_t ntiles;
+    int64_t ntiles;     // tiles launched (packed tiles: an upper bound; Control::packed_tiles is the count)
+    int64_t *tmap;      // packed tiles: tile -> (analyze block << 5) | tile of the block
 };
 
 // Operands as the kernels see them.  B rows are reached through a directory
@@ -189,6 +200,12 @@ bool prepare_numeric_kernels(size_t lds_per_block);  // numeric.hip; false: wide
 // rows_per_wave: 1 or 2 (rows of a tile per wave; the tile has kFusedTpb/64
 // waves), or kFusedDealRows (dealt 32-row tiles).
 int64_t fused_tiles(int32_t rows, int rows_per_wave);
+// Packed tiles: per analyze block max(1, ceil(single-pass rows / kFusedPackCap))
+// from the analyze partials (`part`, `nblocks` blocks) into tcnt, their
+// exclusive scan into toff (total into ctrl->packed_tiles; tmp as launch_scan)
+// and the tile map (fo.tmap, at most `ntiles` entries).
+void launch_pack_tiles(const int64_t *part, int64_t nblocks, int64_t *tcnt, int64_t *toff, int64_t *tmap,
+                       int64_t *tmp, Control *ctrl, hipStream_t stream);
 void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b, int rows_per_wave,
                   hipStream_t stream);
 // Exclusive scan of n int64 counts into out[0..n]; out[n] = total; total also

@@ -46,7 +46,7 @@ EXPORTED = [
     "spg_sfull_next_row", "spg_sfull_row_cells", "spg_sfull_finish", "spg_sfull_close", "spg_sha256_hex",
 ]
 
-PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE, PATH_SINGLE_PASS_DEALT = 0, 1, 2, 3
+PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE, PATH_SINGLE_PASS_DEALT, PATH_SINGLE_PASS_PACKED = 0, 1, 2, 3, 4
 
 
 @dataclass

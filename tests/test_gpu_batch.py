@@ -77,21 +77,22 @@ def test_batch_graph_replay_and_autotune(engine):
         batch.run()
         assert [digest(batch.result(i)) for i in range(len(a1))] == ref
     choice = batch.autotune(1)
-    assert choice in (0, 1, 2, 3)
+    assert choice in (0, 1, 2, 3, 4)
     for rep in range(2):
         batch.run()
         assert [digest(batch.result(i)) for i in range(len(a1))] == ref
     batch.close()
 
 
-def test_batch_dealt_tiles_equal_standalone(engine):
-    # Single pass over 32-row tiles with dealt small rows (the batch inherits
-    # the engine's path), eager and replayed from a graph, against each
-    # product alone on the default path.
+@pytest.mark.parametrize("path", [S.PATH_SINGLE_PASS_DEALT, S.PATH_SINGLE_PASS_PACKED])
+def test_batch_dealt_tiles_equal_standalone(engine, path):
+    # Single pass over dealt 32-row tiles or packed tiles of small rows (the
+    # batch inherits the engine's path), eager and replayed from a graph,
+    # against each product alone on the default path.
     jobs = small_jobs(20, seed=13)
     _, _, a1, b1 = waves(jobs)
     want = [digest(engine.multiply(a, b)) for a, b in zip(a1, b1)]
-    engine.set_path(S.PATH_SINGLE_PASS_DEALT)
+    engine.set_path(path)
     try:
         batch = S.Batch(engine, a1, b1)
         batch.run()

@@ -40,7 +40,7 @@ def all_cases():
 
 
 PATHS = [("auto", S.PATH_AUTO), ("single_pass", S.PATH_SINGLE_PASS), ("two_phase", S.PATH_TWO_PHASE),
-         ("single_pass_dealt", S.PATH_SINGLE_PASS_DEALT)]
+         ("single_pass_dealt", S.PATH_SINGLE_PASS_DEALT), ("single_pass_packed", S.PATH_SINGLE_PASS_PACKED)]
 
 
 @pytest.fixture(params=PATHS, ids=[p[0] for p in PATHS])
