@@ -851,6 +851,8 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused_packed(Opera
     __shared__ int64_t s_rp[BR];  // output offset of each owned row
     __shared__ uint64_t s_fm[BR / 64];
     __shared__ int64_t s_dist[NW];
+    __shared__ int64_t s_a0[BR];  // the block's rows: first A entry, entries, work bound
+    __shared__ int s_na[BR], s_ub[BR];
     const int wv = threadIdx.x >> 6;
     const int lane = lane_id();
     const int tid = threadIdx.x;
@@ -871,12 +873,18 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused_packed(Opera
     const int64_t r = br0 + tid;
     const bool valid = tid < BR && r < op.arows;
     const int64_t ubr = valid ? ws.ub[r] : 0;
-    const int64_t nar = valid ? op.aptr[r + 1] - op.aptr[r] : 0;
+    const int64_t a0r = valid ? op.aptr[r] : 0;
+    const int64_t nar = valid ? op.aptr[r + 1] - a0r : 0;
     const bool sp = valid && ubr > 0 && ubr <= kFusedCap && nar <= kFusedCap;
     const int64_t snr = valid && ubr > 0 && !sp ? ws.snnz[r] : 0;
     {
         const uint64_t m = __ballot(sp);
         if (wv < BR / 64 && lane == 0) s_fm[wv] = m;
+        if (sp) {  // the single-pass rows' data for the waves that take them
+            s_a0[tid] = a0r;
+            s_na[tid] = (int)nar;
+            s_ub[tid] = (int)ubr;
+        }
     }
     __syncthreads();
     uint64_t fm[BR / 64];
@@ -886,7 +894,8 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused_packed(Opera
         fm[i] = s_fm[i];
         nfb += __popcll(fm[i]);
     }
-    // Row (offset in the block) of single-pass rank q, -1 if none.
+    // Row (offset in the block) of single-pass rank q, -1 if none: the word
+    // holding it, then its bit by halving (popcount of the lower half).
     auto rank_row = [&](int q) -> int {
         if (q < 0 || q >= nfb) return -1;
 #pragma unroll
@@ -894,8 +903,20 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused_packed(Opera
             const int c = __popcll(fm[i]);
             if (q < c) {
                 uint64_t m = fm[i];
-                for (int k = 0; k < q; ++k) m &= m - 1ull;
-                return i * 64 + __ffsll((long long)m) - 1;
+                int pos = 0;
+#pragma unroll
+                for (int width = 32; width > 0; width >>= 1) {
+                    const uint64_t low = m & ((1ull << width) - 1ull);
+                    const int lc = __popcll(low);
+                    if (q >= lc) {
+                        q -= lc;
+                        m >>= width;
+                        pos += width;
+                    } else {
+                        m = low;
+                    }
+                }
+                return i * 64 + pos;
             }
             q -= c;
         }
@@ -914,11 +935,10 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused_packed(Opera
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         brow[k] = rank_row(q0 + R * wv + k);
-        fused[k] = brow[k] >= 0 && q0 + R * wv + k < q0 + kFusedPackCap;
-        const int64_t rr = br0 + (fused[k] ? brow[k] : 0);
-        a0[k] = fused[k] ? op.aptr[rr] : 0;
-        na[k] = fused[k] ? (int)(op.aptr[rr + 1] - a0[k]) : 0;
-        ubk[k] = fused[k] ? (int)ws.ub[rr] : 0;
+        fused[k] = brow[k] >= 0;
+        a0[k] = fused[k] ? s_a0[brow[k]] : 0;
+        na[k] = fused[k] ? s_na[brow[k]] : 0;
+        ubk[k] = fused[k] ? s_ub[brow[k]] : 0;
         cnt[k] = 0;
         out[k].exnz = 0;
         out[k].tot = 0;
