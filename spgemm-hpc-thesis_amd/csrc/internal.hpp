@@ -198,7 +198,8 @@ bool prepare_numeric_kernels(size_t lds_per_block);  // numeric.hip; false: wide
 // W2 and reserves the symbolic counts of the others.  small_b: every B entry
 // index i has 8*i < 2^32.
 // rows_per_wave: 1 or 2 (rows of a tile per wave; the tile has kFusedTpb/64
-// waves), or kFusedDealRows (dealt 32-row tiles).
+// waves), kFusedDealRows (dealt 32-row tiles) or kFusedPackRows (packed
+// tiles; fused_tiles does not apply, the plan launches an upper bound).
 int64_t fused_tiles(int32_t rows, int rows_per_wave);
 // Packed tiles: per analyze block max(1, ceil(single-pass rows / kFusedPackCap))
 // from the analyze partials (`part`, `nblocks` blocks) into tcnt, their

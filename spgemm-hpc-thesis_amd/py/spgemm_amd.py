@@ -533,6 +533,9 @@ class Batch:
         self._e._check(self._L.spg_batch_set_graph(self._h, 1 if on else 0))
 
     def autotune(self, repeats: int = 2) -> int:
+        """Time the pipeline variants and keep the fastest; returns its index:
+        0 single pass (2 rows per wave), 1 single pass (1 row per wave),
+        2 two-phase, 3 dealt 32-row tiles, 4 packed tiles of small rows."""
         c = C.c_int()
         self._bcheck(self._L.spg_batch_autotune(self._h, repeats, C.byref(c)))
         return int(c.value)
