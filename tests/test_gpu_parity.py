@@ -66,6 +66,46 @@ def test_golden_products_bit_exact(path_engine, golden):
             assert st["single_pass"] == (engine._path != S.PATH_TWO_PHASE), name
 
 
+def block_edge_case(rows, small_pattern, seed):
+    # Rows of A whose product is small (<= 256 products: single pass) or
+    # bigger (counted ahead) by small_pattern(r); some rows empty.
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lens = np.array([0 if r % 23 == 5 else (int(rng.integers(1, 9)) if small_pattern(r) else int(rng.integers(30, 60)))
+                     for r in range(rows)], dtype=np.int64)
+    rp = np.zeros(rows + 1, dtype=np.int64)
+    np.cumsum(lens, out=rp[1:])
+    a = O.Csr.make(rows, 700, rp, rng.integers(0, 700, size=int(rp[-1])), 2.0 * rng.random(int(rp[-1])) - 1.0)
+    b = O.Csr.make(700, 900, np.arange(701, dtype=np.int64) * 20, rng.integers(0, 900, size=14000),
+                   2.0 * rng.random(14000) - 1.0)
+    return a, b
+
+
+EDGE_SHAPES = [
+    ("one_row", 1, lambda r: True),
+    ("block_all_small", 256, lambda r: True),
+    ("block_plus_one", 257, lambda r: r % 3 != 0),
+    ("sixteen_per_block", 513, lambda r: r % 16 == 0 or r % 16 == 1),  # 32 small rows per block
+    ("seventeen_small", 300, lambda r: r < 17 or r > 290),
+    ("no_small_rows", 260, lambda r: False),
+    ("alternating", 1000, lambda r: (r // 7) % 2 == 0),
+]
+
+
+@pytest.mark.parametrize("path", [p[1] for p in PATHS], ids=[p[0] for p in PATHS])
+def test_single_pass_tile_edges(engine, path):
+    # Block and tile boundaries of every single-pass tiling (16-row tiles,
+    # dealt 32-row tiles, packed tiles per 256-row block) against the oracle.
+    engine.set_path(path)
+    try:
+        for i, (name, rows, pat) in enumerate(EDGE_SHAPES):
+            print("case", name, flush=True)
+            a, b = block_edge_case(rows, pat, 40 + i)
+            c = engine.multiply(to_s(a), to_s(b))
+            assert digest(c) == O.csr_digest(O.product(a, b)), name
+    finally:
+        engine.set_path(S.PATH_AUTO)
+
+
 def test_validators_accept_engine_output(engine):
     for name, a, b in list(cases.mm_cases()) + cases.synthetic_cases():
         if name == "overflow_product":
