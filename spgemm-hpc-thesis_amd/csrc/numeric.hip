@@ -40,6 +40,15 @@ __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws,
     uint16_t *s_bk = reinterpret_cast<uint16_t *>(s_b64) + 3 * CAP;  // bytes [6 CAP, 8 CAP)
     const int tid = threadIdx.x;
     const int na = cur.na;
+    // Rows of more than 256 products load their output range with their
+    // gathers, so its latency is not exposed after the sort; smaller rows load
+    // it after the sort, where its registers cost no wave per SIMD.
+    constexpr bool kEarlyRange = CAP >= 512;
+    int64_t cbase = 0, lim = 0;
+    if constexpr (kEarlyRange) {
+        cbase = crowptr[cur.row];
+        lim = crowptr[cur.row + 1] - cbase;
+    }
 
     // --- expansion into registers: column, product, serial index s
     int icol[ITEMS], isx[ITEMS];
@@ -144,8 +153,10 @@ __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws,
     __syncthreads();
 
     // --- ordered fold at each column's first position, scan, coalesced write
-    const int64_t cbase = crowptr[cur.row];
-    const int64_t lim = crowptr[cur.row + 1] - cbase;
+    if constexpr (!kEarlyRange) {
+        cbase = crowptr[cur.row];
+        lim = crowptr[cur.row + 1] - cbase;
+    }
     int run = 0;
     for (int x0 = 0; x0 < P; x0 += TPB) {
         const int x = x0 + tid;
