@@ -523,7 +523,7 @@ def main():
             exact &= rows_equal(c, 0, ref, pos[r])
 
     single = bool(st["single_pass"])
-    big_rows = int(sum(st["rows_per_bin"][3:]))  # bins W3, Block, Large run after the single-pass kernel
+    big_rows = int(sum(st["rows_per_bin"][3:]))  # bins W3, W4, Block, Large run after the single-pass kernel
     if single and not chunked and big_rows == 0:
         # the timed values phase is the single-pass kernel alone (named as rocprofv3 names it)
         kernel = f"k_fused<{'true' if b.nnz < (1 << 29) else 'false'}, 2>"

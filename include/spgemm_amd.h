@@ -84,8 +84,8 @@ typedef struct spg_stats {
                                   (scalarMultiplicationCount, spgemm_common.cpp:474-487) */
     int64_t structural_nnz;    /* distinct output coordinates */
     int64_t nnz;               /* stored output entries after exact-zero drop */
-    int64_t rows_per_bin[8];   /* rows per kernel bin: empty, wave<=128, wave<=512,
-                                  block<=2048 products, large (chunked) */
+    int64_t rows_per_bin[8];   /* rows per kernel bin, by products per row: empty,
+                                  <=128, <=256, <=512, <=1024, <=2048, large (chunked) */
     int32_t compacted;         /* 1 if exact zeros forced a compaction pass */
     int32_t bins;              /* number of bins used */
     int32_t single_pass;       /* 1 if the small rows took the single-pass kernel */

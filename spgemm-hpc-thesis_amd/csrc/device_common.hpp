@@ -278,6 +278,7 @@ __device__ __forceinline__ int classify(int64_t ub, int64_t na) {
     if (ub <= kCapW1 && na <= kCapW1) return kBinW1;
     if (ub <= kCapW2 && na <= kCapW2) return kBinW2;
     if (ub <= kCapW3 && na <= kCapW3) return kBinW3;
+    if (ub <= kCapW4 && na <= kCapW4) return kBinW4;
     if (ub <= kCapBlock && na <= kCapBlock) return kBinBlock;
     return kBinLarge;
 }

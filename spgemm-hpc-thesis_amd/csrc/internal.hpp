@@ -14,14 +14,20 @@ enum Bin : int {
     kBinW1 = 1,     // one 64-lane workgroup per row, <= kCapW1 products
     kBinW2 = 2,     // one 64-lane workgroup per row, <= kCapW2 products
     kBinW3 = 3,     // one 64-lane workgroup per row, <= kCapW3 products
-    kBinBlock = 4,  // one 256-lane workgroup per row, <= kCapBlock products
-    kBinLarge = 5,  // chunked fold onto a windowed dense accumulator in HBM
-    kNumBins = 6
+    kBinW4 = 4,     // one 128-lane workgroup per row, <= kCapW4 products
+    kBinBlock = 5,  // one 256-lane workgroup per row, <= kCapBlock products
+    kBinLarge = 6,  // chunked fold onto a windowed dense accumulator in HBM
+    kNumBins = 7
 };
 
 constexpr int kCapW1 = 128;
 constexpr int kCapW2 = 256;
 constexpr int kCapW3 = 512;
+// Bin W4 holds rows of up to 1024 products in half the LDS and registers per
+// row of bin Block, so twice as many rows are resident per CU (six against
+// three for the values, seven against three for the counts).
+constexpr int kCapW4 = 1024;
+constexpr int kTpbW4 = 128;
 constexpr int kCapBlock = 2048;
 constexpr int kTpbBlock = 256;
 // Workgroup sizes of the symbolic kernels for bins W3 and Block: ITEMS =
@@ -33,7 +39,10 @@ constexpr int kTpbBlock = 256;
 #ifndef SPG_SYM_TPB_BLOCK
 #define SPG_SYM_TPB_BLOCK 256
 #endif
-constexpr int kSymTpbW3 = SPG_SYM_TPB_W3, kSymTpbBlock = SPG_SYM_TPB_BLOCK;
+#ifndef SPG_SYM_TPB_W4
+#define SPG_SYM_TPB_W4 256
+#endif
+constexpr int kSymTpbW3 = SPG_SYM_TPB_W3, kSymTpbW4 = SPG_SYM_TPB_W4, kSymTpbBlock = SPG_SYM_TPB_BLOCK;
 // Large-row path: column windows of up to kLdsBitsCols columns keep the
 // presence bitmap in LDS (32 KB), wider windows in HBM.  Products per chunk
 // and workgroup size per case (measured: 1024/256 is 11 % faster than
@@ -185,7 +194,7 @@ void launch_dir32(const int64_t *bptr, int64_t n, uint32_t *out, hipStream_t str
 void launch_sum(const int64_t *in, int64_t n, unsigned long long *dst, hipStream_t stream);
 // Bin sets of the launchers: bit b selects bin b.
 constexpr unsigned kSmallBins = (1u << kBinW1) | (1u << kBinW2);
-constexpr unsigned kBigBins = (1u << kBinW3) | (1u << kBinBlock) | (1u << kBinLarge);
+constexpr unsigned kBigBins = (1u << kBinW3) | (1u << kBinW4) | (1u << kBinBlock) | (1u << kBinLarge);
 constexpr unsigned kAllBins = kSmallBins | kBigBins;
 // Symbolic phase for the selected bins (symbolic.hip); kernels read their row
 // counts from the control block, so no host synchronisation is needed before it.

@@ -658,6 +658,8 @@ void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_coun
         launch_esc<64, kCapW2>(op, ws, kBinW2, bin_counts[kBinW2], cus, crowptr, ccol, cval, stream);
     if (bins & (1u << kBinW3))
         launch_esc<64, kCapW3>(op, ws, kBinW3, bin_counts[kBinW3], cus, crowptr, ccol, cval, stream);
+    if (bins & (1u << kBinW4))
+        launch_esc<kTpbW4, kCapW4>(op, ws, kBinW4, bin_counts[kBinW4], cus, crowptr, ccol, cval, stream);
     if (bins & (1u << kBinBlock))
         launch_esc<kTpbBlock, kCapBlock>(op, ws, kBinBlock, bin_counts[kBinBlock], cus, crowptr, ccol, cval, stream);
     if ((bins & (1u << kBinLarge)) && bin_counts[kBinLarge] > 0) {

@@ -344,6 +344,7 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
     if (bins & (1u << kBinW1)) launch_hash<64, kCapW1>(op, ws, kBinW1, cus, stream);
     if (bins & (1u << kBinW2)) launch_hash<64, kCapW2>(op, ws, kBinW2, cus, stream);
     if (bins & (1u << kBinW3)) launch_hash<kSymTpbW3, kCapW3>(op, ws, kBinW3, cus, stream);
+    if (bins & (1u << kBinW4)) launch_hash<kSymTpbW4, kCapW4>(op, ws, kBinW4, cus, stream);
     if (bins & (1u << kBinBlock)) launch_hash<kSymTpbBlock, kCapBlock>(op, ws, kBinBlock, cus, stream);
     if ((bins & (1u << kBinLarge)) && ws.lblocks > 0) {
         if (ws.lwidth <= kLdsBitsCols)
