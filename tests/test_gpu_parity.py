@@ -106,6 +106,65 @@ def test_single_pass_tile_edges(engine, path):
         engine.set_path(S.PATH_AUTO)
 
 
+
+BIN_CAPS = [(128, 1), (256, 2), (512, 3), (1024, 4), (2048, 5)]  # (products and A entries, bin)
+
+
+def expected_bin(p, na):
+    if p == 0:
+        return 0
+    for cap, b in BIN_CAPS:
+        if p <= cap and na <= cap:
+            return b
+    return 6
+
+
+def bin_edge_case(seed):
+    # Rows of exactly 256/257, 512/513, 1024/1025, 2048/2049 products (and
+    # neighbours), and rows whose A-entry count alone decides the bin: B rows
+    # of 64, 16 and 1 entries make any product count reachable.
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lens_b = np.array([64] * 100 + [16] * 100 + [1] * 100, dtype=np.int64)
+    bp = np.zeros(301, dtype=np.int64)
+    np.cumsum(lens_b, out=bp[1:])
+    ncols = 3000
+    b = O.Csr.make(300, ncols, bp, rng.integers(0, ncols, size=int(bp[-1])), 2.0 * rng.random(int(bp[-1])) - 1.0)
+    rows = []
+    for target in (255, 256, 257, 511, 512, 513, 1023, 1024, 1025, 2047, 2048, 2049):
+        for _ in range(3):
+            q, r = divmod(target, 64)
+            ent = list(rng.integers(0, 100, size=q)) + list(rng.integers(100, 200, size=r // 16)) + \
+                list(rng.integers(200, 300, size=r % 16))
+            rng.shuffle(ent)
+            rows.append(ent)
+    for na in (300, 600, 1100, 2100):  # products = A entries (B rows of one entry)
+        rows.append(list(rng.integers(200, 300, size=na)))
+    rows.insert(5, [])
+    rows.insert(17, list(rng.integers(100, 200, size=3)))
+    rp = np.zeros(len(rows) + 1, dtype=np.int64)
+    np.cumsum([len(e) for e in rows], out=rp[1:])
+    col = np.array([c for e in rows for c in e], dtype=np.int64)
+    a = O.Csr.make(len(rows), 300, rp, col, 2.0 * rng.random(len(col)) - 1.0)
+    p = np.array([int(lens_b[e].sum()) if e else 0 for e in rows])
+    want = np.bincount([expected_bin(int(pi), len(e)) for pi, e in zip(p, rows)], minlength=7)
+    return a, b, want
+
+
+@pytest.mark.parametrize("path", [p[1] for p in PATHS], ids=[p[0] for p in PATHS])
+def test_bin_edges(engine, path):
+    # Every bin boundary (by products and by A entries) on every pipeline,
+    # bit for bit against the oracle.
+    engine.set_path(path)
+    try:
+        a, b, want = bin_edge_case(77)
+        c = engine.multiply(to_s(a), to_s(b))
+        st = engine.stats()
+        assert list(st["rows_per_bin"]) == list(want)
+        assert all(x > 0 for x in want[1:])  # every bin is exercised
+        assert digest(c) == O.csr_digest(O.product(a, b))
+    finally:
+        engine.set_path(S.PATH_AUTO)
+
 def test_validators_accept_engine_output(engine):
     for name, a, b in list(cases.mm_cases()) + cases.synthetic_cases():
         if name == "overflow_product":
