@@ -35,9 +35,11 @@ __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws,
                                         int64_t *s_b64, int *s_c, int *s_wsi, unsigned long long &stored,
                                         unsigned &flags) {
     constexpr int ITEMS = CAP / TPB;
-    int32_t *s_col = reinterpret_cast<int32_t *>(s_b64);             // bytes [0, 4 CAP)
-    uint16_t *s_sx = reinterpret_cast<uint16_t *>(s_b64) + 2 * CAP;  // bytes [4 CAP, 6 CAP)
-    uint16_t *s_bk = reinterpret_cast<uint16_t *>(s_b64) + 3 * CAP;  // bytes [6 CAP, 8 CAP)
+    // Bucketed items are keys (column - row minimum) << 32 | s: one LDS word
+    // per item, ordered as (column, s), the bucket recomputed from the key.
+    // Sorted columns then go to a dense int32 array over the dead keys.
+    uint64_t *s_key = reinterpret_cast<uint64_t *>(s_b64);
+    int32_t *s_col = reinterpret_cast<int32_t *>(s_b64);
     const int tid = threadIdx.x;
     const int na = cur.na;
     // Rows of more than 256 products load their output range with their
@@ -114,9 +116,7 @@ __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws,
     for (int j = 0; j < ITEMS; ++j) {
         if (isx[j] >= 0) {
             const int d = s_c[bk[j]] + pos[j];
-            s_col[d] = icol[j];
-            s_sx[d] = (uint16_t)isx[j];
-            s_bk[d] = (uint16_t)bk[j];
+            s_key[d] = ((uint64_t)(uint32_t)((int64_t)icol[j] - (int64_t)cmn) << 32) | (uint32_t)isx[j];
             s_a[d] = ival[j];
         }
     }
@@ -128,16 +128,13 @@ __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws,
         const int x = tid + j * TPB;
         isx[j] = -1;
         if (x < P) {
-            const int32_t c = s_col[x];
-            const int sx = s_sx[x];
-            const int b = s_bk[x];
+            const uint64_t k = s_key[x];
+            const uint32_t rel = (uint32_t)(k >> 32);
+            const int b = (int)(rel >> sh);
             const int st = s_c[b], en = s_c[b + 1];
             int rank = 0;
-            for (int y = st; y < en; ++y) {
-                const int32_t cy = s_col[y];
-                rank += (cy < c) | ((cy == c) & (s_sx[y] < sx));
-            }
-            icol[j] = c;
+            for (int y = st; y < en; ++y) rank += s_key[y] < k;
+            icol[j] = (int32_t)((int64_t)cmn + (int64_t)rel);
             ival[j] = s_a[x];
             isx[j] = st + rank;
         }
@@ -192,7 +189,7 @@ __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws,
 template <int TPB, int CAP>
 struct EscLds {
     double a[CAP];     // meta: A value of entry e | items: product value
-    int64_t b64[CAP];  // meta: B start of entry e | items: column, serial index, bucket
+    int64_t b64[CAP];  // meta: B start of entry e | items: keys (column, s), then sorted columns
     int c[CAP + 1];    // meta: product offset of entry e | bucket counts, then bucket starts
     int wsi[2 * (TPB / 64) + 2];
 };
