@@ -102,7 +102,10 @@ __device__ __forceinline__ unsigned tile_entries(const Operands &op, const Works
             }
             lv[h] = len;
             if (inv[h]) {
-                __builtin_nontemporal_store(s0, &ws.ebst[ev[h]]);
+                if (ws.mode & kWsNarrow)
+                    __builtin_nontemporal_store((uint32_t)s0, reinterpret_cast<uint32_t *>(ws.ebst) + ev[h]);
+                else
+                    __builtin_nontemporal_store(s0, &ws.ebst[ev[h]]);
                 __builtin_nontemporal_store((int32_t)len, &ws.elen[ev[h]]);
             }
         }
@@ -153,7 +156,7 @@ __device__ __forceinline__ void tile_finish(const Operands &op, const Workspace 
         const unsigned long long m = __ballot(b == bb);
         if (!m) continue;
         if (lane == 0) s_part[2 + bb] = (int64_t)__popcll(m);
-        const bool listed = bb != kBinEmpty && (ws.list_small || (bb != kBinW1 && bb != kBinW2));
+        const bool listed = bb != kBinEmpty && ((ws.mode & kWsListSmall) || (bb != kBinW1 && bb != kBinW2));
         if (!listed) continue;
         int basei = 0;
         if (lane == 0) basei = atomicAdd(&ws.ctrl->bin_count[bb], (int)__popcll(m));

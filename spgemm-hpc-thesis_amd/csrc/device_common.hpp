@@ -329,6 +329,19 @@ __device__ __forceinline__ RowRec rec_decode(int w) {
     return r;
 }
 
+// Start of the B row A entry e references (the analyze pass's stream), read
+// as 32-bit starts when NARROW (mode bit kWsNarrow).  Hot kernels are
+// instantiated for both formats, so they never branch on it; cold paths use
+// entry_bstart_any.
+template <bool NARROW>
+__device__ __forceinline__ int64_t entry_bstart(const Workspace &ws, int64_t e) {
+    if constexpr (NARROW) return (int64_t)__builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(ws.ebst) + e);
+    else return __builtin_nontemporal_load(&ws.ebst[e]);
+}
+__device__ __forceinline__ int64_t entry_bstart_any(const Workspace &ws, int64_t e) {
+    return (ws.mode & kWsNarrow) ? entry_bstart<true>(ws, e) : entry_bstart<false>(ws, e);
+}
+
 // A row's per-entry metadata held in registers (thread e holds entry e), so
 // the next row's metadata can be in flight while the current row computes.
 struct MetaPre {
@@ -340,11 +353,11 @@ struct MetaPre {
 // Loads are unconditional (lanes past the last entry re-read the row's last
 // entry) so the destination registers never need a prior initialisation,
 // which would force a wait on loads still in flight.
-template <bool VALUES>
+template <bool VALUES, bool NARROW>
 __device__ __forceinline__ MetaPre meta_prefetch(const Operands &op, const Workspace &ws, const RowRec &r) {
     MetaPre m;
     const int64_t e = r.a0 + min((int)threadIdx.x, max(r.na - 1, 0));
-    m.bst = __builtin_nontemporal_load(&ws.ebst[e]);
+    m.bst = entry_bstart<NARROW>(ws, e);
     m.len = __builtin_nontemporal_load(&ws.elen[e]);
     m.av = VALUES ? __builtin_nontemporal_load(&op.aval[e]) : 0.0;
     return m;
@@ -353,7 +366,7 @@ __device__ __forceinline__ MetaPre meta_prefetch(const Operands &op, const Works
 // Row metadata into LDS: B starts, (A values), exclusive product offsets
 // off[0..na] and the longest referenced B row.  Entries < TPB come from the
 // prefetched registers when `pre_ok`.
-template <int TPB, bool VALUES>
+template <int TPB, bool VALUES, bool NARROW>
 __device__ __forceinline__ void meta_to_lds(const Operands &op, const Workspace &ws, const RowRec &r,
                                             const MetaPre &pre, bool pre_ok, int64_t *s_bst, double *s_av,
                                             int *s_off, int *s_wsi, int &P, int &L) {
@@ -371,7 +384,7 @@ __device__ __forceinline__ void meta_to_lds(const Operands &op, const Workspace 
                 len = pre.len;
                 av = pre.av;
             } else {
-                bst = __builtin_nontemporal_load(&ws.ebst[r.a0 + e]);
+                bst = entry_bstart<NARROW>(ws, r.a0 + e);
                 len = __builtin_nontemporal_load(&ws.elen[r.a0 + e]);
                 if (VALUES) av = __builtin_nontemporal_load(&op.aval[r.a0 + e]);
             }

@@ -83,6 +83,11 @@ spg_status plan_workspace(spg_context *ctx, ProductPlan &p) {
     Workspace &ws = p.ws;
     ws = Workspace{};
     ws.ebst = ctx->ebst.as<int64_t>();
+    // Every start is at most b_span (missing rows write 0).  When B is small
+    // (b_span < 2^29, the plan's small_b, which also picks the single-pass
+    // kernel's 32-bit addressing) the stream holds 32-bit starts: 4 bytes per
+    // A entry less written by analyze and read by every later kernel.
+    if (p.small_b) ws.mode |= kWsNarrow;
     ws.elen = ctx->elen.as<int32_t>();
     ws.ub = ctx->ub.as<int64_t>();
     ws.bin_recs = ctx->bin_rows.as<RowRec>();
@@ -243,7 +248,7 @@ spg_status run_product(spg_context *ctx, ProductPlan &p) {
     // Single-pass path: only the rows of the big bins are listed and counted
     // ahead; the rows of bins W1, W2 are computed in one pass by the fused kernel.
     const bool try_fused = ctx->path != SPG_PATH_TWO_PHASE;
-    p.ws.list_small = try_fused ? 0 : 1;
+    p.ws.mode = (p.ws.mode & ~kWsListSmall) | (try_fused ? 0 : kWsListSmall);
     enqueue_counts(ctx, p, try_fused ? kBigBins : kAllBins, !try_fused);
     if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
     if ((s = read_control(ctx)) != SPG_OK) return s;
@@ -259,7 +264,7 @@ spg_status run_product(spg_context *ctx, ProductPlan &p) {
         if (!p.fused) {
             // Two-phase after all: list the small rows too (analyze again) and
             // count them.
-            p.ws.list_small = 1;
+            p.ws.mode |= kWsListSmall;
             enqueue_counts(ctx, p, kAllBins, true);
             if ((s = read_control(ctx)) != SPG_OK) return s;
             h = *ctx->h_ctrl;

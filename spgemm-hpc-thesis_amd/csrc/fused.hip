@@ -105,7 +105,7 @@ __device__ __forceinline__ MetaReg<SMALLB> load_meta(const Operands &op, const W
     using bst_t = typename FusedWave<SMALLB>::bst_t;
     const int64_t e = a0 + min(c0 + lane_id(), max(na - 1, 0));
     MetaReg<SMALLB> m;
-    m.bst = (bst_t)__builtin_nontemporal_load(&ws.ebst[e]);
+    m.bst = (bst_t)entry_bstart<SMALLB>(ws, e);  // small B: 32-bit stream
     m.len = __builtin_nontemporal_load(&ws.elen[e]);
     m.av = __builtin_nontemporal_load(&op.aval[e]);
     return m;
@@ -152,7 +152,7 @@ __device__ __forceinline__ MetaReg<SMALLB> load_meta2(const Operands &op, const 
     const int p = min(c0 + lane_id(), max(E - 1, 0));
     const int64_t e = p < nA ? aA + p : aB + (p - nA);
     MetaReg<SMALLB> m;
-    m.bst = (bst_t)__builtin_nontemporal_load(&ws.ebst[e]);
+    m.bst = (bst_t)entry_bstart<SMALLB>(ws, e);  // small B: 32-bit stream
     m.len = __builtin_nontemporal_load(&ws.elen[e]);
     m.av = __builtin_nontemporal_load(&op.aval[e]);
     return m;

@@ -162,6 +162,10 @@ struct RowRec {
     int64_t a0;
 };
 
+// Workspace::mode bits.
+constexpr int kWsListSmall = 1;  // analyze also lists the rows of bins W1, W2 (two-phase path)
+constexpr int kWsNarrow = 2;     // ebst holds 32-bit starts: B has fewer than 2^29 entries (small_b)
+
 struct Workspace {
     int64_t *ebst;        // [nnz(A)] start of the B row each A entry references
     int32_t *elen;        // [nnz(A)] length of that B row (0 for a bad/missing row)
@@ -175,7 +179,7 @@ struct Workspace {
     int64_t *scan_tmp;    // block partials of the scan
     Control *ctrl;
     int64_t *part;        // [analyze blocks * kAnalyzeFields] partial sums
-    int list_small;       // analyze also lists the rows of bins W1, W2 (two-phase path)
+    int mode;             // kWs* bits
     uint32_t *lbits;      // [lblocks * lwidth/32] large-row presence bitmaps (zero between rows)
     double *lacc;         // [lblocks * lwidth] large-row dense accumulators
     int64_t lwidth;       // column window of the large-row path (multiple of 32)

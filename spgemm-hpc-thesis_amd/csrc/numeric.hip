@@ -195,17 +195,17 @@ struct EscLds {
 };
 
 // One row of the software pipeline (see hash_step in symbolic.hip).
-template <int TPB, int CAP>
+template <int TPB, int CAP, bool NARROW>
 __device__ __forceinline__ void esc_step(const Operands &op, const Workspace &ws, const RowRec *list, int count,
                                          int G, int &i, RowRec &cur, const MetaPre &pcur, MetaPre &pnext,
                                          int &wnext, const int64_t *crowptr, int32_t *ccol, double *cval,
                                          EscLds<TPB, CAP> &sh, unsigned long long &stored, unsigned &flags) {
     const int tid = threadIdx.x;
     int P, L;
-    meta_to_lds<TPB, true>(op, ws, cur, pcur, cur.na <= TPB, sh.b64, sh.a, sh.c, sh.wsi, P, L);
+    meta_to_lds<TPB, true, NARROW>(op, ws, cur, pcur, cur.na <= TPB, sh.b64, sh.a, sh.c, sh.wsi, P, L);
     const RowRec nxt = i + G < count ? rec_decode(wnext) : cur;
     wnext = rec_word(list, min(i + 2 * G, count - 1));
-    pnext = meta_prefetch<true>(op, ws, nxt);
+    pnext = meta_prefetch<true, NARROW>(op, ws, nxt);
     if (P > CAP) flags |= kErrInternal;  // cannot happen: binning used the same lengths
     if (P <= CAP) {
         if (tid == 0) sh.c[cur.na] = P;
@@ -217,7 +217,7 @@ __device__ __forceinline__ void esc_step(const Operands &op, const Workspace &ws
     i += G;
 }
 
-template <int TPB, int CAP>
+template <int TPB, int CAP, bool NARROW>
 __global__ __launch_bounds__(TPB) void k_numeric_esc(Operands op, Workspace ws, int binid, const int64_t *crowptr,
                                                      int32_t *ccol, double *cval) {
     static_assert(CAP % TPB == 0 && CAP <= 65536, "CAP must be a multiple of TPB and fit 16-bit indices");
@@ -232,13 +232,15 @@ __global__ __launch_bounds__(TPB) void k_numeric_esc(Operands op, Workspace ws, 
     if (i < count) {
         // Two-step unrolled loop with ping-pong prefetch registers (pa, pb).
         RowRec cur = rec_decode(rec_word(list, i));
-        MetaPre pa = meta_prefetch<true>(op, ws, cur), pb;
+        MetaPre pa = meta_prefetch<true, NARROW>(op, ws, cur), pb;
         int wnext = rec_word(list, min(i + G, count - 1));
         for (;;) {
             if (i >= count) break;
-            esc_step<TPB, CAP>(op, ws, list, count, G, i, cur, pa, pb, wnext, crowptr, ccol, cval, sh, stored, flags);
+            esc_step<TPB, CAP, NARROW>(op, ws, list, count, G, i, cur, pa, pb, wnext, crowptr, ccol, cval, sh, stored,
+                                       flags);
             if (i >= count) break;
-            esc_step<TPB, CAP>(op, ws, list, count, G, i, cur, pb, pa, wnext, crowptr, ccol, cval, sh, stored, flags);
+            esc_step<TPB, CAP, NARROW>(op, ws, list, count, G, i, cur, pb, pa, wnext, crowptr, ccol, cval, sh, stored,
+                                       flags);
         }
     }
     if (tid == 0 && stored) atomicAdd(&ws.ctrl->actual, stored);
@@ -270,7 +272,7 @@ __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Wo
         const int e = f + j;
         if (e < nbt) {
             const int64_t len = ws.elen[e0 + e];
-            s_bst[e] = ws.ebst[e0 + e];
+            s_bst[e] = entry_bstart_any(ws, e0 + e);
             s_av[e] = op.aval[e0 + e];
             s_off[e] = len;
             mylen += len;
@@ -620,7 +622,7 @@ template <int TPB, int CAP>
 void launch_esc(const Operands &op, const Workspace &ws, int bin, int rows, int cus, const int64_t *crowptr,
                 int32_t *ccol, double *cval, hipStream_t stream) {
     if (rows <= 0) return;
-    auto k = k_numeric_esc<TPB, CAP>;
+    auto k = (ws.mode & kWsNarrow) ? k_numeric_esc<TPB, CAP, true> : k_numeric_esc<TPB, CAP, false>;
     const int g = occupancy_grid(k, TPB, cus, rows);
     hipLaunchKernelGGL(k, dim3(g), dim3(TPB), 0, stream, op, ws, bin, crowptr, ccol, cval);
 }

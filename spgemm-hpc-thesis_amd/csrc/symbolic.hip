@@ -45,7 +45,7 @@ struct HashLds {
 // One row of the software pipeline: the row's metadata comes from registers
 // loaded one step earlier (pcur), and the next row's record and metadata are
 // issued into the other register set (pnext) before this row's gathers.
-template <int TPB, int CAP>
+template <int TPB, int CAP, bool NARROW>
 __device__ __forceinline__ void hash_step(const Operands &op, const Workspace &ws, const RowRec *list, int count,
                                           int G, int &i, RowRec &cur, const MetaPre &pcur, MetaPre &pnext,
                                           int &wnext, HashLds<TPB, CAP> &sh, unsigned long long &sym,
@@ -56,10 +56,10 @@ __device__ __forceinline__ void hash_step(const Operands &op, const Workspace &w
     constexpr int BMW = HashLds<TPB, CAP>::BMW;
     const int tid = threadIdx.x;
     int P, L;
-    meta_to_lds<TPB, false>(op, ws, cur, pcur, cur.na <= TPB, sh.bst, nullptr, sh.off, sh.wsi, P, L);
+    meta_to_lds<TPB, false, NARROW>(op, ws, cur, pcur, cur.na <= TPB, sh.bst, nullptr, sh.off, sh.wsi, P, L);
     const RowRec nxt = i + G < count ? rec_decode(wnext) : cur;
     wnext = rec_word(list, min(i + 2 * G, count - 1));
-    pnext = meta_prefetch<false>(op, ws, nxt);
+    pnext = meta_prefetch<false, NARROW>(op, ws, nxt);
     if (P > CAP) flags |= kErrInternal;  // cannot happen: binning used the same lengths
     if (P <= CAP) {
         if (tid == 0) sh.off[cur.na] = P;
@@ -156,8 +156,14 @@ __device__ __forceinline__ void hash_step(const Operands &op, const Workspace &w
     i += G;
 }
 
+// Bins W3 and W4 keep six waves per SIMD (80 VGPRs, no spills; W4's LDS
+// allows seven 256-thread workgroups per CU, so registers bound its resident rows).
 template <int TPB, int CAP>
-__global__ __launch_bounds__(TPB) void k_symbolic_hash(Operands op, Workspace ws, int binid) {
+constexpr int hash_min_waves() { return CAP == kCapW3 || CAP == kCapW4 ? 6 : 1; }
+
+template <int TPB, int CAP, bool NARROW>
+__global__ __launch_bounds__(TPB, (hash_min_waves<TPB, CAP>())) void k_symbolic_hash(Operands op, Workspace ws,
+                                                                                     int binid) {
     __shared__ HashLds<TPB, CAP> sh;
     const int tid = threadIdx.x;
     for (int q = tid; q < HashLds<TPB, CAP>::TABW; q += TPB) sh.tab[q] = kEmptyKey;
@@ -172,13 +178,13 @@ __global__ __launch_bounds__(TPB) void k_symbolic_hash(Operands op, Workspace ws
         // Two-step unrolled loop with ping-pong prefetch registers (pa, pb), so
         // no register that an outstanding load writes is ever copied.
         RowRec cur = rec_decode(rec_word(list, i));
-        MetaPre pa = meta_prefetch<false>(op, ws, cur), pb;
+        MetaPre pa = meta_prefetch<false, NARROW>(op, ws, cur), pb;
         int wnext = rec_word(list, min(i + G, count - 1));
         for (;;) {
             if (i >= count) break;
-            hash_step<TPB, CAP>(op, ws, list, count, G, i, cur, pa, pb, wnext, sh, sym, flags);
+            hash_step<TPB, CAP, NARROW>(op, ws, list, count, G, i, cur, pa, pb, wnext, sh, sym, flags);
             if (i >= count) break;
-            hash_step<TPB, CAP>(op, ws, list, count, G, i, cur, pb, pa, wnext, sh, sym, flags);
+            hash_step<TPB, CAP, NARROW>(op, ws, list, count, G, i, cur, pb, pa, wnext, sh, sym, flags);
         }
     }
     if (tid == 0 && sym) atomicAdd(&ws.ctrl->sym_total, sym);
@@ -197,7 +203,7 @@ __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Wo
         const int e = f + j;
         if (e < nbt) {
             const int64_t len = ws.elen[e0 + e];
-            s_bst[e] = ws.ebst[e0 + e];
+            s_bst[e] = entry_bstart_any(ws, e0 + e);
             if (want_values) s_av[e] = op.aval[e0 + e];
             s_off[e] = len;
             mylen += len;
@@ -307,7 +313,7 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
 
 template <int TPB, int CAP>
 void launch_hash(const Operands &op, const Workspace &ws, int bin, int cus, hipStream_t stream) {
-    auto k = k_symbolic_hash<TPB, CAP>;
+    auto k = (ws.mode & kWsNarrow) ? k_symbolic_hash<TPB, CAP, true> : k_symbolic_hash<TPB, CAP, false>;
     const int g = occupancy_grid(k, TPB, cus, op.arows);
     hipLaunchKernelGGL(k, dim3(g), dim3(TPB), 0, stream, op, ws, bin);
 }
