@@ -29,7 +29,9 @@ value     = 2*P / step time (GFLOP/s), P = scalar products (the reference's flop
 roofline  = HBM-bound: algorithmic bytes of the dominant kernel / its average
             duration, measured with HIP events on the engine's stream around
             that launch.  For the single-pass kernel k_fused: 32 B per row +
-            20 B per A entry + 12 B per scalar product + 12 B per C entry.
+            16 B per A entry (B row start and length, 4 B each when B has fewer
+            than 2^29 entries, else 12 B; A value 8 B) + 12 B per scalar
+            product + 12 B per C entry.
 cpu_baseline = the reference's own spgemm() (oracle/_ref/refdrv, -np 1, OpenMP
             guided/64) on a bounded row sample, or the C oracle when that binary
             is absent.
@@ -523,14 +525,15 @@ def main():
             exact &= rows_equal(c, 0, ref, pos[r])
 
     single = bool(st["single_pass"])
+    entry_bytes = (8 if b.nnz < (1 << 29) else 12) + 8  # B row start + length, A value
     big_rows = int(sum(st["rows_per_bin"][3:]))  # bins W3, W4, Block, Large run after the single-pass kernel
     if single and not chunked and big_rows == 0:
         # the timed values phase is the single-pass kernel alone (named as rocprofv3 names it)
         kernel = f"k_fused<{'true' if b.nnz < (1 << 29) else 'false'}, 2>"
-        kernel_bytes = 32 * a.rows + 20 * a.nnz + 12 * P + 12 * nnz
+        kernel_bytes = 32 * a.rows + entry_bytes * a.nnz + 12 * P + 12 * nnz
     else:
         kernel = "numeric phase (single-pass and big-row kernels)" if single else "numeric phase"
-        kernel_bytes = 40 * a.rows + 20 * a.nnz + 12 * P + 12 * nnz
+        kernel_bytes = 40 * a.rows + entry_bytes * a.nnz + 12 * P + 12 * nnz
     kernel_avg = statistics.mean(kernel_ms)
     achieved = kernel_bytes / (kernel_avg * 1e-3) / 1e9
     model_bytes = W.streaming_bytes(a, b.rows, P, nnz)
