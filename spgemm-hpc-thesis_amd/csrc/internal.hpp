@@ -25,12 +25,12 @@ constexpr int kCapW2 = 256;
 constexpr int kCapW3 = 512;
 // Bin W4 holds rows of up to 1024 products in half the LDS and registers per
 // row of bin Block, so twice as many rows are resident per CU (six against
-// three for the values, seven against three for the counts).
+// three, for the values and for the counts).
 constexpr int kCapW4 = 1024;
 constexpr int kTpbW4 = 128;
 constexpr int kCapBlock = 2048;
 constexpr int kTpbBlock = 256;
-// Workgroup sizes of the symbolic kernels for bins W3 and Block: ITEMS =
+// Workgroup sizes of the symbolic kernels for bins W3, W4 and Block: ITEMS =
 // capacity / size products per thread.  128 for W3 measured 11 % faster than
 // 64 on batched products (more resident waves per CU); wider was slower.
 #ifndef SPG_SYM_TPB_W3
