@@ -30,9 +30,13 @@ extern "C" {
  *                       src/common/spgemm_common.cpp:106-110; malformed CSR views)
  *   SPG_EOVERFLOW    -> std::overflow_error (src/common/csr_matrix.hpp:13-41)
  *   SPG_ERANGE       -> std::runtime_error ("A contains a column index outside B's
- *                       row range", src/common/spgemm_common.cpp:141-143)
+ *                       row range", src/common/spgemm_common.cpp:141-143): batched
+ *                       and served products, whose semantics are the serial
+ *                       product's
  *   SPG_EMISSING_ROW -> fail("missing a remote sparse row required by A"),
- *                       src/baselines/spgemm_exchange.cpp:82-85
+ *                       src/baselines/spgemm_exchange.cpp:82-85: the drop-in
+ *                       multiply, for every A column outside the local block
+ *                       and the halo
  *   SPG_ENOMEM, SPG_EDEVICE, SPG_ENODEVICE, SPG_ESTATE, SPG_EPARSE ->
  *                       std::runtime_error / std::bad_alloc */
 typedef enum spg_status {
@@ -89,7 +93,8 @@ typedef struct spg_stats {
     int32_t compacted;         /* 1 if exact zeros forced a compaction pass */
     int32_t bins;              /* number of bins used */
     int32_t single_pass;       /* 1 if the small rows took the single-pass kernel */
-    int32_t reserved_;
+    int32_t wide_format;       /* 1 if the 64-bit entry stream and B addressing ran
+                                  (B of 2^29 entries or more, or SPG_FORMAT_WIDE) */
     double ms_total;           /* device time of the whole pipeline (events), if timing on */
     double ms_symbolic;        /* device time of the symbolic phase, if timing on */
     double ms_numeric;         /* device time of the numeric phase, if timing on */
@@ -129,6 +134,16 @@ enum {
     SPG_PATH_SINGLE_PASS_PACKED = 4
 };
 spg_status spg_set_path(spg_context *ctx, int path);
+/* Encoding of the per-A-entry B-row stream and of B addressing in the
+ * single-pass kernel:
+ *   SPG_FORMAT_AUTO  32-bit B-row starts and byte offsets when B (local block
+ *                    plus halo) has fewer than 2^29 entries, else 64-bit
+ *   SPG_FORMAT_WIDE  always the 64-bit kernels, those that large B operands
+ *                    and big batches take, so that small products reach them
+ * Results are identical in both formats.  Batches and accumulators created
+ * from the context inherit the setting.  No counterpart in the reference. */
+enum { SPG_FORMAT_AUTO = 0, SPG_FORMAT_WIDE = 1 };
+spg_status spg_set_format(spg_context *ctx, int format);
 /* Record HIP events around the phases of each multiply (spg_stats.ms_*). */
 spg_status spg_set_timing(spg_context *ctx, int enabled);
 spg_status spg_last_stats(const spg_context *ctx, spg_stats *out);
@@ -146,9 +161,13 @@ spg_status spg_last_stats(const spg_context *ctx, spg_stats *out);
  * B row k of the product is local row k - b_first_row when
  * b_first_row <= k < b_first_row + B_local->rows, else the halo slot holding
  * global row k (bRowForColumn, spgemm_exchange.cpp:75-87).  With halo == NULL
- * and b_first_row == 0 this is the -np 1 product and A->cols must equal
- * B_local->rows.  The product is computed on the device; *nnz_c receives
- * nnz(C).  The result stays on the context until the next multiply. */
+ * and b_first_row == 0 this is the -np 1 product.  As in the reference's
+ * spgemm(), the shapes are not compared: an A column that names no local row
+ * and no halo slot (a column >= B_local->rows at -np 1, or a negative one) is
+ * SPG_EMISSING_ROW, the reference's fail("missing a remote sparse row required
+ * by A") (spgemm_exchange.cpp:82-85).  The product is computed on the device;
+ * *nnz_c receives nnz(C).  The result stays on the context until the next
+ * multiply. */
 spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B_local,
                               int32_t b_first_row, const spg_halo_view *halo, int64_t *nnz_c);
 /* Rows [row_begin, row_end) of the same product: C's rows row_begin.. as a

@@ -193,10 +193,11 @@ spg_status spg_accumulator_add(spg_accumulator *acc, const spg_csr_view *A, cons
     op.keep_zeros = 1;
     plan.rows = (int32_t)R;
     plan.cols = acc->cols;
-    plan.small_b = bn < (int64_t(1) << 29);
-    plan.b_span = bn;
     spg_context *own = acc->own;
     own->path = ctx->path;
+    own->force_wide = ctx->force_wide;
+    plan.small_b = narrow_b(own, bn);
+    plan.b_span = bn;
     if ((s = run_product(own, plan)) != SPG_OK) return set_err(ctx, s, own->err);
     // The stage's result is the new state.
     const int64_t n = own->res_nnz;

@@ -136,6 +136,7 @@ spg_status new_batch(spg_context *ctx, spg_batch **out) {
     b->own->stream = ctx->stream;
     b->own->path = ctx->path;
     b->own->fused_rows = ctx->fused_rows;
+    b->own->force_wide = ctx->force_wide;
     b->own->timing = ctx->timing;
     *out = b;
     return SPG_OK;
@@ -157,7 +158,7 @@ void set_operands(spg_batch *b) {
     b->plan.op = op;
     b->plan.rows = (int32_t)b->total_rows;
     b->plan.cols = b->max_cols;
-    b->plan.small_b = b->b_nnz < (int64_t(1) << 29);
+    b->plan.small_b = narrow_b(b->own, b->b_nnz);
     b->plan.b_span = b->b_nnz;
 }
 

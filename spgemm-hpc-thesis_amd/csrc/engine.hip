@@ -209,6 +209,7 @@ spg_status finish_product(spg_context *ctx, const ProductPlan &p, const Control 
     stats.bins = kNumBins;
     stats.structural_nnz = p.fused ? (int64_t)(h.distinct + h.sym_total) : (int64_t)h.structural;
     stats.single_pass = p.fused ? 1 : 0;
+    stats.wide_format = p.small_b ? 0 : 1;
     const int64_t nnz = p.fused ? (int64_t)(h.reserved - h.sym_total + h.actual) : (int64_t)h.actual;
     int result = 0;
     if (nnz != layout) {
@@ -418,6 +419,13 @@ spg_status spg_set_path(spg_context *ctx, int path) {
     return SPG_OK;
 }
 
+spg_status spg_set_format(spg_context *ctx, int format) {
+    if (!ctx) return SPG_EINVAL;
+    if (format != SPG_FORMAT_AUTO && format != SPG_FORMAT_WIDE) return set_err(ctx, SPG_EINVAL, "unknown format");
+    ctx->force_wide = format == SPG_FORMAT_WIDE;
+    return SPG_OK;
+}
+
 spg_status spg_set_timing(spg_context *ctx, int enabled) {
     if (!ctx) return SPG_EINVAL;
     ctx->timing = enabled != 0;
@@ -442,13 +450,16 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
     spg_status s;
     if ((s = check_host_csr(ctx, A, "A")) != SPG_OK) return s;
     if ((s = check_host_csr(ctx, B, "B_local")) != SPG_OK) return s;
+    // The reference's spgemm() compares no shapes: A's columns are resolved
+    // row by row, and one that is neither local nor in the halo is fail()ed
+    // (reported below as SPG_EMISSING_ROW).
     const bool local_only = halo == nullptr && b_first_row == 0;
-    if (local_only && A->cols != B->rows)
-        return set_err(ctx, SPG_EINVAL, "SpGEMM requires A.cols == B.rows");
-    if (b_first_row < 0 || (int64_t)b_first_row + B->rows > (int64_t)A->cols)
-        return set_err(ctx, SPG_EINVAL, "local B block outside A's column range");
+    if (b_first_row < 0 || (int64_t)b_first_row + B->rows > INT32_MAX)
+        return set_err(ctx, SPG_EINVAL, "local B block outside the int32 row range");
     if (row_begin < 0 || row_end < row_begin || row_end > A->rows)
         return set_err(ctx, SPG_EINVAL, "row range outside A");
+    // Global B rows the directory covers: A's column range and the local block.
+    const int32_t ndir = std::max(A->cols, b_first_row + B->rows);
 
     Operands op{};
     op.arows = row_end - row_begin;
@@ -471,7 +482,7 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
         // Device-resident local block and halo: directory built on the device.
         if (B->location != SPG_DEVICE || (halo && halo->location != SPG_DEVICE))
             return set_err(ctx, SPG_EINVAL, "B_local and the halo must both be host or both device views");
-        if ((s = device_halo_operands(ctx, A, B, b_first_row, halo, op, b_span)) != SPG_OK) return s;
+        if ((s = device_halo_operands(ctx, A, B, b_first_row, halo, ndir, op, b_span)) != SPG_OK) return s;
     } else {
         // Directory over global B rows: local block first, then halo slots
         // (bRowForColumn order, spgemm_exchange.cpp:75-87).  Entries are
@@ -481,7 +492,7 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
         const int64_t hn = halo ? halo->n_rows : 0;
         const int64_t hnnz = hn ? halo->row_ptr[hn] : 0;
         b_span = B->nnz + hnnz;
-        std::vector<int64_t> ds((size_t)A->cols, -1), de((size_t)A->cols, -1);
+        std::vector<int64_t> ds((size_t)ndir, -1), de((size_t)ndir, -1);
         for (int32_t r = 0; r < B->rows; ++r) {
             ds[(size_t)b_first_row + r] = B->row_ptr[r];
             de[(size_t)b_first_row + r] = B->row_ptr[r + 1];
@@ -489,7 +500,7 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
         std::unordered_set<int32_t> seen;
         for (int64_t slot = 0; slot < hn; ++slot) {
             const int32_t g = halo->global_rows[slot];
-            if (g < 0 || g >= A->cols) return set_err(ctx, SPG_EINVAL, "halo row outside A's column range");
+            if (g < 0 || g >= ndir) return set_err(ctx, SPG_EINVAL, "halo row outside A's column range");
             if (!seen.insert(g).second) return set_err(ctx, SPG_EINVAL, "duplicate halo row");
             if (halo->row_ptr[slot + 1] < halo->row_ptr[slot] || halo->row_ptr[slot] < 0)
                 return set_err(ctx, SPG_EINVAL, "halo row_ptr is negative or not nondecreasing");
@@ -500,8 +511,8 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
         const size_t tot = (size_t)(B->nnz + hnnz);
         SPG_HIP(ctx->b_col.ensure(tot * 4, ctx->stream), "hipMalloc(B col)");
         SPG_HIP(ctx->b_val.ensure(tot * 8, ctx->stream), "hipMalloc(B val)");
-        SPG_HIP(ctx->dir_start.ensure((size_t)A->cols * 8, ctx->stream), "hipMalloc(dir)");
-        SPG_HIP(ctx->dir_end.ensure((size_t)A->cols * 8, ctx->stream), "hipMalloc(dir)");
+        SPG_HIP(ctx->dir_start.ensure((size_t)ndir * 8, ctx->stream), "hipMalloc(dir)");
+        SPG_HIP(ctx->dir_end.ensure((size_t)ndir * 8, ctx->stream), "hipMalloc(dir)");
         if (B->nnz) {
             SPG_HIP(hipMemcpyAsync(ctx->b_col.p, B->col_idx, (size_t)B->nnz * 4, hipMemcpyHostToDevice, ctx->stream), "upload B col");
             SPG_HIP(hipMemcpyAsync(ctx->b_val.p, B->values, (size_t)B->nnz * 8, hipMemcpyHostToDevice, ctx->stream), "upload B val");
@@ -512,12 +523,12 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
             SPG_HIP(hipMemcpyAsync(ctx->b_val.as<double>() + B->nnz, halo->values, (size_t)hnnz * 8,
                                    hipMemcpyHostToDevice, ctx->stream), "upload halo val");
         }
-        if (A->cols) {
+        if (ndir) {
             SPG_HIP(hipMemcpyAsync(ctx->dir_start.p, ds.data(), ds.size() * 8, hipMemcpyHostToDevice, ctx->stream), "upload dir");
             SPG_HIP(hipMemcpyAsync(ctx->dir_end.p, de.data(), de.size() * 8, hipMemcpyHostToDevice, ctx->stream), "upload dir");
         }
         SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");  // host vectors go out of scope
-        op.ndir = A->cols;
+        op.ndir = ndir;
         op.bstart = ctx->dir_start.as<int64_t>();
         op.bend = ctx->dir_end.as<int64_t>();
         op.bcol = ctx->b_col.as<int32_t>();
@@ -525,14 +536,17 @@ spg_status multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_
     }
     // Every B entry index i reachable through the directory has 8*i < 2^32:
     // the single-pass kernel then addresses B with 32-bit offsets.
-    const bool small_b = b_span < (int64_t(1) << 29);
     ProductPlan plan;
     plan.op = op;
     plan.rows = op.arows;
     plan.cols = B->cols;
-    plan.small_b = small_b;
+    plan.small_b = narrow_b(ctx, b_span);
     plan.b_span = b_span;
-    if ((s = run_product(ctx, plan)) != SPG_OK) return s;
+    s = run_product(ctx, plan);
+    // bRowForColumn (spgemm_exchange.cpp:75-87): a column outside the
+    // directory names a row that is neither local nor in the halo.
+    if (s == SPG_ERANGE) return set_err(ctx, SPG_EMISSING_ROW, "missing a remote sparse row required by A");
+    if (s != SPG_OK) return s;
     *nnz_c = ctx->res_nnz;
     return SPG_OK;
 }

@@ -83,6 +83,7 @@ struct spg_context {
     spg::DevBuf pack_cnt, pack_off, pack_map;  // packed single-pass tiles
     int path = SPG_PATH_AUTO;
     int fused_rows = spg::kFusedRows;
+    bool force_wide = false;  // SPG_FORMAT_WIDE: 64-bit entry stream and B addressing whatever B's size
     int64_t lbits_width = 0;
     // results (ping-pong for compaction)
     spg::DevBuf rowptr[2], ccol[2], cval[2];
@@ -95,14 +96,22 @@ struct spg_context {
 
 namespace spg {
 
+// Whether a product whose B rows hold b_span entries takes the 32-bit entry
+// stream and B addressing (every B entry index i then has 8*i < 2^32), unless
+// the context pins the 64-bit format.
+inline bool narrow_b(const spg_context *ctx, int64_t b_span) {
+    return !ctx->force_wide && b_span < (int64_t(1) << 29);
+}
+
 spg_status set_err(spg_context *c, spg_status s, const std::string &msg);
 spg_status hip_err(spg_context *c, hipError_t e, const char *what);
 spg_status check_host_csr(spg_context *ctx, const spg_csr_view *m, const char *name);
 // Operands of a product whose local B block and halo rows are device views:
 // entries [B_local | halo] and the global-row directory built on the device
-// (halo.hip).  b_span receives the number of B entries.
+// (halo.hip), covering global rows [0, ndir).  b_span receives the number of B
+// entries.
 spg_status device_halo_operands(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int32_t b_first_row,
-                                const spg_halo_view *halo, Operands &op, int64_t &b_span);
+                                const spg_halo_view *halo, int32_t ndir, Operands &op, int64_t &b_span);
 spg_status read_control(spg_context *ctx);
 spg_status flag_error(spg_context *ctx, unsigned long long flags);
 

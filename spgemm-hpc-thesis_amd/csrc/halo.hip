@@ -117,7 +117,7 @@ spg_status read_flags(spg_context *ctx, unsigned &flags) {
 }  // namespace
 
 spg_status device_halo_operands(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int32_t b_first_row,
-                                const spg_halo_view *halo, Operands &op, int64_t &b_span) {
+                                const spg_halo_view *halo, int32_t ndir, Operands &op, int64_t &b_span) {
     hipStream_t st = ctx->stream;
     const int32_t hn = halo ? halo->n_rows : 0;
     if (hn < 0 || (hn > 0 && (!halo->global_rows || !halo->row_ptr)))
@@ -134,8 +134,8 @@ spg_status device_halo_operands(spg_context *ctx, const spg_csr_view *A, const s
     const size_t tot = (size_t)b_span;
     SPG_HIP(ctx->b_col.ensure(tot * 4, st), "hipMalloc(B col)");
     SPG_HIP(ctx->b_val.ensure(tot * 8, st), "hipMalloc(B val)");
-    SPG_HIP(ctx->dir_start.ensure((size_t)A->cols * 8, st), "hipMalloc(dir)");
-    SPG_HIP(ctx->dir_end.ensure((size_t)A->cols * 8, st), "hipMalloc(dir)");
+    SPG_HIP(ctx->dir_start.ensure((size_t)ndir * 8, st), "hipMalloc(dir)");
+    SPG_HIP(ctx->dir_end.ensure((size_t)ndir * 8, st), "hipMalloc(dir)");
     SPG_HIP(ctx->halo_err.ensure(sizeof(unsigned), st), "hipMalloc(halo flags)");
     // Entries [B_local | halo], device to device.
     if (B->nnz) {
@@ -151,16 +151,16 @@ spg_status device_halo_operands(spg_context *ctx, const spg_csr_view *A, const s
     int64_t *ds = ctx->dir_start.as<int64_t>(), *de = ctx->dir_end.as<int64_t>();
     unsigned *err = ctx->halo_err.as<unsigned>();
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, err);
-    if (A->cols > 0) {
-        hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(A->cols, 1024)), dim3(256), 0, st, ds, (int64_t)A->cols, (int64_t)-1);
-        hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(A->cols, 1024)), dim3(256), 0, st, de, (int64_t)A->cols, (int64_t)-1);
+    if (ndir > 0) {
+        hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ndir, 1024)), dim3(256), 0, st, ds, (int64_t)ndir, (int64_t)-1);
+        hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ndir, 1024)), dim3(256), 0, st, de, (int64_t)ndir, (int64_t)-1);
     }
     if (B->rows > 0)
         hipLaunchKernelGGL(k_dir_local, dim3(grid_for(B->rows, 256)), dim3(256), 0, st, B->row_ptr, B->rows, b_first_row,
                            ds, de);
     if (hn > 0)
         hipLaunchKernelGGL(k_dir_halo, dim3(grid_for(hn, 256)), dim3(256), 0, st, halo->global_rows, halo->row_ptr, hn,
-                           b_first_row, B->rows, A->cols, B->nnz, ds, de, err);
+                           b_first_row, B->rows, ndir, B->nnz, ds, de, err);
     SPG_HIP(hipGetLastError(), "kernel launch");
     unsigned flags = 0;
     spg_status s = read_flags(ctx, flags);
@@ -168,7 +168,7 @@ spg_status device_halo_operands(spg_context *ctx, const spg_csr_view *A, const s
     if (flags & kHaloRange) return set_err(ctx, SPG_EINVAL, "halo row outside A's column range");
     if (flags & kHaloDuplicate) return set_err(ctx, SPG_EINVAL, "duplicate halo row");
     if (flags & kHaloBadPtr) return set_err(ctx, SPG_EINVAL, "halo row_ptr is negative or not nondecreasing");
-    op.ndir = A->cols;
+    op.ndir = ndir;
     op.bstart = ds;
     op.bend = de;
     op.bcol = ctx->b_col.as<int32_t>();

@@ -62,12 +62,18 @@ int main(int argc, char **argv) {
                     (long long)c.rowPtr.back(), (long long)st.products);
 
         bool ok = true;
-        // A.cols != B.rows without a halo: std::invalid_argument
-        // (spgemm_common.cpp:106-110).
-        CsrMatrix bad = b;
-        bad.rows += 1;
-        bad.rowPtr.push_back(bad.rowPtr.back());
-        ok &= throws<std::invalid_argument>([&] { spgemm(engine, a, RowBlock{0, bad.rows}, bad); }, "shape");
+        // spgemm() compares no shapes (spgemm_exchange.cpp:258-325): B with an
+        // extra empty row (A.cols != B.rows) gives the same C.
+        CsrMatrix wider = b;
+        wider.rows += 1;
+        wider.rowPtr.push_back(wider.rowPtr.back());
+        {
+            const CsrMatrix cw = spgemm(engine, a, RowBlock{0, wider.rows}, wider);
+            const bool same = cw.rowPtr == c.rowPtr && cw.columnIndices == c.columnIndices &&
+                              std::memcmp(cw.values.data(), c.values.data(), c.values.size() * 8) == 0;
+            if (!same) std::fprintf(stderr, "product with an extra empty B row differs\n");
+            ok &= same;
+        }
         // Missing halo row: MissingRowError (fail() in spgemm_exchange.cpp:82-85).
         if (a.rowPtr.back() > 0 && b.rows > 1) {
             CsrMatrix top;  // B without its last row: that row must come from a halo
@@ -82,6 +88,9 @@ int main(int argc, char **argv) {
                 HaloRows empty;
                 ok &= throws<MissingRowError>([&] { spgemm(engine, a, RowBlock{0, top.rows}, top, &empty); },
                                               "missing row");
+                // Without a halo at all (-np 1 with A.cols > B.rows): the same fail().
+                ok &= throws<MissingRowError>([&] { spgemm(engine, a, RowBlock{0, top.rows}, top); },
+                                              "missing row without halo");
                 HaloRows halo;  // the same row supplied as a halo slot gives the same C
                 halo.globalRows.push_back(b.rows - 1);
                 const int64_t s = b.rowPtr[b.rows - 1], e = b.rowPtr[b.rows];

@@ -7,11 +7,12 @@
 //   spgemm(...)          src/baselines/spgemm_exchange.hpp:37-39 (body spgemm_exchange.cpp:258-325)
 //   LocalAccumulator     src/trident/common/matrix_blocks.hpp:23-32 (body matrix_blocks.cpp:152-200)
 //   readMatrixMarket     src/common/matrix_market.cpp:28-144
-// Error behaviour follows the reference: std::invalid_argument for mismatched
-// shapes, std::overflow_error for count overflow, std::runtime_error for a
-// column outside B; a B row that is neither local nor in the halo raises
-// MissingRowError, which a caller holding an MPI communicator turns into the
-// reference's fail() (spgemm_exchange.cpp:82-85).
+// Error behaviour follows the reference: spgemm() compares no shapes; an A
+// column that names neither a local B row nor a halo slot (at -np 1: a column
+// outside B's rows) raises MissingRowError, which a caller holding an MPI
+// communicator turns into the reference's fail() (spgemm_exchange.cpp:82-85).
+// std::invalid_argument reports malformed operands and the accumulator's
+// mismatched shapes, std::overflow_error count overflow.
 //
 // Header-only; link with -lspgemm_amd.
 #pragma once

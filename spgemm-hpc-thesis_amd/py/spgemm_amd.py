@@ -32,7 +32,7 @@ EXPORTED = [
     "spg_accumulator_create", "spg_accumulator_add", "spg_accumulator_finish", "spg_accumulator_copy",
     "spg_accumulator_destroy", "spg_matrix_read_mm", "spg_matrix_view", "spg_matrix_free",
     "spg_matrix_last_error", "spg_matrix_upload", "spg_device_matrix_view", "spg_device_matrix_free",
-    "spg_synchronize", "spg_set_path", "spg_multiply_rows", "spg_result_rows",
+    "spg_synchronize", "spg_set_path", "spg_set_format", "spg_multiply_rows", "spg_result_rows",
     "spg_batch_create", "spg_batch_create_chained", "spg_batch_refresh", "spg_batch_run", "spg_batch_set_graph",
     "spg_batch_autotune", "spg_batch_product_nnz", "spg_batch_copy", "spg_batch_info", "spg_batch_last_stats",
     "spg_batch_destroy", "spg_batch_rebuild", "spg_batch_set_stream", "spg_batch_last_error", "spg_context_stream", "spg_context_bind",
@@ -47,6 +47,7 @@ EXPORTED = [
 ]
 
 PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE, PATH_SINGLE_PASS_DEALT, PATH_SINGLE_PASS_PACKED = 0, 1, 2, 3, 4
+FORMAT_AUTO, FORMAT_WIDE = 0, 1
 
 
 @dataclass
@@ -81,13 +82,13 @@ class HaloView(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("products", C.c_int64), ("structural_nnz", C.c_int64), ("nnz", C.c_int64),
                 ("rows_per_bin", C.c_int64 * 8), ("compacted", C.c_int32), ("bins", C.c_int32),
-                ("single_pass", C.c_int32), ("reserved_", C.c_int32),
+                ("single_pass", C.c_int32), ("wide_format", C.c_int32),
                 ("ms_total", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double)]
 
     def as_dict(self) -> dict:
         return {"products": self.products, "structural_nnz": self.structural_nnz, "nnz": self.nnz,
                 "rows_per_bin": list(self.rows_per_bin)[: self.bins], "compacted": bool(self.compacted),
-                "single_pass": bool(self.single_pass),
+                "single_pass": bool(self.single_pass), "wide_format": bool(self.wide_format),
                 "ms_total": self.ms_total, "ms_symbolic": self.ms_symbolic, "ms_numeric": self.ms_numeric}
 
 
@@ -154,6 +155,7 @@ def lib():
         L.spg_device_matrix_free.argtypes = [vp]
         L.spg_synchronize.argtypes = [vp]
         L.spg_set_path.argtypes = [vp, C.c_int]
+        L.spg_set_format.argtypes = [vp, C.c_int]
         L.spg_result_rows.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, C.c_int64, P(C.c_int64)]
         L.spg_batch_create.argtypes = [vp, C.c_int32, P(CsrView), P(CsrView), P(vp)]
         L.spg_batch_create_chained.argtypes = [vp, vp, C.c_int32, P(C.c_int32), P(CsrView), P(vp)]
@@ -297,6 +299,7 @@ class Engine:
         self._h = h
         self._L = L
         self._path = PATH_AUTO
+        self._format = FORMAT_AUTO
         if timing:
             L.spg_set_timing(h, 1)
 
@@ -331,6 +334,12 @@ class Engine:
         """PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE or PATH_SINGLE_PASS_DEALT (results are identical)."""
         self._check(self._L.spg_set_path(self._h, int(path)))
         self._path = int(path)
+
+    def set_format(self, fmt: int):
+        """FORMAT_AUTO or FORMAT_WIDE (64-bit entry stream and B addressing
+        whatever B's size; results are identical)."""
+        self._check(self._L.spg_set_format(self._h, int(fmt)))
+        self._format = int(fmt)
 
     def stats(self) -> dict:
         st = Stats()

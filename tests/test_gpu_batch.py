@@ -120,3 +120,51 @@ def test_batch_errors(engine):
         batch.run()
     assert e.value.name == "SPG_ERANGE"
     batch.close()
+
+
+def big_bin_jobs(seed=23):
+    """Plain products whose rows reach bins W4 (<= 1024 products), Block
+    (<= 2048) and Large: 24..60 uniform columns per row in A and B, plus a few
+    of the small jobs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for rows in (300, 1100, 2500, 700):
+        out.append((W.variable_uniform(rows, rows, 24, 60, rng), W.variable_uniform(rows, rows, 24, 60, rng)))
+    out += [(j[1], j[2]) for j in small_jobs(9, seed=seed) if j[0] == "ab"]
+    return out
+
+
+@pytest.mark.parametrize("path,fmt", [(S.PATH_AUTO, S.FORMAT_WIDE), (S.PATH_SINGLE_PASS_PACKED, S.FORMAT_WIDE),
+                                      (S.PATH_TWO_PHASE, S.FORMAT_AUTO), (S.PATH_SINGLE_PASS_DEALT, S.FORMAT_WIDE)],
+                         ids=["auto-wide", "packed-wide", "two_phase-narrow", "dealt-wide"])
+def test_batch_big_bins_and_wide_format(engine, path, fmt):
+    # A batch whose rows fill bins W4, Block and Large, in the 64-bit format
+    # of the big batches (B arenas of 2^29 entries or more) and the 32-bit
+    # one, eager and replayed from a graph: every C_i equals its standalone
+    # product on the default path, and every third the oracle.
+    pairs = big_bin_jobs()
+    a1, b1 = [a for a, _ in pairs], [b for _, b in pairs]
+    want = [digest(engine.multiply(a, b)) for a, b in zip(a1, b1)]
+    for i in range(0, len(pairs), 3):
+        a, b = pairs[i]
+        ref = O.product(O.Csr.make(a.rows, a.cols, a.row_ptr, a.col, a.val),
+                        O.Csr.make(b.rows, b.cols, b.row_ptr, b.col, b.val))
+        assert O.csr_digest(ref) == want[i], i
+    engine.set_path(path)
+    engine.set_format(fmt)
+    try:
+        batch = S.Batch(engine, a1, b1)
+        batch.run()
+        st = batch.stats()
+        assert all(st["rows_per_bin"][k] > 0 for k in (4, 5, 6)), st["rows_per_bin"]
+        assert st["wide_format"] == (fmt == S.FORMAT_WIDE)
+        assert [digest(batch.result(i)) for i in range(len(a1))] == want
+        batch.set_graph(True)
+        for rep in range(2):
+            print("graph run", rep, flush=True)
+            batch.run()
+            assert [digest(batch.result(i)) for i in range(len(a1))] == want
+        batch.close()
+    finally:
+        engine.set_path(S.PATH_AUTO)
+        engine.set_format(S.FORMAT_AUTO)

@@ -16,20 +16,9 @@ import cases
 import oracle as O
 import spgemm_amd as S
 import workloads as W
+from parity_util import check_sampled_rows, digest, sorted_rows, to_o, to_s
 
 pytestmark = pytest.mark.gpu
-
-
-def to_s(m) -> S.Csr:
-    return S.Csr.make(m.rows, m.cols, m.row_ptr, m.col, m.val)
-
-
-def to_o(m) -> O.Csr:
-    return O.Csr.make(m.rows, m.cols, m.row_ptr, m.col, m.val)
-
-
-def digest(m) -> str:
-    return O.csr_digest(to_o(m))
 
 
 def all_cases():
@@ -41,14 +30,30 @@ def all_cases():
 
 PATHS = [("auto", S.PATH_AUTO), ("single_pass", S.PATH_SINGLE_PASS), ("two_phase", S.PATH_TWO_PHASE),
          ("single_pass_dealt", S.PATH_SINGLE_PASS_DEALT), ("single_pass_packed", S.PATH_SINGLE_PASS_PACKED)]
+# Every pipeline in both encodings of the entry stream: the 32-bit one that
+# small B operands take, and the 64-bit one of B with 2^29 entries or more
+# (big products and batches), forced here on small inputs.
+FORMATS = [("narrow", S.FORMAT_AUTO), ("wide", S.FORMAT_WIDE)]
+PATH_FORMATS = [(f"{p}-{f}", pv, fv) for p, pv in PATHS for f, fv in FORMATS]
 
 
-@pytest.fixture(params=PATHS, ids=[p[0] for p in PATHS])
-def path_engine(request, engine):
-    """The session engine pinned to one pipeline (results must not depend on it)."""
-    engine.set_path(request.param[1])
-    yield engine
+def pin(engine, path, fmt):
+    engine.set_path(path)
+    engine.set_format(fmt)
+
+
+def unpin(engine):
     engine.set_path(S.PATH_AUTO)
+    engine.set_format(S.FORMAT_AUTO)
+
+
+@pytest.fixture(params=PATH_FORMATS, ids=[p[0] for p in PATH_FORMATS])
+def path_engine(request, engine):
+    """The session engine pinned to one pipeline and one entry-stream format
+    (results must not depend on either)."""
+    pin(engine, request.param[1], request.param[2])
+    yield engine
+    unpin(engine)
 
 
 def test_golden_products_bit_exact(path_engine, golden):
@@ -61,6 +66,7 @@ def test_golden_products_bit_exact(path_engine, golden):
         assert st["products"] == g["products"], name
         assert c.nnz == g["c_nnz"], name
         assert digest(c) == g["digest"], name
+        assert st["wide_format"] == (engine._format == S.FORMAT_WIDE), name
         if a.nnz:
             # small products take the single pass unless two-phase is pinned
             assert st["single_pass"] == (engine._path != S.PATH_TWO_PHASE), name
@@ -91,11 +97,11 @@ EDGE_SHAPES = [
 ]
 
 
-@pytest.mark.parametrize("path", [p[1] for p in PATHS], ids=[p[0] for p in PATHS])
-def test_single_pass_tile_edges(engine, path):
+@pytest.mark.parametrize("path,fmt", [p[1:] for p in PATH_FORMATS], ids=[p[0] for p in PATH_FORMATS])
+def test_single_pass_tile_edges(engine, path, fmt):
     # Block and tile boundaries of every single-pass tiling (16-row tiles,
     # dealt 32-row tiles, packed tiles per 256-row block) against the oracle.
-    engine.set_path(path)
+    pin(engine, path, fmt)
     try:
         for i, (name, rows, pat) in enumerate(EDGE_SHAPES):
             print("case", name, flush=True)
@@ -103,7 +109,7 @@ def test_single_pass_tile_edges(engine, path):
             c = engine.multiply(to_s(a), to_s(b))
             assert digest(c) == O.csr_digest(O.product(a, b)), name
     finally:
-        engine.set_path(S.PATH_AUTO)
+        unpin(engine)
 
 
 
@@ -150,20 +156,21 @@ def bin_edge_case(seed):
     return a, b, want
 
 
-@pytest.mark.parametrize("path", [p[1] for p in PATHS], ids=[p[0] for p in PATHS])
-def test_bin_edges(engine, path):
-    # Every bin boundary (by products and by A entries) on every pipeline,
-    # bit for bit against the oracle.
-    engine.set_path(path)
+@pytest.mark.parametrize("path,fmt", [p[1:] for p in PATH_FORMATS], ids=[p[0] for p in PATH_FORMATS])
+def test_bin_edges(engine, path, fmt):
+    # Every bin boundary (by products and by A entries) on every pipeline and
+    # entry-stream format, bit for bit against the oracle.
+    pin(engine, path, fmt)
     try:
         a, b, want = bin_edge_case(77)
         c = engine.multiply(to_s(a), to_s(b))
         st = engine.stats()
         assert list(st["rows_per_bin"]) == list(want)
         assert all(x > 0 for x in want[1:])  # every bin is exercised
+        assert st["wide_format"] == (fmt == S.FORMAT_WIDE)
         assert digest(c) == O.csr_digest(O.product(a, b))
     finally:
-        engine.set_path(S.PATH_AUTO)
+        unpin(engine)
 
 def test_validators_accept_engine_output(engine):
     for name, a, b in list(cases.mm_cases()) + cases.synthetic_cases():
@@ -236,15 +243,26 @@ def test_empty_and_degenerate_shapes(engine):
 
 
 def test_errors(engine):
+    # The reference's spgemm() compares no shapes (only serialSpGEMM does,
+    # spgemm_common.cpp:106-110): A.cols != B.rows multiplies when every A
+    # column names a row of B ...
     a = csr(2, 3, [0, 1, 2], [0, 2], [1.0, 1.0])
     b = csr(4, 2, [0, 1, 1, 1, 2], [0, 1], [1.0, 1.0])
-    with pytest.raises(S.SpgError) as e:
-        engine.multiply(a, b)
-    assert e.value.name == "SPG_EINVAL"
-    bad = csr(1, 3, [0, 1], [7], [1.0])  # column outside B's rows
-    with pytest.raises(S.SpgError) as e:
-        engine.multiply(bad, csr(3, 2, [0, 1, 1, 2], [0, 1], [1.0, 1.0]))
-    assert e.value.name == "SPG_ERANGE"
+    c = engine.multiply(a, b)
+    assert c.row_ptr.tolist() == [0, 1, 1] and c.col.tolist() == [0] and c.val.tolist() == [1.0]
+    a5 = csr(1, 5, [0, 2], [1, 3], [2.0, 3.0])
+    c = engine.multiply(a5, b)
+    assert c.row_ptr.tolist() == [0, 1] and c.col.tolist() == [1] and c.val.tolist() == [3.0]
+    # ... and a column that names no local row (and no halo slot) is the
+    # reference's fail("missing a remote sparse row required by A")
+    # (bRowForColumn, spgemm_exchange.cpp:75-87), at either end of the range
+    for col in (7, 3, -1):
+        bad = csr(1, 8, [0, 2], [0, col], [1.0, 1.0])
+        with pytest.raises(S.SpgError) as e:
+            engine.multiply(bad, csr(3, 2, [0, 1, 1, 2], [0, 1], [1.0, 1.0]))
+        assert e.value.name == "SPG_EMISSING_ROW", col
+        assert "missing a remote sparse row required by A" in str(e.value)
+    # a malformed CSR view
     with pytest.raises(S.SpgError) as e:
         engine.multiply(csr(1, 1, [0, 2], [0], [1.0]), csr(1, 1, [0, 1], [0], [1.0]))
     assert e.value.name == "SPG_EINVAL"
@@ -321,36 +339,6 @@ def test_uniform_16384_x8_full(engine):
     assert rep["passed"] and rep["max_abs_error"] == 0.0
 
 
-def sorted_rows(c) -> bool:
-    d = np.diff(c.col.astype(np.int64))
-    starts = c.row_ptr[1:-1]
-    ok = d > 0
-    ok[starts[(starts > 0) & (starts < c.nnz)] - 1] = True
-    return bool(np.all(ok))
-
-
-def sample_rows(a, rows):
-    rp = [0]
-    cols, vals = [], []
-    for r in rows:
-        s, e = a.row_ptr[r], a.row_ptr[r + 1]
-        cols.append(a.col[s:e])
-        vals.append(a.val[s:e])
-        rp.append(rp[-1] + int(e - s))
-    return O.Csr.make(len(rows), a.cols, rp, np.concatenate(cols), np.concatenate(vals))
-
-
-def check_sampled_rows(a, b, c, nsample, seed):
-    rng = np.random.Generator(np.random.PCG64(seed))
-    rows = np.sort(rng.choice(a.rows, size=min(nsample, a.rows), replace=False))
-    sub = O.product(sample_rows(to_o(a), rows), to_o(b))
-    for i, r in enumerate(rows):
-        s, e = c.row_ptr[r], c.row_ptr[r + 1]
-        ss, se = sub.row_ptr[i], sub.row_ptr[i + 1]
-        assert np.array_equal(c.col[s:e], sub.col[ss:se]), r
-        assert np.array_equal(c.val[s:e].view(np.uint64), sub.val[ss:se].view(np.uint64)), r
-
-
 def test_uniform_2p20_x16_properties(engine):
     # 2^20 x 2^20, 16 uniform columns per row: properties at full size plus
     # bit-exact comparison with the oracle on 4096 sampled rows.
@@ -424,7 +412,7 @@ def test_heavy_row_tiles_in_analyze(engine):
     bad[int(rp[64]) - 3] = 3000 + 11
     with pytest.raises(S.SpgError) as e:
         engine.multiply(S.Csr.make(300, 3000, rp, bad, a.val), to_s(b))
-    assert e.value.name == "SPG_ERANGE"
+    assert e.value.name == "SPG_EMISSING_ROW"
 
 
 def test_large_rows_heaviest_first_order(engine):
