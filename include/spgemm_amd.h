@@ -179,6 +179,24 @@ spg_status spg_multiply_begin(spg_context *ctx, const spg_csr_view *A, const spg
 spg_status spg_multiply_rows(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B_local,
                              int32_t b_first_row, const spg_halo_view *halo, int32_t row_begin, int32_t row_end,
                              int64_t *nnz_c);
+/* The product of device views A and B (the -np 1 product of
+ * spg_multiply_begin: b_first_row 0, no halo) without any device memory
+ * allocation.  No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  C goes
+ * straight into caller-owned device buffers: row_ptr (A->rows + 1 entries),
+ * col_idx and values (`capacity` entries each); *nnz_c receives nnz(C).  The
+ * small rows always take the single-pass kernel (with the context's tiling),
+ * and results are bit-identical to spg_multiply_begin's.  Every intermediate
+ * array lives in a fixed device array of 512 MiB (2^29 bytes) that each GPU
+ * the library runs on holds until the process ends; a product whose arrays
+ * need more (about 300 MB for 2^20 rows with 2^24 A entries, plus 12 bytes
+ * per entry of C's rows of more than 256 products) is SPG_ENOMEM.  So is an
+ * output capacity below C's symbolic bound (the small rows' product counts
+ * plus the other rows' exact counts), which *nnz_c then receives.  All calls
+ * on one GPU share that array: calls that overlap on one GPU race.  The
+ * context's last result (spg_multiply_finish) is not set. */
+spg_status spg_multiply_noalloc(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int64_t *row_ptr,
+                                int32_t *col_idx, double *values, int64_t capacity, int64_t *nnz_c);
 /* Copy the result of the last multiply into caller-owned host buffers of
  * sizes A->rows+1, nnz_c and nnz_c (the shim resizes its std::vectors first). */
 spg_status spg_multiply_finish(spg_context *ctx, int64_t *row_ptr, int32_t *col_idx, double *values);

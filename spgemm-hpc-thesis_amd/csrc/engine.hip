@@ -361,6 +361,7 @@ spg_status spg_create(spg_context **out, int device) {
     hipDeviceProp_t prop{};
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
     if (e == hipSuccess) e = prepare_kernels(prop.sharedMemPerBlock);
+    if (e == hipSuccess) noalloc_setup(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_ctrl, sizeof(Control), hipHostMallocDefault);
     for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&ctx->ev[i]);

@@ -113,6 +113,9 @@ spg_status check_host_csr(spg_context *ctx, const spg_csr_view *m, const char *n
 spg_status device_halo_operands(spg_context *ctx, const spg_csr_view *A, const spg_csr_view *B, int32_t b_first_row,
                                 const spg_halo_view *halo, int32_t ndir, Operands &op, int64_t &b_span);
 spg_status read_control(spg_context *ctx);
+// Device address of the allocation-free entry point's scratch array on
+// `device` (the current device), looked up once per device (noalloc.hip).
+void noalloc_setup(int device);
 spg_status flag_error(spg_context *ctx, unsigned long long flags);
 
 #define SPG_HIP(call, what)                                       \

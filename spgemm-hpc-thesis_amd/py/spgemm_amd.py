@@ -33,6 +33,7 @@ EXPORTED = [
     "spg_accumulator_destroy", "spg_matrix_read_mm", "spg_matrix_view", "spg_matrix_free",
     "spg_matrix_last_error", "spg_matrix_upload", "spg_device_matrix_view", "spg_device_matrix_free",
     "spg_synchronize", "spg_set_path", "spg_set_format", "spg_multiply_rows", "spg_result_rows",
+    "spg_multiply_noalloc",
     "spg_batch_create", "spg_batch_create_chained", "spg_batch_refresh", "spg_batch_run", "spg_batch_set_graph",
     "spg_batch_autotune", "spg_batch_product_nnz", "spg_batch_copy", "spg_batch_info", "spg_batch_last_stats",
     "spg_batch_destroy", "spg_batch_rebuild", "spg_batch_set_stream", "spg_batch_last_error", "spg_context_stream", "spg_context_bind",
@@ -156,6 +157,7 @@ def lib():
         L.spg_synchronize.argtypes = [vp]
         L.spg_set_path.argtypes = [vp, C.c_int]
         L.spg_set_format.argtypes = [vp, C.c_int]
+        L.spg_multiply_noalloc.argtypes = [vp, P(CsrView), P(CsrView), vp, vp, vp, C.c_int64, P(C.c_int64)]
         L.spg_result_rows.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, C.c_int64, P(C.c_int64)]
         L.spg_batch_create.argtypes = [vp, C.c_int32, P(CsrView), P(CsrView), P(vp)]
         L.spg_batch_create_chained.argtypes = [vp, vp, C.c_int32, P(C.c_int32), P(CsrView), P(vp)]
@@ -404,6 +406,20 @@ class Engine:
         self._check(self._L.spg_multiply_rows(self._h, C.byref(av), C.byref(bv), 0, None, int(rows[0]),
                                               int(rows[1]), C.byref(n)))
         return int(n.value)
+
+    def multiply_noalloc(self, a: DeviceMatrix, b: DeviceMatrix, row_ptr: int, col: int, val: int,
+                         capacity: int) -> tuple[int, int]:
+        """spg_multiply_noalloc into caller-owned device buffers (addresses);
+        returns (status, nnz) without raising, so that the capacity query
+        (SPG_ENOMEM with the symbolic bound) can be read."""
+        n = C.c_int64()
+        av, bv = a.view(), b.view()
+        s = self._L.spg_multiply_noalloc(self._h, C.byref(av), C.byref(bv), row_ptr, col, val, int(capacity),
+                                         C.byref(n))
+        return int(s), int(n.value)
+
+    def last_error(self) -> str:
+        return self._L.spg_last_error(self._h).decode(errors="replace")
 
     def result_view(self) -> CsrView:
         v = CsrView()
