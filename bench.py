@@ -1,45 +1,54 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X SpGEMM engine (one JSON line on stdout).
 
-Default workload: one rank's share of the large uniform product --
-A and B 1,048,576 x 1,048,576 with 16 uniform random columns per row, fp64
-values U[-1,1) (seeds 42 and 43); C = A.B through the plain drop-in path with
-operands already resident in HBM.  A step is one complete product (analyze,
-single-pass kernel or symbolic + numeric phases; output left in HBM).
+Default run: the single product of one rank's share -- A and B 1,048,576 x
+1,048,576 with 16 uniform random columns per row, fp64 values U[-1,1) (seeds
+42 and 43); C = A.B through the plain drop-in path with operands already
+resident in HBM -- as the headline `value`, followed by two sections of the
+same line measured in the same run:
+  config.batched  4096 independent jobs (rows log-uniform in [256, 65536], 4..32
+                  uniform columns per row, every third a Galerkin triple R.A.P on
+                  a 27-point stencil with 2x2x2 aggregation) as one batched
+                  pipeline replayed from a HIP graph, against sequential plain
+                  launches of the same jobs
+  config.served   a Poisson request stream (2000/s for 3 s) of the same jobs'
+                  products plus 5% products of the 2^20 x16 shape over registered,
+                  device-resident operands, served with continuous batching under
+                  a 200 GB budget, against one product per launch; p50/p99 latency
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                    [--no-sections] [--no-cpu-baseline]
 
-Workloads (--workload):
-  uniform_2p20x16   2^20 x 2^20, 16 uniform columns per row, A.B (default)
-  uniform_16384x8   16384 x 16384, 8 per row, A.B
-  rmat_s20_AA       R-MAT scale 20, edge factor 32, (.57,.19,.19,.05), A.A; the
-                    output (~2.3e10 entries) exceeds device memory, so a step is
-                    one sweep over row chunks (each chunk's C overwrites the last)
-  rmat_s20_rect     the same R-MAT A times B 2^20 x 65536 with 24 uniform columns per row
-  served_poisson    a Poisson request stream (fixed rate) of products over registered,
-                    device-resident operands: c3-style products with 5% products of the
-                    2^20 x16 shape, served with continuous batching under a 200 GB budget
-                    and, as the baseline, one product per launch; p50/p99 latency
-  batched_4096      4096 independent jobs (rows log-uniform in [256, 65536], 4..32 uniform
-                    columns per row; every third job a Galerkin triple R.A.P on a 27-point
-                    stencil with 2x2x2 aggregation) as one batched pipeline replayed from a
-                    HIP graph, against sequential plain launches of the same jobs
+Timing follows the reference's protocol (src/baselines/two_sided_spgemm.cpp:
+126-192): W untimed warmup products, then K timed products in trials of 10
+repeats, each product timed on its own; the reported time is the nearest-rank
+P90 of the samples (percentile90, src/common/spgemm_common.cpp:916-924), and
+value = 2 P / P90 with P the scalar products (scalarMultiplicationCount).  The
+mean (ms_per_step) and the median are reported beside it.  Defaults: warmup 2,
+50 steps (5 trials x 10 repeats), the reference CLI's defaults.
 
-value     = 2*P / step time (GFLOP/s), P = scalar products (the reference's flop basis).
+Other workloads (--workload, printed as their own line): uniform_16384x8;
+rmat_s20_AA (R-MAT scale 20, edge factor 32, (.57,.19,.19,.05), A.A, whose
+~2.3e10-entry output exceeds device memory: a step is one sweep over row chunks);
+rmat_s20_rect (the same R-MAT times B 2^20 x 65536, 24 per row); batched_4096;
+served_poisson.
+
 roofline  = HBM-bound: algorithmic bytes of the dominant kernel / its average
-            duration, measured with HIP events on the engine's stream around
-            that launch.  For the single-pass kernel k_fused: 32 B per row +
-            16 B per A entry (B row start and length, 4 B each when B has fewer
-            than 2^29 entries, else 12 B; A value 8 B) + 12 B per scalar
-            product + 12 B per C entry.
+            duration from HIP events on the engine's stream.  For the single-pass
+            kernel k_fused: 32 B per row + 16 B per A entry (B row start and
+            length, 4 B each when B has fewer than 2^29 entries, else 12 B; A
+            value 8 B) + 12 B per scalar product + 12 B per C entry.  `traffic`
+            is the calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE per launch
+            recorded in profiles/traffic_<workload>.md.
 cpu_baseline = the reference's own spgemm() (oracle/_ref/refdrv, -np 1, OpenMP
-            guided/64) on a bounded row sample, or the C oracle when that binary
-            is absent.
+            guided/64, every host core) under the same protocol on a bounded row
+            sample; the C oracle when that binary is absent.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -53,6 +62,8 @@ sys.path.insert(0, os.path.join(ROOT, "spgemm-hpc-thesis_amd", "py"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+REPEATS = 10          # repeats per trial (the reference CLI's default)
+CPU_WARMUP, CPU_REPEATS, CPU_TRIALS = 2, 10, 5
 CHUNK_PRODUCTS = 3 << 30  # scalar products per row chunk of a sweep (bounds C per chunk)
 
 
@@ -78,17 +89,85 @@ WORKLOADS = {
     "rmat_s20_AA": dict(kind="chunked", desc={"rmat_scale": 20, "edge_factor": 32, "abcd": [0.57, 0.19, 0.19, 0.05],
                                               "seed": 42, "product": "A.A"},
                         make=lambda: (lambda a: (a, a))(_rmat())),
-    "served_poisson": dict(kind="served", desc={"rate_per_s": 2000, "duration_s": 3.0, "big_fraction": 0.05,
-                                                "big_shape": "2^20 x 2^20, 16/row", "pool_jobs": 384, "seed": 42,
-                                                "budget_bytes": 200 << 30},
-                           make=None),
-    "batched_4096": dict(kind="batched", desc={"jobs": 4096, "seed": 42, "rows": "log-uniform [256, 65536]",
-                                               "nnz_per_row": "uniform 4..32", "galerkin_fraction": "1/3 (n in 16..48)"},
-                         make=None),
     "rmat_s20_rect": dict(kind="single", desc={"rmat_scale": 20, "edge_factor": 32, "seed": 42,
                                                "B": "2^20 x 65536, 24 uniform columns per row, seed 43"},
                           make=lambda: (_rmat(), _uniform(1 << 20, 65536, 24, 43))),
+    "batched_4096": dict(kind="batched", desc={"jobs": 4096, "seed": 42, "rows": "log-uniform [256, 65536]",
+                                               "nnz_per_row": "uniform 4..32", "galerkin_fraction": "1/3 (n in 16..48)"}),
+    "served_poisson": dict(kind="served", desc={"rate_per_s": 2000, "duration_s": 3.0, "big_fraction": 0.05,
+                                                "big_shape": "2^20 x 2^20, 16/row", "pool_jobs": 384, "seed": 42,
+                                                "budget_bytes": 200 << 30}),
 }
+
+
+# ---- timing protocol -------------------------------------------------------
+
+def p90_nearest_rank(samples):
+    """Nearest-rank 90th percentile: the ceil(0.9 n)-th smallest sample
+    (percentile90, src/common/spgemm_common.cpp:916-924)."""
+    s = sorted(samples)
+    return s[max(1, math.ceil(0.90 * len(s))) - 1]
+
+
+def run_protocol(step, warmup, steps):
+    """warmup untimed steps, then `steps` steps each timed on its own, in
+    trials of REPEATS (two_sided_spgemm.cpp:126-167).  step() returns after
+    its work is complete on the device; returns the samples in seconds."""
+    for _ in range(warmup):
+        step()
+    samples = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        step()
+        samples.append(time.perf_counter() - t0)
+    return samples
+
+
+def timing_fields(samples):
+    n = len(samples)
+    return {"samples": n, "trials": math.ceil(n / REPEATS), "repeats": min(REPEATS, n),
+            "p90_ms": round(p90_nearest_rank(samples) * 1e3, 4),
+            "p50_ms": round(statistics.median(samples) * 1e3, 4),
+            "mean_ms": round(statistics.mean(samples) * 1e3, 4)}
+
+
+def host_cores():
+    """(logical CPUs this process may run on, physical cores of the host)."""
+    try:
+        logical = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        logical = os.cpu_count() or 1
+    cores = set()
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("physical id"):
+                    phys = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    core = line.split(":")[1].strip()
+                elif not line.strip() and core is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return logical, (len(cores) or logical)
+
+
+def row_structure(row_ptr):
+    """Row-count statistics of a CSR (population min/max/mean/stddev/cv and
+    empty rows: appendRowMetrics, src/common/benchmark_metrics.cpp:21-47)."""
+    n = np.diff(np.asarray(row_ptr, dtype=np.int64))
+    if len(n) == 0:
+        return {"row_nnz_min": 0, "row_nnz_max": 0, "mean_nnz_per_row": 0.0, "row_nnz_stddev": 0.0,
+                "row_nnz_cv": 0.0, "empty_rows": 0}
+    mean = float(n.mean())
+    sd = float(n.std())
+    return {"row_nnz_min": int(n.min()), "row_nnz_max": int(n.max()), "mean_nnz_per_row": round(mean, 6),
+            "row_nnz_stddev": round(sd, 6), "row_nnz_cv": round(sd / mean, 6) if mean > 0 else 0.0,
+            "empty_rows": int((n == 0).sum())}
 
 
 def traffic_from_profiles(workload):
@@ -105,16 +184,19 @@ def traffic_from_profiles(workload):
     return None
 
 
-def cpu_baseline(a, b, products_total, target_s=15.0):
-    """The reference's spgemm() on the host cores over a bounded row sample."""
+# ---- CPU baselines (the reference's own code where it is built) -----------
+
+def cpu_baseline(a, b, products_total, target_s=20.0):
+    """The reference's spgemm() on every host core over a bounded row sample,
+    warmup 2, 5 trials x 10 repeats, compute P90."""
     import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
-    rows_total = a.rows
-    # sample: leading rows of A with about target_s seconds of work for the
-    # reference (~3 M products/s per thread with its per-row hash maps)
-    per_row = products_total / max(rows_total, 1)
-    budget_products = 3e6 * threads * target_s / 6.0
-    sample_rows = int(max(256, min(rows_total, budget_products / max(per_row, 1))))
+    threads, physical = host_cores()
+    runs = 1 + CPU_WARMUP + CPU_REPEATS * CPU_TRIALS
+    # leading rows of A holding about target_s / runs seconds of the
+    # reference's work per run (~8e6 products per second per thread)
+    per_row = products_total / max(a.rows, 1)
+    budget_products = 8e6 * threads * target_s / runs
+    sample_rows = int(max(256, min(a.rows, budget_products / max(per_row, 1))))
     sub_p = int(a.row_ptr[sample_rows])
     asub = O.Csr.make(sample_rows, a.cols, a.row_ptr[: sample_rows + 1], a.col[:sub_p], a.val[:sub_p])
     bo = O.Csr.make(b.rows, b.cols, b.row_ptr, b.col, b.val)
@@ -124,24 +206,55 @@ def cpu_baseline(a, b, products_total, target_s=15.0):
             pa, pb = os.path.join(tmp, "a.csr"), os.path.join(tmp, "b.csr")
             O.write_csr(pa, asub)
             O.write_csr(pb, bo)
-            out = O.ref_run("bench", pa, pb, threads, 1, 5, 1, timeout=600)
+            out = O.ref_run("bench", pa, pb, threads, CPU_WARMUP, CPU_REPEATS, CPU_TRIALS, timeout=900)
         r = json.loads(out.strip().splitlines()[-1])
-        p90 = r["compute_p90_s"]
+        p90, p50 = r["compute_p90_s"], r["compute_p50_s"]
         kind = "reference"
     else:
-        times = []
         O.product(asub, bo, threads=threads)
-        for _ in range(5):
-            t0 = time.perf_counter()
-            O.product(asub, bo, threads=threads)
-            times.append(time.perf_counter() - t0)
-        p90 = O.percentile90(times)
+        samples = run_protocol(lambda: O.product(asub, bo, threads=threads), CPU_WARMUP, CPU_REPEATS * CPU_TRIALS)
+        p90, p50 = p90_nearest_rank(samples), statistics.median(samples)
         kind = "port"
     return {"value": round(2.0 * sample_products / p90 / 1e9, 4), "unit": "GFLOP/s", "cores": threads,
-            "kind": kind,
-            "sample": f"first {sample_rows} of {rows_total} rows of A times full B ({sample_products} scalar "
-                      f"products), warmup 1, 5 timed repeats, compute P90; OpenMP guided/64"}
+            "kind": kind, "host_logical_cpus": threads, "host_physical_cores": physical,
+            "p90_ms": round(p90 * 1e3, 3), "p50_ms": round(p50 * 1e3, 3),
+            "sample": f"first {sample_rows} of {a.rows} rows of A times full B ({sample_products} scalar products); "
+                      f"warmup {CPU_WARMUP}, {CPU_TRIALS} trials x {CPU_REPEATS} repeats, compute P90; "
+                      f"OpenMP guided/64 on {threads} threads"}
 
+
+def cpu_baseline_jobs(jobs, njobs):
+    """The reference's spgemm() over the first njobs jobs in sequence (a
+    Galerkin triple as (R.A).P), one sample per pass, warmup 2, 5 trials x 10
+    repeats, P90; jobs/s = njobs / P90."""
+    import oracle as O
+    threads, physical = host_cores()
+    if not O.ref_available():
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        lines = []
+        for i in range(njobs):
+            j = jobs[i]
+            paths = []
+            for k, m in enumerate(j[1:]):
+                pth = os.path.join(tmp, f"j{i}_{k}.csr")
+                O.write_csr(pth, _o(m))
+                paths.append(pth)
+            lines.append(" ".join([j[0]] + paths))
+        man = os.path.join(tmp, "jobs.txt")
+        with open(man, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        out = O.ref_run("benchjobs", man, threads, CPU_WARMUP, CPU_REPEATS, CPU_TRIALS, timeout=900)
+    r = json.loads(out.strip().splitlines()[-1])
+    return {"value": round(r["jobs_per_s_p90"], 3), "unit": "jobs/s", "cores": threads, "kind": "reference",
+            "host_logical_cpus": threads, "host_physical_cores": physical,
+            "gflops": round(r["gflops_p90"], 4), "p90_ms": round(r["p90_s"] * 1e3, 3),
+            "sample": f"first {njobs} jobs in sequence per sample ({r['products']} scalar products), "
+                      f"warmup {CPU_WARMUP}, {CPU_TRIALS} trials x {CPU_REPEATS} repeats, P90; "
+                      f"OpenMP guided/64 on {threads} threads"}
+
+
+# ---- parity helpers (the serial-order oracle on sampled rows) -------------
 
 def oracle_rows(a, b, rows):
     """Serial-order oracle on selected rows of A (bit-exact reference values)."""
@@ -154,7 +267,7 @@ def oracle_rows(a, b, rows):
         vals.append(a.val[s:e])
         rp.append(rp[-1] + int(e - s))
     return O.product(O.Csr.make(len(rows), a.cols, rp, np.concatenate(cols), np.concatenate(vals)),
-                     O.Csr.make(b.rows, b.cols, b.row_ptr, b.col, b.val), threads=max(1, min(16, os.cpu_count() or 1)))
+                     O.Csr.make(b.rows, b.cols, b.row_ptr, b.col, b.val), threads=host_cores()[0])
 
 
 def rows_equal(c, r_c, ref, i):
@@ -170,7 +283,7 @@ class RelErrors:
     agree, else the rows are a mismatch already."""
 
     def __init__(self):
-        self.max_rel, self.diff2, self.ref2, self.ok = 0.0, 0.0, 0.0, True
+        self.max_rel, self.diff2, self.ref2, self.ok, self.finite = 0.0, 0.0, 0.0, True, True
 
     def add(self, c, r_c, ref, i):
         cc = c.col[c.row_ptr[r_c]:c.row_ptr[r_c + 1]]
@@ -180,6 +293,7 @@ class RelErrors:
             return
         cv = c.val[c.row_ptr[r_c]:c.row_ptr[r_c + 1]]
         rv = ref.val[ref.row_ptr[i]:ref.row_ptr[i + 1]]
+        self.finite &= bool(np.all(np.isfinite(cv)))
         d = np.abs(cv - rv)
         nz = rv != 0
         if np.any(nz):
@@ -192,7 +306,7 @@ class RelErrors:
             return {"rows": rows, "columns_match": False}
         fro = (self.diff2 ** 0.5) / (self.ref2 ** 0.5) if self.ref2 > 0 else 0.0
         return {"rows": rows, "columns_match": True, "max_entry_rel_error": self.max_rel,
-                "frobenius_rel_error": fro}
+                "frobenius_rel_error": fro, "values_finite": self.finite}
 
 
 def chunk_bounds(rowp, budget):
@@ -208,7 +322,156 @@ def chunk_bounds(rowp, budget):
     return bounds
 
 
-def run_batched(args, wl):
+def _o(m):
+    import oracle as O
+    return O.Csr.make(m.rows, m.cols, m.row_ptr, m.col, m.val)
+
+
+def streaming_bytes_counts(rows, a_nnz, p, c_nnz):
+    """Streaming-Gustavson bytes of one product: A once, B row bounds and
+    entries once per referencing A entry, C once."""
+    return (8 * (rows + 1) + 12 * a_nnz) + (16 * a_nnz + 12 * p) + (8 * (rows + 1) + 12 * c_nnz)
+
+
+# ---- the single product (headline) ------------------------------------------
+
+def run_single(args, name, wl):
+    import spgemm_amd as S
+    import workloads as W
+
+    chunked = wl["kind"] == "chunked"
+    steps = args.steps if args.steps is not None else (1 if chunked else CPU_REPEATS * CPU_TRIALS)
+    warmup = args.warmup if args.warmup is not None else (0 if chunked else CPU_WARMUP)
+    log(f"bench: generating {name}")
+    a, b = wl["make"]()
+    rowp = W.row_products(a, b)
+    P = int(rowp.sum())
+    eng = S.Engine(0, timing=True)
+    da, db = eng.upload(a), eng.upload(b)  # resident in HBM before timing
+    eng.synchronize()
+    bounds = chunk_bounds(rowp, CHUNK_PRODUCTS) if chunked else [(0, a.rows)]
+    log(f"bench: P={P}, {len(bounds)} row chunk(s)")
+    kernel_ms, sym_ms, nnz_box = [], [], [0]
+
+    def one_step():
+        nnz_total = 0
+        kms = 0.0
+        for r0, r1 in bounds:
+            nnz_total += eng.multiply_device(da, db, (r0, r1) if chunked else None)
+            kms += eng.stats()["ms_numeric"]
+            if chunked:
+                log(f"bench: rows [{r0}, {r1}) done")
+        kernel_ms.append(kms)
+        sym_ms.append(eng.stats()["ms_symbolic"])
+        nnz_box[0] = nnz_total
+
+    log(f"bench: warmup {warmup}, timing {steps} steps")
+    eng.synchronize()  # one GPU, one process: no barrier beyond the device sync
+    samples = run_protocol(one_step, warmup, steps)
+    eng.synchronize()
+    kernel_ms, sym_ms = kernel_ms[warmup:], sym_ms[warmup:]
+    nnz = nnz_box[0]
+    st = eng.stats()
+    tf = timing_fields(samples)
+    structure = None
+    if not chunked:
+        structure = {"a": row_structure(a.row_ptr), "b": row_structure(b.row_ptr),
+                     "c": row_structure(eng.result_row_ptr()), "csr_offset_bits": 64}
+
+    # parity spot check: sampled rows (and the heaviest rows of a sweep)
+    # bit-exact against the serial-order oracle
+    rng = np.random.Generator(np.random.PCG64(11))
+    sample = set(rng.choice(a.rows, size=min(512 if not chunked else 128, a.rows), replace=False).tolist())
+    if chunked:
+        sample.update(int(h) for h in np.argsort(rowp)[-2:])
+    sample = np.array(sorted(sample))
+    log(f"bench: parity check on {len(sample)} rows")
+    ref = oracle_rows(a, b, sample)
+    exact = True
+    rel = RelErrors()
+    pos = {int(r): i for i, r in enumerate(sample)}
+    for r0, r1 in bounds:
+        mine = [int(r) for r in sample if r0 <= r < r1]
+        if not mine:
+            continue
+        if chunked:
+            eng.multiply_device(da, db, (r0, r1))
+        for r in mine:
+            c = eng.result_rows(r - r0, r - r0 + 1)
+            exact &= rows_equal(c, 0, ref, pos[r])
+            rel.add(c, 0, ref, pos[r])
+
+    single = bool(st["single_pass"])
+    entry_bytes = (8 if not st["wide_format"] else 12) + 8  # B row start + length, A value
+    big_rows = int(sum(st["rows_per_bin"][3:]))  # bins W3, W4, Block, Large run after the single-pass kernel
+    if single and not chunked and big_rows == 0:
+        # the timed values phase is the single-pass kernel alone (named as rocprofv3 names it)
+        kernel = f"k_fused<{'false' if st['wide_format'] else 'true'}, 2>"
+        kernel_bytes = 32 * a.rows + entry_bytes * a.nnz + 12 * P + 12 * nnz
+    else:
+        kernel = "numeric phase (single-pass and big-row kernels)" if single else "numeric phase"
+        kernel_bytes = 40 * a.rows + entry_bytes * a.nnz + 12 * P + 12 * nnz
+    kernel_avg = statistics.mean(kernel_ms)
+    achieved = kernel_bytes / (kernel_avg * 1e-3) / 1e9
+    model_bytes = W.streaming_bytes(a, b.rows, P, nnz)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline")
+        try:
+            cpu = cpu_baseline(a, b, P)
+        except Exception as e:  # reported, never fatal
+            cpu = {"value": None, "unit": "GFLOP/s", "cores": 0, "kind": "reference", "sample": f"failed: {e}"}
+
+    p90_s = tf["p90_ms"] * 1e-3
+    cfg = {"workload": name, **wl["desc"], "products": P, "c_nnz": nnz, "a_nnz": a.nnz,
+           "path": "plain drop-in (spg_multiply_begin)" if not chunked else
+                   f"row chunks of <= {CHUNK_PRODUCTS} products (spg_multiply_rows), {len(bounds)} chunks",
+           "timing": {"protocol": "reference: warmup, trials x repeats, each product timed, nearest-rank P90 "
+                                  "(value = 2P / P90)", **tf},
+           "gflops_mean": round(2.0 * P / (tf["mean_ms"] * 1e-3) / 1e9, 3),
+           "pipeline_model_gbs_p90": round(model_bytes / p90_s / 1e9, 1),
+           "single_pass": single,
+           "wide_format": bool(st["wide_format"]),
+           "analyze_symbolic_ms_mean": round(statistics.mean(sym_ms), 4),
+           "dominant_kernel_ms_mean": round(kernel_avg, 4),
+           "rows_per_bin": st["rows_per_bin"],
+           "max_row_products": int(rowp.max()) if len(rowp) else 0,
+           "parity": f"bit-exact on {len(sample)} sampled rows" if exact else "MISMATCH",
+           "full_mode_errors": rel.report(len(sample))}
+    if structure:
+        cfg["row_structure"] = structure
+    out = {
+        "metric": "SpGEMM GFLOP/s (2 x scalar products / P90 product time), single product, plain drop-in",
+        "value": round(2.0 * P / p90_s / 1e9, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": tf["mean_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded CSR generated on the host, resident in HBM before timing)",
+        "config": cfg,
+        "roofline": {"bound": "hbm", "kernel": kernel,
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
+                     "bytes_per_launch": kernel_bytes,
+                     "pipeline_bytes_model": "streaming Gustavson: A once, B row bounds + entries per A entry, "
+                                             "C once",
+                     "pipeline_bytes": model_bytes},
+        "cpu_baseline": cpu,
+    }
+    da.free()
+    db.free()
+    eng.close()
+    return out
+
+
+# ---- batched jobs -------------------------------------------------------------
+
+def run_batched(args, wl, steps=10, warmup=2, cpu=True):
     """Batched jobs/s against sequential plain launches (same resident operands,
     identical bit-exact results)."""
     import oracle as O
@@ -224,7 +487,7 @@ def run_batched(args, wl):
     a1 = [jobs[i][1] for i in ab] + [jobs[i][1] for i in rap]
     b1 = [jobs[i][2] for i in ab] + [jobs[i][2] for i in rap]
     p_list = [jobs[i][3] for i in rap]
-    P1 = sum(W.products(a, b) for a, b in zip(a1, b1))
+    assert all(np.all(np.diff(p.row_ptr) == 1) for p in p_list)  # aggregation: one entry per row
     eng = S.Engine(0, timing=True)
     log("bench: building batch arenas")
     w1 = S.Batch(eng, a1, b1)
@@ -236,24 +499,32 @@ def run_batched(args, wl):
     c2 = w2.autotune(2)
     w1.set_graph(not args.no_graph)
     w2.set_graph(not args.no_graph)
+    gpu_ms, nnz_box = [], [0]
 
     def batch_step():
         n1 = w1.run()
+        t1 = w1.stats()["ms_total"]
         w2.refresh()
         n2 = w2.run()
-        return n1 + n2
+        gpu_ms.append(t1 + w2.stats()["ms_total"])
+        nnz_box[0] = n1 + n2
 
-    steps = args.steps if args.steps is not None else 5
-    warmup = args.warmup if args.warmup is not None else 2
-    for _ in range(warmup):
-        batch_step()
     eng.synchronize()
-    times = []
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        nnz = batch_step()
-        times.append(time.perf_counter() - t0)
-    batch_s = statistics.median(times)
+    samples = run_protocol(batch_step, warmup, steps)
+    gpu_ms = gpu_ms[warmup:]
+    tf = timing_fields(samples)
+    nnz = nnz_box[0]
+
+    # products and algorithmic bytes of both waves (wave 2: (R.A).P with one
+    # P entry per row, so its products equal nnz(R.A))
+    n1 = [w1.product_nnz(i) for i in range(len(a1))]
+    n2 = [w2.product_nnz(k) for k in range(len(rap))]
+    p_wave1 = [W.products(a, b) for a, b in zip(a1, b1)]
+    P_total = sum(p_wave1) + sum(n1[len(ab):])
+    model = sum(streaming_bytes_counts(a.rows, a.nnz, p, c) for a, p, c in zip(a1, p_wave1, n1))
+    model += sum(streaming_bytes_counts(jobs[i][1].rows, n1[len(ab) + k], n1[len(ab) + k], n2[k])
+                 for k, i in enumerate(rap))
+    gpu_mean = statistics.mean(gpu_ms) if gpu_ms else tf["mean_ms"]
 
     # sequential plain launches: one drop-in multiply per product on resident
     # operands; the triples' second product takes the first one's device result
@@ -272,13 +543,8 @@ def run_batched(args, wl):
             else:
                 ra_eng.multiply_device(da, db)
                 seq_eng.begin(ra_eng.result_view(), dp.view())
-    seq_all()  # warm
-    seq_times = []
-    for _ in range(2):
-        t0 = time.perf_counter()
-        seq_all()
-        seq_times.append(time.perf_counter() - t0)
-    seq_s = min(seq_times)
+    seq_samples = run_protocol(seq_all, 1, 2)
+    seq_s = p90_nearest_rank(seq_samples)
 
     # bit-exact: sampled jobs from the batch equal their standalone products
     rng = np.random.Generator(np.random.PCG64(5))
@@ -305,46 +571,51 @@ def run_batched(args, wl):
         else:
             ref = O.product(ref, _o(jobs[i][3]))
             exact &= O.csr_digest(ref) == O.csr_digest(_o(w2.result(pos2[i])))
+    wide = w1.stats()["wide_format"]
+    for tup in dev:
+        for m in tup:
+            if m is not None:
+                m.free()
+    for o in (w2, w1, seq_eng, ra_eng, eng):
+        o.close()
 
-    P_total = P1 + sum(W.products(_o(w1.result(len(ab) + k)), p_list[k]) for k in range(len(rap)))
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("bench: cpu baseline (C oracle, first 96 jobs)")
-        nsample = min(96, njobs)
-        t0 = time.perf_counter()
-        for i in range(nsample):
-            r = O.product(_o(jobs[i][1]), _o(jobs[i][2]), threads=procs)
-            if jobs[i][0] == "rap":
-                O.product(r, _o(jobs[i][3]), threads=procs)
-        dt = time.perf_counter() - t0
-        cpu = {"value": round(nsample / dt, 3), "unit": "jobs/s", "cores": procs, "kind": "port",
-               "sample": f"first {nsample} jobs, serial-order C restatement of spgemm() with OpenMP static rows"}
-    out = {
+    cpu_rec = None
+    if cpu and not args.no_cpu_baseline:
+        log("bench: cpu baseline (reference spgemm over the first 24 jobs)")
+        try:
+            cpu_rec = cpu_baseline_jobs(jobs, min(24, njobs))
+        except Exception as e:  # reported, never fatal
+            cpu_rec = {"value": None, "unit": "jobs/s", "cores": 0, "kind": "reference", "sample": f"failed: {e}"}
+    p90_s = tf["p90_ms"] * 1e-3
+    achieved = model / (gpu_mean * 1e-3) / 1e9
+    return {
         "metric": f"batched SpGEMM jobs/s ({njobs} independent jobs, 1/3 Galerkin R.A.P), one batched "
-                  + ("pipeline launched eagerly" if args.no_graph else "pipeline replayed from a HIP graph"),
-        "value": round(njobs / batch_s, 2),
+                  + ("pipeline launched eagerly" if args.no_graph else "pipeline replayed from a HIP graph")
+                  + " (jobs / P90 batch time)",
+        "value": round(njobs / p90_s, 2),
         "unit": "jobs/s",
-        "n_gpus": 1,
         "steps": steps,
         "warmup": warmup,
-        "ms_per_step": round(batch_s * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded, generated on the host, uploaded once into resident batch arenas)",
-        "config": {"workload": "batched_4096", **wl["desc"], "jobs": njobs, "products": int(P_total), "c_nnz": int(nnz),
+        "ms_per_step": tf["mean_ms"],
+        "config": {"workload": "batched_4096", **wl["desc"], "jobs": njobs, "products": int(P_total),
+                   "c_nnz": int(nnz), "timing": {"protocol": "reference (nearest-rank P90 of timed batches)", **tf},
+                   "gpu_ms_mean": round(gpu_mean, 3),
                    "sequential_jobs_per_s": round(njobs / seq_s, 2),
-                   "speedup_vs_sequential": round(seq_s / batch_s, 3),
-                   "gflops": round(2.0 * P_total / batch_s / 1e9, 2),
-                   "autotune_choice": {"wave1": c1, "wave2": c2},
+                   "speedup_vs_sequential": round(seq_s / p90_s, 3),
+                   "gflops": round(2.0 * P_total / p90_s / 1e9, 2),
+                   "autotune_choice": {"wave1": c1, "wave2": c2}, "wide_format_wave1": bool(wide),
                    "parity": ("bit-exact: 64 sampled jobs equal their standalone products, 8 equal the oracle"
                               if exact else "MISMATCH")},
-        "roofline": None,
-        "cpu_baseline": cpu,
+        "roofline": {"bound": "hbm", "kernel": "whole batched pipeline (both waves, HIP events around each graph)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(model),
+                     "pipeline_bytes_model": "streaming Gustavson summed over every product of both waves"},
+        "cpu_baseline": cpu_rec,
     }
-    print(json.dumps(out), flush=True)
 
+
+# ---- served requests ---------------------------------------------------------
 
 def run_served(args, wl):
     """Poisson stream of products over resident operands: continuous batching
@@ -383,6 +654,7 @@ def run_served(args, wl):
         big_h = (srv.register(big_a), srv.register(big_b))
         rids = [None] * n_req
         lat = np.zeros(n_req)
+        nnzs = np.zeros(n_req, np.int64)
         done_t = np.zeros(n_req)
         ready = threading.Semaphore(0)
 
@@ -391,7 +663,7 @@ def run_served(args, wl):
                 ready.acquire()
                 while rids[k] is None:
                     time.sleep(0.0001)
-                _, lat[k] = srv.wait(rids[k])
+                nnzs[k], lat[k] = srv.wait(rids[k])
                 done_t[k] = time.perf_counter()
                 srv.release(rids[k])
 
@@ -422,46 +694,52 @@ def run_served(args, wl):
         srv.close()
         return {"throughput": n_req / wall, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                 "p99_ms": float(np.percentile(lat, 99) * 1e3), "wall_s": wall, "batches": st["batches"],
-                "exact": exact}
+                "busy_s": st["busy_s"], "nnz": nnzs, "exact": exact}
 
     log("bench: served with continuous batching")
     cb = serve(4096)
     log("bench: served one product per launch")
     seq = serve(1)
-    cfg = {"workload": "served_poisson", **{k: v for k, v in d.items() if k != "budget_bytes"},
-           "budget_gb": d["budget_bytes"] / 2 ** 30, "requests": n_req,
-           "batches": cb["batches"], "p50_ms": round(cb["p50_ms"], 3), "p99_ms": round(cb["p99_ms"], 3),
-           "one_per_launch": {"products_per_s": round(seq["throughput"], 1), "p50_ms": round(seq["p50_ms"], 3),
-                              "p99_ms": round(seq["p99_ms"], 3)},
-           "p99_speedup_vs_one_per_launch": round(seq["p99_ms"] / cb["p99_ms"], 3),
-           "parity": ("bit-exact: 24 sampled requests per mode equal the product computed alone"
-                      if cb["exact"] and seq["exact"] else "MISMATCH"),
-           "config_reading": "configs are read 0-based: config-3-style = the batched jobs' products, "
-                             "config-1 shape = the 2^20 x16 product"}
-    out = {
+    eng.close()
+    # algorithmic bytes of the served stream (every request's product once)
+    p_pair = {}
+    total_bytes = 0
+    for k in range(n_req):
+        key = -1 if is_big[k] else int(pick[k])
+        if key not in p_pair:
+            a, b = (big_a, big_b) if key < 0 else pairs[key]
+            p_pair[key] = (a.rows, a.nnz, W.products(a, b))
+        rows, annz, p = p_pair[key]
+        total_bytes += streaming_bytes_counts(rows, annz, p, int(cb["nnz"][k]))
+    achieved = total_bytes / cb["busy_s"] / 1e9 if cb["busy_s"] > 0 else 0.0
+    return {
         "metric": f"served SpGEMM products/s at a Poisson offered rate of {d['rate_per_s']}/s (p50/p99 latency in "
                   "config)",
         "value": round(cb["throughput"], 1),
         "unit": "products/s",
-        "n_gpus": 1,
         "steps": 1,
         "warmup": 0,
         "ms_per_step": round(cb["wall_s"] * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded operand pool registered once, device-resident)",
-        "config": cfg,
-        "roofline": None,
+        "config": {"workload": "served_poisson", **{k: v for k, v in d.items() if k != "budget_bytes"},
+                   "budget_gb": d["budget_bytes"] / 2 ** 30, "requests": n_req,
+                   "batches": cb["batches"], "p50_ms": round(cb["p50_ms"], 3), "p99_ms": round(cb["p99_ms"], 3),
+                   "device_busy_s": round(cb["busy_s"], 4), "wall_s": round(cb["wall_s"], 4),
+                   "one_per_launch": {"products_per_s": round(seq["throughput"], 1),
+                                      "p50_ms": round(seq["p50_ms"], 3), "p99_ms": round(seq["p99_ms"], 3)},
+                   "p50_speedup_vs_one_per_launch": round(seq["p50_ms"] / cb["p50_ms"], 3),
+                   "p99_speedup_vs_one_per_launch": round(seq["p99_ms"] / cb["p99_ms"], 3),
+                   "parity": ("bit-exact: 24 sampled requests per mode equal the product computed alone"
+                              if cb["exact"] and seq["exact"] else "MISMATCH"),
+                   "config_reading": "configs are read 0-based: config-3-style = the batched jobs' products, "
+                                     "config-1 shape = the 2^20 x16 product"},
+        "roofline": {"bound": "hbm", "kernel": "served batches (device busy time of the continuous-batching runs)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": None,
+                     "pipeline_bytes_model": f"streaming Gustavson summed over the {n_req} served products "
+                                             "(offered-rate bound: the device idles between arrivals)",
+                     "pipeline_bytes": int(total_bytes)},
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
-
-
-def _o(m):
-    import oracle as O
-    return O.Csr.make(m.rows, m.cols, m.row_ptr, m.col, m.val)
 
 
 def main():
@@ -470,6 +748,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="uniform_2p20x16", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-sections", action="store_true",
+                    help="default workload: skip the batched and served sections (for profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--jobs", type=int, default=0, help="batched workload: number of jobs (default 4096)")
     ap.add_argument("--no-graph", action="store_true",
@@ -477,149 +757,29 @@ def main():
                          "(for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this engine computes one rank's local product; "
-              "--gpus must be 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: this engine computes one rank's local product, and only a "
+              "single-GPU figure is measured here; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     wl = WORKLOADS[args.workload]
     if wl["kind"] == "batched":
-        run_batched(args, wl)
-        return
-    if wl["kind"] == "served":
-        run_served(args, wl)
-        return
-    chunked = wl["kind"] == "chunked"
-    steps = args.steps if args.steps is not None else (1 if chunked else 20)
-    warmup = args.warmup if args.warmup is not None else (0 if chunked else 3)
-
-    import spgemm_amd as S
-    import workloads as W
-
-    log(f"bench: generating {args.workload}")
-    a, b = wl["make"]()
-    rowp = W.row_products(a, b)
-    P = int(rowp.sum())
-    eng = S.Engine(0, timing=True)
-    da, db = eng.upload(a), eng.upload(b)  # resident in HBM before timing
-    eng.synchronize()
-    bounds = chunk_bounds(rowp, CHUNK_PRODUCTS) if chunked else [(0, a.rows)]
-    log(f"bench: P={P}, {len(bounds)} row chunk(s)")
-
-    def one_step():
-        nnz_total = 0
-        kms = []
-        for r0, r1 in bounds:
-            nnz_total += eng.multiply_device(da, db, (r0, r1) if chunked else None)
-            kms.append(eng.stats()["ms_numeric"])
-            if chunked:
-                log(f"bench: rows [{r0}, {r1}) done")
-        return nnz_total, kms
-
-    log("bench: warmup")
-    nnz = 0
-    for _ in range(warmup):
-        nnz, _ = one_step()
-    eng.synchronize()
-
-    log(f"bench: timing {steps} steps")
-    step_ms, kernel_ms, sym_ms = [], [], []
-    eng.synchronize()  # one GPU, one process: no barrier beyond the device sync
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ts = time.perf_counter()
-        nnz, kms = one_step()  # returns after the product is complete on the device
-        step_ms.append((time.perf_counter() - ts) * 1e3)
-        kernel_ms.append(sum(kms))
-        sym_ms.append(eng.stats()["ms_symbolic"])
-    eng.synchronize()
-    total_s = time.perf_counter() - t0
-    ms_per_step = total_s * 1e3 / steps
-    st = eng.stats()
-
-    # parity spot check: sampled rows (and the heaviest rows of a sweep)
-    # bit-exact against the serial-order oracle
-    rng = np.random.Generator(np.random.PCG64(11))
-    sample = set(rng.choice(a.rows, size=min(512 if not chunked else 128, a.rows), replace=False).tolist())
-    if chunked:
-        sample.update(int(h) for h in np.argsort(rowp)[-2:])
-    sample = np.array(sorted(sample))
-    log(f"bench: parity check on {len(sample)} rows")
-    ref = oracle_rows(a, b, sample)
-    exact = True
-    rel = RelErrors()
-    pos = {int(r): i for i, r in enumerate(sample)}
-    for r0, r1 in bounds:
-        mine = [int(r) for r in sample if r0 <= r < r1]
-        if not mine:
-            continue
-        if chunked:
-            eng.multiply_device(da, db, (r0, r1))
-        for r in mine:
-            c = eng.result_rows(r - r0, r - r0 + 1)
-            exact &= rows_equal(c, 0, ref, pos[r])
-            rel.add(c, 0, ref, pos[r])
-
-    single = bool(st["single_pass"])
-    entry_bytes = (8 if b.nnz < (1 << 29) else 12) + 8  # B row start + length, A value
-    big_rows = int(sum(st["rows_per_bin"][3:]))  # bins W3, W4, Block, Large run after the single-pass kernel
-    if single and not chunked and big_rows == 0:
-        # the timed values phase is the single-pass kernel alone (named as rocprofv3 names it)
-        kernel = f"k_fused<{'true' if b.nnz < (1 << 29) else 'false'}, 2>"
-        kernel_bytes = 32 * a.rows + entry_bytes * a.nnz + 12 * P + 12 * nnz
+        steps = args.steps if args.steps is not None else 10
+        warmup = args.warmup if args.warmup is not None else 2
+        out = run_batched(args, wl, steps, warmup)
+    elif wl["kind"] == "served":
+        out = run_served(args, wl)
     else:
-        kernel = "numeric phase (single-pass and big-row kernels)" if single else "numeric phase"
-        kernel_bytes = 40 * a.rows + entry_bytes * a.nnz + 12 * P + 12 * nnz
-    kernel_avg = statistics.mean(kernel_ms)
-    achieved = kernel_bytes / (kernel_avg * 1e-3) / 1e9
-    model_bytes = W.streaming_bytes(a, b.rows, P, nnz)
-    traffic = traffic_from_profiles(args.workload)
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("bench: cpu baseline")
-        try:
-            cpu = cpu_baseline(a, b, P)
-        except Exception as e:  # reported, never fatal
-            cpu = {"value": None, "unit": "GFLOP/s", "cores": 0, "kind": "reference", "sample": f"failed: {e}"}
-
-    cfg = {"workload": args.workload, **wl["desc"], "products": P, "c_nnz": nnz, "a_nnz": a.nnz,
-           "path": "plain drop-in (spg_multiply_begin)" if not chunked else
-                   f"row chunks of <= {CHUNK_PRODUCTS} products (spg_multiply_rows), {len(bounds)} chunks",
-           "p50_ms": round(statistics.median(step_ms), 4),
-           "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
-           "pipeline_model_gbs": round(model_bytes / (ms_per_step * 1e-3) / 1e9, 1),
-           "single_pass": single,
-           "analyze_symbolic_ms_last": round(statistics.mean(sym_ms), 4),
-           "dominant_kernel_ms_avg": round(kernel_avg, 4),
-           "rows_per_bin": st["rows_per_bin"],
-           "max_row_products": int(rowp.max()) if len(rowp) else 0,
-           "parity": f"bit-exact on {len(sample)} sampled rows" if exact else "MISMATCH",
-           "full_mode_errors": rel.report(len(sample))}
-    out = {
-        "metric": "SpGEMM GFLOP/s (2 x scalar products / product time), single product, plain drop-in",
-        "value": round(2.0 * P / (ms_per_step * 1e-3) / 1e9, 3),
-        "unit": "GFLOP/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded CSR generated on the host, resident in HBM before timing)",
-        "config": cfg,
-        "roofline": {"bound": "hbm", "kernel": kernel,
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_launch": kernel_bytes,
-                     "pipeline_bytes_model": "streaming Gustavson: A once, B row bounds + entries per A entry, "
-                                             "C once",
-                     "pipeline_bytes": model_bytes},
-        "cpu_baseline": cpu,
-    }
+        out = run_single(args, args.workload, wl)
+        if args.workload == "uniform_2p20x16" and not args.no_sections:
+            sec_args = argparse.Namespace(**{**vars(args), "steps": None, "warmup": None})
+            b = run_batched(sec_args, WORKLOADS["batched_4096"])
+            out["config"]["batched"] = {k: v for k, v in b.items() if k != "metric"} | {"metric": b["metric"]}
+            s = run_served(sec_args, WORKLOADS["served_poisson"])
+            out["config"]["served"] = {k: v for k, v in s.items() if k != "metric"} | {"metric": s["metric"]}
+    for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "dtype", "data"):
+        out.setdefault(k, {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                           "dtype": "f64", "data": "synthetic (seeded operands generated on the host, resident "
+                                                   "in HBM before timing)"}[k])
     print(json.dumps(out), flush=True)
-    da.free()
-    db.free()
-    eng.close()
 
 
 if __name__ == "__main__":

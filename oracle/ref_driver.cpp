@@ -16,7 +16,8 @@
 //   percentile90                src/common/spgemm_common.cpp:916-924
 //   trident::LocalAccumulator   src/trident/common/matrix_blocks.cpp:152-200
 // The timing loop restates src/baselines/two_sided_spgemm.cpp:126-167 for one
-// rank (omp_set_schedule as at :70-72).
+// rank (omp_set_schedule as at :70-72), for one product (bench) or for a
+// sequence of independent products timed as one sample (benchjobs).
 //
 // Binary CSR files (*.csr): "SPGCSR01", int32 rows, int32 cols, int64 nnz,
 // int64 row_ptr[rows+1], int32 col[nnz], float64 val[nnz], little endian.
@@ -100,7 +101,9 @@ int usage() {
     std::cerr << "usage: refdrv read A out.csr | spgemm A B out.csr | serial A B out.csr |\n"
                  "       partial A B out.bin | count A B |\n"
                  "       accumulate out.csr rows cols A1 B1 [A2 B2 ...] |\n"
-                 "       bench A B threads warmup repeats trials [row_limit]\n";
+                 "       bench A B threads warmup repeats trials [row_limit] |\n"
+                 "       benchjobs manifest threads warmup repeats trials\n"
+                 "(manifest lines: \"ab A B\" for A*B, \"rap R A P\" for (R*A)*P)\n";
     return 2;
 }
 
@@ -159,6 +162,71 @@ int main(int argc, char** argv) {
                         "\"compute_p90_s\": %.9g, \"compute_p50_s\": %.9g, \"gflops_p90\": %.6g, "
                         "\"samples\": %zu}\n",
                         a.rows, static_cast<long long>(products), c.values.size(), threads, p90, p50,
+                        p90 > 0 ? 2.0 * static_cast<double>(products) / p90 / 1e9 : 0.0, samples.size());
+        } else if (cmd == "benchjobs" && argc == 7) {
+            // A sequence of independent products timed as one sample, with
+            // the timing loop of two_sided_spgemm.cpp:126-167 (first run,
+            // warmup, trials x repeats, nearest-rank P90).
+            const int threads = std::stoi(argv[3]);
+            const int warmup = std::stoi(argv[4]);
+            const int repeats = std::stoi(argv[5]);
+            const int trials = std::stoi(argv[6]);
+            struct Job {
+                CsrMatrix a, b, p;
+                bool triple;
+            };
+            std::vector<Job> jobs;
+            std::ifstream manifest(argv[2]);
+            std::string kind;
+            while (manifest >> kind) {
+                Job j;
+                std::string pa, pb, pp;
+                if (kind == "ab") {
+                    manifest >> pa >> pb;
+                    j.triple = false;
+                } else if (kind == "rap") {
+                    manifest >> pa >> pb >> pp;
+                    j.triple = true;
+                    j.p = load(pp);
+                } else {
+                    throw std::runtime_error("bad manifest line kind: " + kind);
+                }
+                j.a = load(pa);
+                j.b = load(pb);
+                jobs.push_back(std::move(j));
+            }
+            omp_set_num_threads(threads);
+            std::int64_t products = 0, cnnz = 0;
+            auto pass = [&](bool count) {
+                for (const Job& j : jobs) {
+                    CsrMatrix c = localSpgemm(j.a, j.b);
+                    if (count) products += scalarMultiplicationCount(j.a, j.b);
+                    if (j.triple) {
+                        if (count) products += scalarMultiplicationCount(c, j.p);
+                        c = localSpgemm(c, j.p);
+                    }
+                    if (count) cnnz += static_cast<std::int64_t>(c.values.size());
+                }
+            };
+            pass(true);  // first run
+            for (int i = 0; i < warmup; ++i) pass(false);
+            std::vector<double> samples;
+            for (int t = 0; t < trials; ++t) {
+                for (int r = 0; r < repeats; ++r) {
+                    const auto start = std::chrono::steady_clock::now();
+                    pass(false);
+                    samples.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count());
+                }
+            }
+            std::vector<double> sorted = samples;
+            std::sort(sorted.begin(), sorted.end());
+            const double p90 = percentile90(samples);
+            const double p50 = sorted[(sorted.size() - 1) / 2];
+            std::printf("{\"jobs\": %zu, \"products\": %lld, \"c_nnz\": %lld, \"threads\": %d, "
+                        "\"p90_s\": %.9g, \"p50_s\": %.9g, \"jobs_per_s_p90\": %.6g, \"gflops_p90\": %.6g, "
+                        "\"samples\": %zu}\n",
+                        jobs.size(), static_cast<long long>(products), static_cast<long long>(cnnz), threads, p90,
+                        p50, p90 > 0 ? static_cast<double>(jobs.size()) / p90 : 0.0,
                         p90 > 0 ? 2.0 * static_cast<double>(products) / p90 / 1e9 : 0.0, samples.size());
         } else {
             return usage();

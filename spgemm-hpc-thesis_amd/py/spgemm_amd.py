@@ -441,6 +441,17 @@ class Engine:
         v = self.result_view()
         return Csr(r1 - r0, v.cols, rp, col[:cnt], val[:cnt])
 
+    def result_row_ptr(self) -> np.ndarray:
+        """Row offsets of the last result (rows + 1 int64), without its entries."""
+        v = self.result_view()
+        out = np.empty(v.rows + 1, np.int64)
+        hip = C.CDLL("libamdhip64.so")
+        self.synchronize()
+        rc = hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(v.row_ptr), C.c_size_t(out.nbytes), 2)
+        if rc != 0:
+            raise SpgError(6, f"hipMemcpy of the result's row offsets failed ({rc})")
+        return out
+
     def result_host(self) -> Csr:
         v = self.result_view()
         return self.finish(v.rows, v.cols, v.nnz)
