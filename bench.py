@@ -344,6 +344,13 @@ def run_single(args, name, wl):
     warmup = args.warmup if args.warmup is not None else (0 if chunked else CPU_WARMUP)
     log(f"bench: generating {name}")
     a, b = wl["make"]()
+    if args.row_range:
+        # rows [r0, r1) of A only: that block of C, as one product
+        r0, r1 = (int(x) for x in args.row_range.split(":"))
+        s0, s1 = int(a.row_ptr[r0]), int(a.row_ptr[r1])
+        a = S.Csr.make(r1 - r0, a.cols, a.row_ptr[r0:r1 + 1] - s0, a.col[s0:s1], a.val[s0:s1])
+        chunked = False
+        name = f"{name}[rows {r0}:{r1}]"
     rowp = W.row_products(a, b)
     P = int(rowp.sum())
     eng = S.Engine(0, timing=True)
@@ -646,7 +653,9 @@ def run_served(args, wl):
     is_big = rng.random(n_req) < d["big_fraction"]
     pick = rng.integers(0, len(pairs), size=n_req)
 
-    def serve(max_batch):
+    def serve(max_batch, saturated=False):
+        """The request stream against a server; saturated: every request
+        submitted at once (throughput bound, admission under pressure)."""
         srv = S.Server(0, budget_bytes=d["budget_bytes"], max_batch=max_batch)
         handles = []
         for a, b in pairs:
@@ -673,7 +682,7 @@ def run_served(args, wl):
         t0 = time.perf_counter()
         for k in range(n_req):
             delay = t0 + arrivals[k] - time.perf_counter()
-            if delay > 0:
+            if delay > 0 and not saturated:
                 time.sleep(delay)
             h = big_h if is_big[k] else handles[pick[k]]
             rids[k] = srv.submit_resident(*h)
@@ -684,7 +693,7 @@ def run_served(args, wl):
         st = srv.stats()
         # the same products alone (engine on the same GPU) for a sample of requests
         exact = True
-        for k in rng.choice(n_req, size=min(24, n_req), replace=False):
+        for k in rng.choice(n_req, size=0 if saturated else min(24, n_req), replace=False):
             a, b = (big_a, big_b) if is_big[k] else pairs[pick[k]]
             rid = srv.submit_resident(*(big_h if is_big[k] else handles[pick[k]]))
             got = srv.result(rid)
@@ -700,6 +709,9 @@ def run_served(args, wl):
     cb = serve(4096)
     log("bench: served one product per launch")
     seq = serve(1)
+    log("bench: saturated stream (every request at once), both modes")
+    sat_cb = serve(4096, saturated=True)
+    sat_seq = serve(1, saturated=True)
     eng.close()
     # algorithmic bytes of the served stream (every request's product once)
     p_pair = {}
@@ -726,6 +738,11 @@ def run_served(args, wl):
                    "device_busy_s": round(cb["busy_s"], 4), "wall_s": round(cb["wall_s"], 4),
                    "one_per_launch": {"products_per_s": round(seq["throughput"], 1),
                                       "p50_ms": round(seq["p50_ms"], 3), "p99_ms": round(seq["p99_ms"], 3)},
+                   "saturated": {"requests": n_req,
+                                 "products_per_s": round(sat_cb["throughput"], 1),
+                                 "one_per_launch_products_per_s": round(sat_seq["throughput"], 1),
+                                 "speedup_vs_one_per_launch": round(sat_cb["throughput"] / sat_seq["throughput"], 3),
+                                 "batches": sat_cb["batches"], "p99_ms": round(sat_cb["p99_ms"], 3)},
                    "p50_speedup_vs_one_per_launch": round(seq["p50_ms"] / cb["p50_ms"], 3),
                    "p99_speedup_vs_one_per_launch": round(seq["p99_ms"] / cb["p99_ms"], 3),
                    "parity": ("bit-exact: 24 sampled requests per mode equal the product computed alone"
@@ -751,6 +768,9 @@ def main():
     ap.add_argument("--no-sections", action="store_true",
                     help="default workload: skip the batched and served sections (for profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--row-range", default=None, metavar="R0:R1",
+                    help="single-product workloads: only rows [R0, R1) of A (that block of C, as one product; "
+                         "for profiling a region such as R-MAT's heaviest rows)")
     ap.add_argument("--jobs", type=int, default=0, help="batched workload: number of jobs (default 4096)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched workload: launch the pipeline eagerly instead of replaying a HIP graph "
@@ -769,7 +789,7 @@ def main():
         out = run_served(args, wl)
     else:
         out = run_single(args, args.workload, wl)
-        if args.workload == "uniform_2p20x16" and not args.no_sections:
+        if args.workload == "uniform_2p20x16" and not args.no_sections and not args.row_range:
             sec_args = argparse.Namespace(**{**vars(args), "steps": None, "warmup": None})
             b = run_batched(sec_args, WORKLOADS["batched_4096"])
             out["config"]["batched"] = {k: v for k, v in b.items() if k != "metric"} | {"metric": b["metric"]}

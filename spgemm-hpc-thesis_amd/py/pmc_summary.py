@@ -1,0 +1,113 @@
+"""Reduce rocprofv3 output to the per-kernel tables kept under profiles/.
+
+    python spgemm-hpc-thesis_amd/py/pmc_summary.py counters <counter_collection.csv> [--kernel SUBSTR]
+        [--dispatch last|all|N] [--factor COUNTER=F ...]
+    python spgemm-hpc-thesis_amd/py/pmc_summary.py trace <kernel_stats.csv> [--top N]
+
+counters: one row per (kernel, counter) with the value per dispatch of the
+  chosen dispatches (`last`: the last dispatch of each kernel, which in a
+  bench run is the last timed product; `all`: the mean over every dispatch;
+  `median`: the lower median over every dispatch; N: the N-th dispatch,
+  0-based).  Counters with a calibration factor
+  (--factor FETCH_SIZE=2.0, from a plain device copy of known size) are also
+  printed in bytes (KiB x 1024 x factor).  Derived ratios are printed when
+  their counters are present: SQ_LDS_BANK_CONFLICT / SQ_ACTIVE_INST_LDS and
+  SQ_WAIT_ANY / SQ_WAVE_CYCLES.
+trace: the --kernel-trace --stats summary (calls, average and total
+  milliseconds, share) as a markdown table.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+from collections import OrderedDict, defaultdict
+
+
+def _short(name: str) -> str:
+    """Kernel name without its namespace and argument list."""
+    n = name
+    for prefix in ("void ", "spg::(anonymous namespace)::", "(anonymous namespace)::", "spg::dev::", "spg::"):
+        n = n.replace(prefix, "")
+    return n.split("(")[0].strip()
+
+
+def counters(path, kernel=None, dispatch="last", factors=None):
+    factors = factors or {}
+    rows = defaultdict(lambda: defaultdict(list))  # kernel -> counter -> [(dispatch id, value)]
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            k = _short(r["Kernel_Name"])
+            if kernel and kernel not in k:
+                continue
+            rows[k][r["Counter_Name"]].append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+    out = []
+    for k, cs in rows.items():
+        vals = OrderedDict()
+        ndisp = 0
+        for c, lst in cs.items():
+            lst.sort()
+            ndisp = max(ndisp, len(lst))
+            if dispatch == "last":
+                v = lst[-1][1]
+            elif dispatch == "all":
+                v = sum(x for _, x in lst) / len(lst)
+            elif dispatch == "median":
+                xs = sorted(x for _, x in lst)
+                v = xs[(len(xs) - 1) // 2]
+            else:
+                v = lst[int(dispatch)][1]
+            vals[c] = v
+        out.append((k, ndisp, vals))
+    lines = []
+    for k, ndisp, vals in out:
+        lines.append(f"### `{k}` ({ndisp} dispatches; {dispatch})")
+        lines.append("")
+        lines.append("| counter | per dispatch | bytes |")
+        lines.append("|---|---|---|")
+        for c, v in vals.items():
+            b = f"{v * 1024 * factors[c]:,.0f}" if c in factors else ""
+            lines.append(f"| {c} | {v:,.1f} | {b} |")
+        if "SQ_LDS_BANK_CONFLICT" in vals and vals.get("SQ_ACTIVE_INST_LDS"):
+            lines.append(f"| bank-conflict cycles / LDS-active cycles | "
+                         f"{vals['SQ_LDS_BANK_CONFLICT'] / vals['SQ_ACTIVE_INST_LDS']:.3f} | |")
+        if "SQ_WAIT_ANY" in vals and vals.get("SQ_WAVE_CYCLES"):
+            lines.append(f"| wait cycles / wave cycles | {vals['SQ_WAIT_ANY'] / vals['SQ_WAVE_CYCLES']:.3f} | |")
+        lines.append("")
+    return "\n".join(lines)
+
+
+def trace(path, top=20):
+    with open(path, newline="") as f:
+        rs = list(csv.DictReader(f))
+    rs.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    lines = ["| kernel | calls | avg ms | total ms | % |", "|---|---|---|---|---|"]
+    for r in rs[:top]:
+        lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(lines)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="mode", required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("csv")
+    c.add_argument("--kernel", default=None)
+    c.add_argument("--dispatch", default="last")
+    c.add_argument("--factor", action="append", default=[])
+    t = sub.add_parser("trace")
+    t.add_argument("csv")
+    t.add_argument("--top", type=int, default=20)
+    a = ap.parse_args()
+    if a.mode == "counters":
+        fac = {}
+        for item in a.factor:
+            name, val = item.split("=")
+            fac[name] = float(val)
+        print(counters(a.csv, a.kernel, a.dispatch, fac))
+    else:
+        print(trace(a.csv, a.top))
+
+
+if __name__ == "__main__":
+    main()
