@@ -135,8 +135,35 @@ void enqueue_counts(spg_context *ctx, const ProductPlan &p, unsigned bins, bool 
     launch_zero_control(p.ws.ctrl, st);
     if (p.ws.bdir32) launch_dir32(p.op.bstart, (int64_t)p.op.ndir + 1, p.ws.bdir32, st);
     launch_analyze(p.op, p.ws, st);
-    launch_symbolic(p.op, p.ws, bins, ctx->cus, st);
+    launch_symbolic(p.op, p.ws, bins, ctx->cus, st, p.lsave);
     if (scan) launch_scan(p.ws.snnz, ctx->rowptr[0].as<int64_t>(), p.rows, p.ws.scan_tmp, &p.ws.ctrl->structural, st);
+}
+
+// The counting phase of a plan whose large rows take the compact value
+// kernel: their count saves each row's presence bitmap into a buffer sized by
+// the number of large rows, which is known only after the other bins are
+// counted, so the large rows are counted after a first read of the control
+// block (then C's row offsets are scanned when `scan`).  Leaves the control
+// block's current state in h.
+spg_status counts_compact(spg_context *ctx, ProductPlan &p, unsigned bins, bool scan, Control &h) {
+    hipStream_t st = ctx->stream;
+    p.lsave = nullptr;
+    enqueue_counts(ctx, p, bins & ~(1u << kBinLarge), false);
+    spg_status s;
+    if ((s = read_control(ctx)) != SPG_OK) return s;
+    h = *ctx->h_ctrl;
+    if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
+    const int nlarge = (bins & (1u << kBinLarge)) ? h.bin_count[kBinLarge] : 0;
+    if (nlarge == 0 && !scan) return SPG_OK;
+    if (nlarge > 0) {
+        SPG_HIP(ctx->lsave.ensure((size_t)nlarge * (size_t)(p.ws.lwidth / 8), st), "hipMalloc(large-row bitmaps)");
+        p.lsave = ctx->lsave.as<uint32_t>();
+        launch_symbolic(p.op, p.ws, 1u << kBinLarge, ctx->cus, st, p.lsave);
+    }
+    if (scan) launch_scan(p.ws.snnz, ctx->rowptr[0].as<int64_t>(), p.rows, p.ws.scan_tmp, &p.ws.ctrl->structural, st);
+    if ((s = read_control(ctx)) != SPG_OK) return s;
+    h = *ctx->h_ctrl;
+    return SPG_OK;
 }
 
 spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h) {
@@ -191,7 +218,7 @@ void enqueue_values(spg_context *ctx, const ProductPlan &p) {
         launch_fused(p.op, p.ws, p.fo, p.small_b, p.fused_rows, st);
     }
     launch_numeric(p.op, p.ws, p.bin_count, p.fused ? kBigBins : kAllBins, ctx->cus, ctx->rowptr[0].as<int64_t>(),
-                   ctx->ccol[0].as<int32_t>(), ctx->cval[0].as<double>(), st);
+                   ctx->ccol[0].as<int32_t>(), ctx->cval[0].as<double>(), st, p.lsave, ctx->side, ctx->fork, ctx->join);
     if (p.fused && p.fo.ntiles > 0) launch_sum(p.fo.distinct, p.fo.ntiles, &p.ws.ctrl->distinct, st);
 }
 
@@ -250,10 +277,20 @@ spg_status run_product(spg_context *ctx, ProductPlan &p) {
     // ahead; the rows of bins W1, W2 are computed in one pass by the fused kernel.
     const bool try_fused = ctx->path != SPG_PATH_TWO_PHASE;
     p.ws.mode = (p.ws.mode & ~kWsListSmall) | (try_fused ? 0 : kWsListSmall);
-    enqueue_counts(ctx, p, try_fused ? kBigBins : kAllBins, !try_fused);
-    if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
-    if ((s = read_control(ctx)) != SPG_OK) return s;
-    Control h = *ctx->h_ctrl;
+    // Large rows of products whose B columns fit the compact kernel's window
+    // fold in LDS; their count then saves their bitmaps.
+    p.compact = compact_large_available(p.ws.lwidth) && (int64_t)p.cols <= p.ws.lwidth;
+    p.lsave = nullptr;
+    Control h{};
+    if (p.compact) {
+        if ((s = counts_compact(ctx, p, try_fused ? kBigBins : kAllBins, !try_fused, h)) != SPG_OK) return s;
+        if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
+    } else {
+        enqueue_counts(ctx, p, try_fused ? kBigBins : kAllBins, !try_fused);
+        if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
+        if ((s = read_control(ctx)) != SPG_OK) return s;
+        h = *ctx->h_ctrl;
+    }
     if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
     // C capacity of the single-pass path: the small rows' work bounds plus the
     // big rows' exact counts.  Taken when it fits half of the device memory
@@ -266,9 +303,13 @@ spg_status run_product(spg_context *ctx, ProductPlan &p) {
             // Two-phase after all: list the small rows too (analyze again) and
             // count them.
             p.ws.mode |= kWsListSmall;
-            enqueue_counts(ctx, p, kAllBins, true);
-            if ((s = read_control(ctx)) != SPG_OK) return s;
-            h = *ctx->h_ctrl;
+            if (p.compact) {
+                if ((s = counts_compact(ctx, p, kAllBins, true, h)) != SPG_OK) return s;
+            } else {
+                enqueue_counts(ctx, p, kAllBins, true);
+                if ((s = read_control(ctx)) != SPG_OK) return s;
+                h = *ctx->h_ctrl;
+            }
             if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
         }
     }
@@ -365,6 +406,9 @@ spg_status spg_create(spg_context **out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_ctrl, sizeof(Control), hipHostMallocDefault);
     for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&ctx->ev[i]);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         spg_destroy(ctx);
@@ -380,7 +424,7 @@ spg_status spg_create(spg_context **out, int device) {
 spg_status spg_destroy(spg_context *ctx) {
     if (!ctx) return SPG_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->a_ptr, &ctx->a_col, &ctx->a_val, &ctx->b_ptr, &ctx->b_col, &ctx->b_val,
+    DevBuf *bufs[] = {&ctx->lsave, &ctx->a_ptr, &ctx->a_col, &ctx->a_val, &ctx->b_ptr, &ctx->b_col, &ctx->b_val,
                       &ctx->dir_start, &ctx->dir_end, &ctx->ebst, &ctx->elen, &ctx->ub, &ctx->bin_rows, &ctx->snnz,
                       &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->tile_status, &ctx->tile_distinct, &ctx->part, &ctx->dir32, &ctx->halo_err, &ctx->halo_cnt, &ctx->halo_tmp, &ctx->pack_cnt, &ctx->pack_off, &ctx->pack_map, &ctx->rowptr[0],
                       &ctx->rowptr[1], &ctx->ccol[0], &ctx->ccol[1], &ctx->cval[0], &ctx->cval[1]};
@@ -388,6 +432,9 @@ spg_status spg_destroy(spg_context *ctx) {
     for (hipEvent_t &e : ctx->ev)
         if (e) hipEventDestroy(e);
     if (ctx->h_ctrl) hipHostFree(ctx->h_ctrl);
+    if (ctx->fork) hipEventDestroy(ctx->fork);
+    if (ctx->join) hipEventDestroy(ctx->join);
+    if (ctx->side) hipStreamDestroy(ctx->side);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return SPG_OK;

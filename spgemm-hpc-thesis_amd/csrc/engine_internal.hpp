@@ -61,6 +61,8 @@ struct ProductPlan {
     int64_t pack_blocks = 0;             // packed tiles: analyze blocks
     int64_t *pack_cnt = nullptr;         // packed tiles: tiles per block, then their offsets
     int64_t *pack_off = nullptr;
+    bool compact = false;                // large rows take the compact value kernel (B columns <= kCompactCols)
+    uint32_t *lsave = nullptr;           // compact: the large rows' saved presence bitmaps, sized for this plan
 };
 
 }  // namespace spg
@@ -73,6 +75,8 @@ struct spg_context {
     hipStream_t stream = nullptr;
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t side = nullptr;                      // second stream of the large-row kernels
+    hipEvent_t fork = nullptr, join = nullptr;       // its fork and join (timing disabled)
     std::string err;
     spg::Control *h_ctrl = nullptr;  // pinned mirror
     // uploaded operands (host inputs) and the halo directory
@@ -81,6 +85,7 @@ struct spg_context {
     spg::DevBuf ebst, elen, ub, bin_rows, snnz, annz, scan_tmp, ctrl, lbits, lacc, tile_status, tile_distinct, part, dir32;
     spg::DevBuf halo_err, halo_cnt, halo_tmp;  // device halo directory and packing
     spg::DevBuf pack_cnt, pack_off, pack_map;  // packed single-pass tiles
+    spg::DevBuf lsave;                          // large rows' presence bitmaps (compact value kernel)
     int path = SPG_PATH_AUTO;
     int fused_rows = spg::kFusedRows;
     bool force_wide = false;  // SPG_FORMAT_WIDE: 64-bit entry stream and B addressing whatever B's size

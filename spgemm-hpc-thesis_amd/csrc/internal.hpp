@@ -60,6 +60,13 @@ constexpr int kCapLargeWideNum = 512;
 constexpr int kTpbLargeWideNum = 512;
 constexpr int kCapLargeHbm = 2048;
 constexpr int kTpbLargeHbm = 512;
+// Large rows of products whose B has at most kCompactCols columns fold into
+// an accumulator in LDS indexed by each column's rank among the row's
+// distinct columns, from the presence bitmap the count kernel saves per row
+// (kCompactCols / 8 bytes each).
+constexpr int64_t kCompactCols = int64_t(1) << 16;
+constexpr int kCapLargeCompact = 1024;
+constexpr int kTpbLargeCompact = 512;
 // Single-pass path: rows with at most kFusedCap products and A entries are
 // computed by one wave each, kFusedRows consecutive rows per wave, in tiles of
 // (kFusedTpb / 64) * kFusedRows rows (these are exactly the rows of bins W1, W2).
@@ -115,8 +122,8 @@ struct Control {
     unsigned long long distinct;    // distinct coordinates of the single-pass rows
     int bin_count[8];               // rows listed per bin (list lengths)
     long long bin_rows[8];          // rows per bin (statistics; listed or not)
-    int large_next[2];              // next large row to take: symbolic, numeric kernel
-    int pad_[2];
+    int large_next[3];              // next large row to take: count, HBM-window values, compact values
+    int pad_;
     unsigned long long packed_tiles;  // tiles of the packed single-pass kernel
 };
 
@@ -202,7 +209,11 @@ constexpr unsigned kBigBins = (1u << kBinW3) | (1u << kBinW4) | (1u << kBinBlock
 constexpr unsigned kAllBins = kSmallBins | kBigBins;
 // Symbolic phase for the selected bins (symbolic.hip); kernels read their row
 // counts from the control block, so no host synchronisation is needed before it.
-void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream);
+// With `lsave`, the large-row count also saves each row's presence bitmap
+// (row i of the large-row list at word i * lwidth/32; B's columns must fit
+// one window) for the compact value kernel.
+void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream,
+                     uint32_t *lsave = nullptr);
 // Per-process kernel attributes (dynamic LDS limits); called when a context
 // is created, with the device's LDS bytes per workgroup.
 hipError_t prepare_kernels(size_t lds_per_block);
@@ -228,8 +239,19 @@ size_t scan_tmp_elems(int64_t n);
 void launch_scan(const int64_t *in, int64_t *out, int64_t n, int64_t *tmp, unsigned long long *total_dst,
                  hipStream_t stream);
 // Numeric phase for the selected bins whose host-side counts are nonzero (numeric.hip).
+// With `lsave` (bitmaps saved by the count), large rows take the compact
+// value kernel, which needs compact_large_available(ws.lwidth).
+// The compact kernel runs on `side` when given (forked from and joined back
+// to `stream` with the events fork/join), beside the HBM-window kernel: the
+// window kernel's heaviest rows bound its duration while its other workgroups
+// finish early, and the compact kernel's workgroups fill those CUs.
 void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, unsigned bins, int cus,
-                    const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream);
+                    const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream,
+                    const uint32_t *lsave = nullptr, hipStream_t side = nullptr, hipEvent_t fork = nullptr,
+                    hipEvent_t join = nullptr);
+// Whether the compact large-row value kernel runs for windows of `lwidth`
+// columns on this device (its LDS accumulator holds at least one chunk).
+bool compact_large_available(int64_t lwidth);
 // Copy each row's entries from (rowptr_in) to (rowptr_out); the count of row r
 // is rowptr_out[r+1] - rowptr_out[r] (scan.hip).
 void launch_compact(int32_t rows, const int64_t *rowptr_in, const int32_t *col_in, const double *val_in,

@@ -320,7 +320,7 @@ __device__ __forceinline__ void load_chunk(const Operands &op, const int64_t *s_
 // workgroup's bitmap in HBM, loaded together with the running sum.
 template <int TPB, int CAP, bool LDSB>
 __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws, const int64_t *crowptr,
-                                                       int32_t *ccol, double *cval) {
+                                                       int32_t *ccol, double *cval, int64_t skip_le) {
     constexpr int ITEMS = CAP / TPB;
     extern __shared__ uint32_t s_lbits[];
     constexpr int kExtract = 8;      // accumulator loads in flight per thread during extraction
@@ -347,6 +347,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
     __shared__ int s_next;
     for (int i; (i = next_large_row(&ws.ctrl->large_next[1], &s_next)) < count;) {
         const int row = list[i].row;
+        if (ws.snnz[row] <= skip_le) continue;  // taken by the compact kernel
         const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
         const int64_t cbase = crowptr[row];
         const int64_t lim = crowptr[row + 1] - cbase;
@@ -618,6 +619,214 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
     if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
 }
 
+// Large rows whose B columns fit one window of at most kCompactCols columns,
+// with the presence bitmap the count kernel saved for the row (row i of the
+// large-row list at word i * lwidth/32).  A column's rank among the row's
+// distinct columns -- the set bits before it, from per-word prefix counts in
+// LDS -- indexes a dense accumulator in LDS, so a fold never leaves the CU.
+// Each chunk of CAP products (serial order) is sorted by (rank, serial index)
+// and each rank's run is added, left to right, to its running sum.  Every
+// sum starts at -0.0: -0.0 + p == p for every product p (signed zeros,
+// infinities and NaNs included), which is "the first product initialises the
+// sum" of the serial order.  Rows with more distinct columns than the
+// accumulator holds (acc_cap) are taken in passes over consecutive ranks;
+// each pass streams the row's products again.  The sums come out in column
+// order, exact zeros dropped unless kept.
+template <int TPB, int CAP>
+__global__ __launch_bounds__(TPB) void k_large_compact(Operands op, Workspace ws, const uint32_t *lsave, int acc_cap,
+                                                       int64_t take_le, const int64_t *crowptr, int32_t *ccol,
+                                                       double *cval) {
+    constexpr int ITEMS = CAP / TPB;
+    extern __shared__ double s_acc[];  // acc_cap sums, then the bitmap and its per-word prefix counts
+    __shared__ int64_t s_bst[CAP];
+    __shared__ double s_av[CAP];
+    __shared__ int64_t s_off[CAP + 1];
+    __shared__ uint64_t s_key[CAP];
+    __shared__ double s_val[CAP];
+    __shared__ int s_cnt[CAP + 1];
+    __shared__ int s_wsi[2 * (TPB / 64) + 2];
+    __shared__ int64_t s_ws64[TPB / 64 + 1];
+    __shared__ int s_next;
+    const int tid = threadIdx.x;
+    const RowRec *list = large_list(ws);
+    const int count = ws.ctrl->bin_count[kBinLarge];
+    const int nw = (int)(ws.lwidth / 32);   // words per saved bitmap
+    const int nwc = (op.bcols + 31) / 32;   // words covering B's columns (<= nw)
+    uint32_t *s_bits = reinterpret_cast<uint32_t *>(s_acc + acc_cap);
+    int *s_pre = reinterpret_cast<int *>(s_bits + nw);
+    // Each thread owns consecutive bitmap words: prefix counts, extraction.
+    const int wpt = (nwc + TPB - 1) / TPB;
+    const int q0 = min(tid * wpt, nwc), q1 = min(q0 + wpt, nwc);
+    unsigned flags = 0;
+    unsigned long long stored = 0;
+    for (int i; (i = next_large_row(&ws.ctrl->large_next[2], &s_next)) < count;) {
+        const int row = list[i].row;
+        if (ws.snnz[row] > take_le) continue;  // taken by the HBM-window kernel
+        const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
+        const int64_t cbase = crowptr[row];
+        const int64_t lim = crowptr[row + 1] - cbase;
+        const uint32_t *saved = lsave + (size_t)i * (size_t)nw;
+        int mine = 0;
+        for (int q = q0; q < q1; ++q) {
+            const uint32_t w = saved[q];
+            s_bits[q] = w;
+            mine += __popc(w);
+        }
+        int64_t T;
+        int64_t run = block_excl_scan<TPB>((int64_t)mine, s_ws64, T);
+        for (int q = q0; q < q1; ++q) {
+            s_pre[q] = (int)run;
+            run += __popc(s_bits[q]);
+        }
+        __syncthreads();  // bitmap and prefix counts in place
+        int64_t outn = 0;
+        for (int64_t r0 = 0; r0 < T; r0 += acc_cap) {
+            const int racc = (int)min((int64_t)acc_cap, T - r0);
+            for (int q = tid; q < racc; q += TPB) s_acc[q] = -0.0;
+            __syncthreads();
+            for (int64_t e0 = a0; e0 < a1; e0 += CAP) {
+                const int nbt = (int)min((int64_t)CAP, a1 - e0);
+                const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, s_av, s_off, s_ws64);
+                // Each chunk's products are loaded one chunk ahead.
+                int pcol[ITEMS], pent[ITEMS];
+                double pbv[ITEMS];
+                load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, 0, pcol, pbv, pent);
+                for (int64_t c0 = 0; c0 < Pb; c0 += CAP) {
+                    int rk[ITEMS];
+                    double ival[ITEMS];
+                    bool ok[ITEMS];
+                    int rmn = INT_MAX, rmx = INT_MIN, nvalid = 0;
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        const int c = pcol[j];  // -1 past the batch
+                        bool in = c >= 0 && c < op.bcols;
+                        int r = 0;
+                        if (in) {
+                            const uint32_t wd = s_bits[c >> 5];
+                            if (!((wd >> (c & 31)) & 1u)) flags |= kErrInternal;  // the count saw every column
+                            r = s_pre[c >> 5] + __popc(wd & ((1u << (c & 31)) - 1u)) - (int)r0;
+                            in = r >= 0 && r < racc;
+                        }
+                        ok[j] = in;
+                        rk[j] = in ? r : 0;
+                        ival[j] = in ? s_av[pent[j]] * pbv[j] : 0.0;
+                        if (in) {
+                            rmn = min(rmn, r);
+                            rmx = max(rmx, r);
+                            ++nvalid;
+                        }
+                    }
+                    load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, pcol, pbv, pent);
+                    const int V = block_sum<TPB>(nvalid, s_wsi);
+                    if (V == 0) continue;  // uniform across the block
+                    block_minmax<TPB>(rmn, rmx, s_wsi);
+                    int nb = TPB;
+                    while (nb < V && nb < CAP) nb <<= 1;
+                    const int lognb = 31 - __clz(nb);
+                    const uint32_t range = (uint32_t)(rmx - rmn) + 1u;
+                    const int lr = range <= 1u ? 0 : 32 - __clz((int)(range - 1u));
+                    const int sh = lr > lognb ? lr - lognb : 0;
+                    for (int t = tid; t < nb; t += TPB) s_cnt[t] = 0;
+                    __syncthreads();
+                    int bk[ITEMS], pos[ITEMS];
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        bk[j] = 0;
+                        pos[j] = 0;
+                        if (ok[j]) {
+                            bk[j] = (int)((uint32_t)(rk[j] - rmn) >> sh);
+                            pos[j] = atomicAdd(&s_cnt[bk[j]], 1);
+                        }
+                    }
+                    __syncthreads();
+                    block_scan_runs<TPB>(s_cnt, nb, s_wsi);
+                    if (tid == 0) s_cnt[nb] = V;
+                    __syncthreads();
+                    uint64_t key[ITEMS];
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        key[j] = ((uint64_t)(uint32_t)rk[j] << 32) | (uint32_t)(tid + j * TPB);
+                        if (ok[j]) s_key[s_cnt[bk[j]] + pos[j]] = key[j];
+                    }
+                    __syncthreads();
+                    // Rank inside the bucket by (rank, serial index): the
+                    // item's sorted position.  Keys are distinct.
+                    int x[ITEMS];
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        x[j] = -1;
+                        if (ok[j]) {
+                            const int lo = s_cnt[bk[j]], hi = s_cnt[bk[j] + 1];
+                            int rank = 0;
+                            for (int y = lo; y < hi; ++y) rank += s_key[y] < key[j];
+                            x[j] = lo + rank;
+                        }
+                    }
+                    __syncthreads();  // every bucket read before the sorted placement
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        if (ok[j]) {
+                            s_key[x[j]] = key[j];
+                            s_val[x[j]] = ival[j];
+                        }
+                    }
+                    __syncthreads();
+                    // Each rank's first sorted position adds the rank's run,
+                    // in serial order, to its running sum.
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        const int y = tid + j * TPB;
+                        if (y < V && (y == 0 || (s_key[y - 1] >> 32) != (s_key[y] >> 32))) {
+                            const uint32_t r = (uint32_t)(s_key[y] >> 32);
+                            double v = s_acc[r] + s_val[y];
+                            for (int z = y + 1; z < V && (uint32_t)(s_key[z] >> 32) == r; ++z) v = v + s_val[z];
+                            s_acc[r] = v;
+                        }
+                    }
+                    __syncthreads();  // sums in place; chunk arrays reused next
+                }
+            }
+            // Extraction of ranks [r0, r0 + racc) in column order: the set
+            // bits of the thread's words whose rank falls in the pass.
+            int cntp = 0;
+            for (int q = q0; q < q1; ++q) {
+                uint32_t m = s_bits[q];
+                int r = s_pre[q] - (int)r0;
+                if (r >= racc || r + __popc(m) <= 0) continue;
+                for (; m; m &= m - 1u, ++r)
+                    if (r >= 0 && r < racc && (op.keep_zeros || s_acc[r] != 0.0)) ++cntp;
+            }
+            int64_t ptotal;
+            int64_t o = outn + block_excl_scan<TPB>((int64_t)cntp, s_ws64, ptotal);
+            for (int q = q0; q < q1; ++q) {
+                uint32_t m = s_bits[q];
+                int r = s_pre[q] - (int)r0;
+                if (r >= racc || r + __popc(m) <= 0) continue;
+                for (; m; m &= m - 1u, ++r) {
+                    if (r < 0 || r >= racc) continue;
+                    const double v = s_acc[r];
+                    if (op.keep_zeros || v != 0.0) {
+                        if (o < lim) {
+                            ccol[cbase + o] = q * 32 + (__ffs((int)m) - 1);
+                            cval[cbase + o] = v;
+                        }
+                        ++o;
+                    }
+                }
+            }
+            outn += ptotal;
+            __syncthreads();  // sums read before the next pass resets them
+        }
+        if (tid == 0) {
+            ws.annz[row] = outn;
+            stored += (unsigned long long)outn;
+            if (outn > lim || T > lim) flags |= kErrInternal;
+        }
+    }
+    if (tid == 0 && stored) atomicAdd(&ws.ctrl->actual, stored);
+    if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
+}
+
 template <int TPB, int CAP>
 void launch_esc(const Operands &op, const Workspace &ws, int bin, int rows, int cus, const int64_t *crowptr,
                 int32_t *ccol, double *cval, hipStream_t stream) {
@@ -629,10 +838,32 @@ void launch_esc(const Operands &op, const Workspace &ws, int bin, int rows, int 
 
 // Set once per process by prepare_numeric_kernels.
 std::atomic<bool> g_num_wide{false};
+std::atomic<int> g_compact_dyn{0};  // dynamic LDS bytes the compact large-row kernel may use
 
 }  // namespace
 
+bool compact_large_available(int64_t lwidth) {
+    if (lwidth > kCompactCols) return false;
+    // the bitmap and its per-word prefix counts: 2 x lwidth/32 words
+    const int64_t acc = ((int64_t)g_compact_dyn.load(std::memory_order_relaxed) - lwidth / 4) / 8;
+    return acc >= kCapLargeCompact;
+}
+
 bool prepare_numeric_kernels(size_t lds_per_block) {
+    {
+        // The compact large-row kernel's accumulator takes whatever LDS its
+        // static arrays leave.
+        const void *fk = reinterpret_cast<const void *>(&k_large_compact<kTpbLargeCompact, kCapLargeCompact>);
+        hipFuncAttributes attr{};
+        int dyn = 0;
+        if (hipFuncGetAttributes(&attr, fk) == hipSuccess && lds_per_block > attr.sharedSizeBytes) {
+            dyn = (int)(lds_per_block - attr.sharedSizeBytes);
+            dyn &= ~255;
+            if (hipFuncSetAttribute(fk, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess) dyn = 0;
+        }
+        (void)hipGetLastError();
+        g_compact_dyn.store(dyn, std::memory_order_relaxed);
+    }
     const size_t dyn = (size_t)(kLdsBitsColsWide / 8);
     constexpr int C = kCapLargeWideNum;
     // static arrays: batch metadata 3 x 8 B, two chunk buffers 2 x 16 B, bucket counts 4 B per product
@@ -650,7 +881,8 @@ bool prepare_numeric_kernels(size_t lds_per_block) {
 }
 
 void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, unsigned bins, int cus,
-                    const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream) {
+                    const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream, const uint32_t *lsave,
+                    hipStream_t side, hipEvent_t fork, hipEvent_t join) {
     if (bins & (1u << kBinW1))
         launch_esc<64, kCapW1>(op, ws, kBinW1, bin_counts[kBinW1], cus, crowptr, ccol, cval, stream);
     if (bins & (1u << kBinW2))
@@ -662,16 +894,40 @@ void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_coun
     if (bins & (1u << kBinBlock))
         launch_esc<kTpbBlock, kCapBlock>(op, ws, kBinBlock, bin_counts[kBinBlock], cus, crowptr, ccol, cval, stream);
     if ((bins & (1u << kBinLarge)) && bin_counts[kBinLarge] > 0) {
+        // Rows whose distinct columns fit the compact kernel's accumulator
+        // (bitmaps saved by the count) fold in LDS; wider rows fold onto the
+        // HBM windows.  Each kernel skips the other's rows.
+        const bool compact = lsave && compact_large_available(ws.lwidth);
+        const int dyn = g_compact_dyn.load(std::memory_order_relaxed);
+        const int acc_cap = compact ? (int)(((int64_t)dyn - ws.lwidth / 4) / 8) & ~63 : 0;
+        const int64_t skip_le = compact ? acc_cap : -1;
+        const bool forked = compact && side && fork && join;
+        if (forked) {
+            (void)hipEventRecord(fork, stream);
+            (void)hipStreamWaitEvent(side, fork, 0);
+        }
         const int g = bin_counts[kBinLarge] < ws.lblocks ? bin_counts[kBinLarge] : ws.lblocks;
         if (ws.lwidth <= kLdsBitsCols)
             hipLaunchKernelGGL((k_large_numeric<kTpbLarge, kCapLarge, true>), dim3(g), dim3(kTpbLarge),
-                               (unsigned)(ws.lwidth / 8), stream, op, ws, crowptr, ccol, cval);
+                               (unsigned)(ws.lwidth / 8), stream, op, ws, crowptr, ccol, cval, skip_le);
         else if (ws.lwidth <= kLdsBitsColsWide && g_num_wide.load(std::memory_order_relaxed))
             hipLaunchKernelGGL((k_large_numeric<kTpbLargeWideNum, kCapLargeWideNum, true>), dim3(g),
-                               dim3(kTpbLargeWideNum), (unsigned)(ws.lwidth / 8), stream, op, ws, crowptr, ccol, cval);
+                               dim3(kTpbLargeWideNum), (unsigned)(ws.lwidth / 8), stream, op, ws, crowptr, ccol, cval,
+                               skip_le);
         else
             hipLaunchKernelGGL((k_large_numeric<kTpbLargeHbm, kCapLargeHbm, false>), dim3(g), dim3(kTpbLargeHbm), 0,
-                               stream, op, ws, crowptr, ccol, cval);
+                               stream, op, ws, crowptr, ccol, cval, skip_le);
+        if (compact) {
+            // One workgroup per CU (its accumulator fills the CU's LDS).
+            const int gc = bin_counts[kBinLarge] < cus ? bin_counts[kBinLarge] : cus;
+            hipLaunchKernelGGL((k_large_compact<kTpbLargeCompact, kCapLargeCompact>), dim3(gc),
+                               dim3(kTpbLargeCompact), (unsigned)((size_t)acc_cap * 8 + (size_t)(ws.lwidth / 4)),
+                               forked ? side : stream, op, ws, lsave, acc_cap, (int64_t)acc_cap, crowptr, ccol, cval);
+            if (forked) {
+                (void)hipEventRecord(join, side);
+                (void)hipStreamWaitEvent(stream, join, 0);
+            }
+        }
     }
 }
 

@@ -224,9 +224,12 @@ __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Wo
 }
 
 // LDSB: the window's bitmap lives in dynamic LDS (lwidth / 8 bytes) and the
-// claims are LDS atomics; else it is the workgroup's bitmap in HBM.
+// claims are LDS atomics; else it is the workgroup's bitmap in HBM.  With
+// `lsave` (LDSB, B's columns in one window) each row's bitmap is also saved,
+// row i of the large-row list at word i * lwidth/32, for the compact value
+// kernel.
 template <int TPB, int CAP, bool LDSB>
-__global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace ws) {
+__global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace ws, uint32_t *lsave) {
     constexpr int UNROLL = LDSB ? 2 * kLargeUnroll : kLargeUnroll;
     extern __shared__ uint32_t s_lbits[];
     __shared__ int64_t s_bst[CAP];
@@ -296,9 +299,14 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                 __syncthreads();  // batch metadata is overwritten next
             }
             const int64_t nwords = (wend - w0 + 31) / 32;
+            uint32_t *save = LDSB && lsave ? lsave + (size_t)i * (size_t)(W / 32) : nullptr;
             for (int64_t q = tid; q < nwords; q += TPB) {
-                if (LDSB) s_lbits[q] = 0u;
-                else st_bits(&bits[q], 0u);
+                if (LDSB) {
+                    if (save) save[q] = s_lbits[q];
+                    s_lbits[q] = 0u;
+                } else {
+                    st_bits(&bits[q], 0u);
+                }
             }
             __threadfence();
             __syncthreads();
@@ -345,7 +353,8 @@ hipError_t prepare_kernels(size_t lds_per_block) {
     return hipSuccess;
 }
 
-void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream) {
+void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream,
+                     uint32_t *lsave) {
     if (op.arows <= 0) return;
     if (bins & (1u << kBinW1)) launch_hash<64, kCapW1>(op, ws, kBinW1, cus, stream);
     if (bins & (1u << kBinW2)) launch_hash<64, kCapW2>(op, ws, kBinW2, cus, stream);
@@ -355,13 +364,14 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
     if ((bins & (1u << kBinLarge)) && ws.lblocks > 0) {
         if (ws.lwidth <= kLdsBitsCols)
             hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, true>), dim3(ws.lblocks), dim3(kTpbLarge),
-                               (unsigned)(ws.lwidth / 8), stream, op, ws);
+                               (unsigned)(ws.lwidth / 8), stream, op, ws, op.bcols <= ws.lwidth ? lsave : nullptr);
         else if (ws.lwidth <= kLdsBitsColsWide && g_wide_lds.load(std::memory_order_relaxed))
             hipLaunchKernelGGL((k_large_symbolic<kTpbLargeWide, kCapLarge, true>), dim3(ws.lblocks),
-                               dim3(kTpbLargeWide), (unsigned)(ws.lwidth / 8), stream, op, ws);
+                               dim3(kTpbLargeWide), (unsigned)(ws.lwidth / 8), stream, op, ws,
+                               (uint32_t *)nullptr);
         else
             hipLaunchKernelGGL((k_large_symbolic<kTpbLargeHbm, kCapLargeHbm, false>), dim3(ws.lblocks),
-                               dim3(kTpbLargeHbm), 0, stream, op, ws);
+                               dim3(kTpbLargeHbm), 0, stream, op, ws, (uint32_t *)nullptr);
     }
 }
 

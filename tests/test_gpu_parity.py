@@ -393,6 +393,32 @@ def test_large_rows_exact_zero_sums(engine, spread, cols):
     assert digest(c) == O.csr_digest(ref)
 
 
+@pytest.mark.parametrize("path", [S.PATH_AUTO, S.PATH_TWO_PHASE], ids=["auto", "two_phase"])
+def test_large_rows_rank_passes(engine, path):
+    # Rows of 60,000 +-1 products into 50,000 columns (about 26,600 distinct
+    # each, more than one LDS accumulator pass holds) and 3 rows of 12,000
+    # products into 24 columns: the compact large-row kernel takes several
+    # passes over rank ranges, running sums cancel to exactly zero inside and
+    # across passes, and zeros are dropped; bit-exact against the oracle.
+    rng = np.random.Generator(np.random.PCG64(33))
+    b = pm1_csr(rng, 3000, 50000, 20)
+    wide = pm1_csr(rng, 12, 3000, 3000)
+    narrow_b = O.Csr.make(3000, 50000, b.row_ptr, (b.col.astype(np.int64) % 24) * 2083, b.val)
+    a = O.Csr.make(15, 3000, np.concatenate([wide.row_ptr, wide.row_ptr[-1] + np.array([600, 1200, 1800])]),
+                   np.concatenate([wide.col, rng.integers(0, 3000, size=1800)]),
+                   np.concatenate([wide.val, rng.choice(np.array([-1.0, 1.0]), size=1800)]))
+    engine.set_path(path)
+    try:
+        for bb in (b, narrow_b):
+            c = engine.multiply(to_s(a), to_s(bb))
+            assert engine.stats()["rows_per_bin"][6] == 15
+            ref = O.product(a, bb)
+            assert digest(c) == O.csr_digest(ref)
+        assert ref.nnz < 3 * 24 + 12 * 50000  # exact zeros were dropped
+    finally:
+        engine.set_path(S.PATH_AUTO)
+
+
 def test_heavy_row_tiles_in_analyze(engine):
     # Rows 0..63 hold 400 entries each (25,600 entries: one tile walked by all
     # waves of its workgroup), the other rows 5; whole product bit-exact, and
