@@ -411,7 +411,12 @@ def run_single(args, name, wl):
     single = bool(st["single_pass"])
     entry_bytes = (8 if not st["wide_format"] else 12) + 8  # B row start + length, A value
     big_rows = int(sum(st["rows_per_bin"][3:]))  # bins W3, W4, Block, Large run after the single-pass kernel
-    if single and not chunked and big_rows == 0:
+    if st["direct"]:
+        # direct single pass (no analysis): per row A's and C's row offsets,
+        # per A entry its column, value and B's two 32-bit row offsets
+        kernel = f"k_fused<{'false' if st['wide_format'] else 'true'}, 2, true>"
+        kernel_bytes = 16 * a.rows + 20 * a.nnz + 12 * P + 12 * nnz
+    elif single and not chunked and big_rows == 0:
         # the timed values phase is the single-pass kernel alone (named as rocprofv3 names it)
         kernel = f"k_fused<{'false' if st['wide_format'] else 'true'}, 2>"
         kernel_bytes = 32 * a.rows + entry_bytes * a.nnz + 12 * P + 12 * nnz
@@ -438,6 +443,7 @@ def run_single(args, name, wl):
            "gflops_mean": round(2.0 * P / (tf["mean_ms"] * 1e-3) / 1e9, 3),
            "pipeline_model_gbs_p90": round(model_bytes / p90_s / 1e9, 1),
            "single_pass": single,
+           "direct_single_pass": bool(st["direct"]),
            "wide_format": bool(st["wide_format"]),
            "analyze_symbolic_ms_mean": round(statistics.mean(sym_ms), 4),
            "dominant_kernel_ms_mean": round(kernel_avg, 4),

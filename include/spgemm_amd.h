@@ -98,6 +98,9 @@ typedef struct spg_stats {
     double ms_total;           /* device time of the whole pipeline (events), if timing on */
     double ms_symbolic;        /* device time of the symbolic phase, if timing on */
     double ms_numeric;         /* device time of the numeric phase, if timing on */
+    int32_t direct;            /* 1 if every row fit the single pass by the longest rows of A
+                                  and B alone, so the per-row analysis was skipped */
+    int32_t reserved_;
 } spg_stats;
 
 /* ---- context ---------------------------------------------------------- */

@@ -12,6 +12,8 @@
 // gather B's row offsets again.
 #include "device_common.hpp"
 
+#include <algorithm>
+
 namespace spg {
 namespace {
 
@@ -30,6 +32,45 @@ __global__ void k_zero_words(uint32_t *p, size_t n) {
 constexpr int kAnalyzeTpb = 256;
 static_assert(kAnalyzeTpb / 64 * 64 == kAnalyzeRows, "packed single-pass tiles take an analyze block's rows");
 constexpr int kAnChunks = 2;  // 64-entry chunks in flight per wave (1 and 4 measured no faster)
+
+// Longest row of A and of B (block maxima, one atomic per block each) and
+// B's row offsets in 32 bits.
+__global__ void k_rowmax(const int64_t *aptr, int32_t arows, const int64_t *bptr, int32_t brows, uint32_t *dir32,
+                         Control *ctrl) {
+    __shared__ unsigned long long s_m[2][4];
+    unsigned long long ma = 0, mb = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= (int64_t)brows; i += stride) {
+        const int64_t v = bptr[i];
+        dir32[i] = (uint32_t)v;
+        if (i < brows) {
+            const int64_t l = bptr[i + 1] - v;
+            mb = max(mb, (unsigned long long)(l > 0 ? l : 0));
+        }
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)arows; i += stride) {
+        const int64_t l = aptr[i + 1] - aptr[i];
+        ma = max(ma, (unsigned long long)(l > 0 ? l : 0));
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        ma = max(ma, (unsigned long long)__shfl_xor((long long)ma, d, 64));
+        mb = max(mb, (unsigned long long)__shfl_xor((long long)mb, d, 64));
+    }
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s_m[0][w] = ma;
+        s_m[1][w] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            ma = max(ma, s_m[0][k]);
+            mb = max(mb, s_m[1][k]);
+        }
+        if (ma) atomicMax(&ctrl->max_a_row, ma);
+        if (mb) atomicMax(&ctrl->max_b_row, mb);
+    }
+}
 
 __global__ void k_dir32(const int64_t *bptr, int64_t n, uint32_t *out) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -309,6 +350,15 @@ void launch_dir32(const int64_t *bptr, int64_t n, uint32_t *out, hipStream_t str
     int64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_dir32, dim3((unsigned)blocks), dim3(256), 0, stream, bptr, n, out);
+}
+
+void launch_rowmax(const int64_t *aptr, int32_t arows, const int64_t *bptr, int32_t brows, uint32_t *dir32,
+                   Control *ctrl, hipStream_t stream) {
+    const int64_t n = std::max<int64_t>((int64_t)arows, (int64_t)brows + 1);
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_rowmax, dim3((unsigned)blocks), dim3(256), 0, stream, aptr, arows, bptr, brows, dir32, ctrl);
 }
 
 int64_t analyze_blocks(int32_t rows) {

@@ -89,6 +89,7 @@ struct spg_context {
     int path = SPG_PATH_AUTO;
     int fused_rows = spg::kFusedRows;
     bool force_wide = false;  // SPG_FORMAT_WIDE: 64-bit entry stream and B addressing whatever B's size
+    bool allow_direct = true; // products whose rows all fit the single pass skip the analysis (not for batch plans)
     int64_t lbits_width = 0;
     // results (ping-pong for compaction)
     spg::DevBuf rowptr[2], ccol[2], cval[2];

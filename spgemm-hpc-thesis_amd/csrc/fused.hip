@@ -99,14 +99,26 @@ __device__ __forceinline__ T ld_byte(const T *base, uint32_t byte_off) {
 
 // Entry lane of a row: lanes past the row's last entry re-read it, so the
 // load is unconditional (no register initialisation that would wait on it).
-template <bool SMALLB>
+// DIRECT: the B row comes from the entry's column through B's 32-bit row
+// offsets (ws.bdir32) instead of the analysis stream; a column outside B's
+// rows reads as an empty row and raises kErrRange.  na must be positive.
+template <bool SMALLB, bool DIRECT = false>
 __device__ __forceinline__ MetaReg<SMALLB> load_meta(const Operands &op, const Workspace &ws, int64_t a0, int na,
-                                                     int c0) {
+                                                     int c0, unsigned &flags) {
     using bst_t = typename FusedWave<SMALLB>::bst_t;
     const int64_t e = a0 + min(c0 + lane_id(), max(na - 1, 0));
     MetaReg<SMALLB> m;
-    m.bst = (bst_t)entry_bstart<SMALLB>(ws, e);  // small B: 32-bit stream
-    m.len = __builtin_nontemporal_load(&ws.elen[e]);
+    if constexpr (DIRECT) {
+        const int c = __builtin_nontemporal_load(&op.acol[e]);
+        const bool ok = (unsigned)c < (unsigned)op.ndir;
+        if (__ballot(!ok)) flags |= kErrRange;  // uniform: the kernel publishes lane 0's flags
+        const uint32_t b0 = ws.bdir32[ok ? c : 0], b1 = ws.bdir32[ok ? c + 1 : 0];
+        m.bst = (bst_t)b0;
+        m.len = ok ? (int)(b1 - b0) : 0;
+    } else {
+        m.bst = (bst_t)entry_bstart<SMALLB>(ws, e);  // small B: 32-bit stream
+        m.len = __builtin_nontemporal_load(&ws.elen[e]);
+    }
     m.av = __builtin_nontemporal_load(&op.aval[e]);
     return m;
 }
@@ -123,13 +135,13 @@ struct RowItems {
 // the exclusive prefix of the B row lengths over all E entries (off[E] is
 // the total).  m0 holds entries 0..63.  Lanes past E write harmless values
 // below kFusedCap; off[E] is written last.
-template <bool SMALLB>
+template <bool SMALLB, bool DIRECT = false>
 __device__ __forceinline__ void stage_meta(const Operands &op, const Workspace &ws, FusedWave<SMALLB> &w, int64_t a0,
-                                           int E, const MetaReg<SMALLB> &m0) {
+                                           int E, const MetaReg<SMALLB> &m0, unsigned &flags) {
     const int lane = lane_id();
     int carry = 0;
     for (int c0 = 0; c0 < E; c0 += 64) {
-        const MetaReg<SMALLB> m = c0 == 0 ? m0 : load_meta<SMALLB>(op, ws, a0, E, c0);
+        const MetaReg<SMALLB> m = c0 == 0 ? m0 : load_meta<SMALLB, DIRECT>(op, ws, a0, E, c0, flags);
         const int e = c0 + lane;
         const int len = e < E ? m.len : 0;
         const int incl = wave_incl_sum_dpp(len);
@@ -489,9 +501,15 @@ __device__ __forceinline__ int64_t lane64(int64_t v, int k) {
 #define SPG_FUSED_ATTR
 #endif
 
-template <bool SMALLB, int R>
+// DIRECT: every row has at most kFusedCap products and A entries (the host
+// checked the longest rows of A and B), so no analysis ran: each row's
+// metadata comes from its columns and B's 32-bit row offsets, its product
+// count from the staged lengths, and the tile also publishes the statistics
+// the analysis would have gathered (products, rows per bin).
+template <bool SMALLB, int R, bool DIRECT = false>
 __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op, Workspace ws, FusedOut fo) {
     constexpr int NW = kFusedTpb / 64;
+    constexpr int NS = kDirectFields - 1;  // DIRECT statistics besides the distinct columns
     __shared__ FusedWave<SMALLB> s_w[NW];
     __shared__ int64_t s_wtot[NW];
     __shared__ int64_t s_base;
@@ -499,6 +517,7 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op,
     const int lane = lane_id();
     FusedWave<SMALLB> &w = s_w[wv];
     __shared__ int64_t s_dist[NW];
+    __shared__ int64_t s_dstat[DIRECT ? NW : 1][NS];
     unsigned flags = 0;
     int64_t distinct_sum = 0;
 
@@ -510,40 +529,51 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op,
     // Row offsets and work bounds of this wave's rows (lane k: row r0+k).
     const int64_t ap = lane <= R ? op.aptr[min(r0 + lane, (int64_t)op.arows)] : 0;
     const int64_t rq = min(r0 + lane, (int64_t)op.arows - 1);
-    const int64_t ubl = lane < R ? ws.ub[rq] : 0;
-    const int64_t snz = lane < R ? ws.snnz[rq] : 0;
+    const int64_t ubl = !DIRECT && lane < R ? ws.ub[rq] : 0;
+    const int64_t snz = !DIRECT && lane < R ? ws.snnz[rq] : 0;
     RowOut out[R];
-    int cnt[R];
+    int cnt[R], P[R];
     bool fused[R];
     int64_t wave_total = 0;
     bool joint = true;  // every row single-pass or without products
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const int64_t ub = lane64(ubl, k);
         const int64_t na = lane64(ap, k + 1) - lane64(ap, k);
-        fused[k] = r0 + k < op.arows && ub > 0 && ub <= kFusedCap && na <= kFusedCap;
-        joint &= fused[k] || ub == 0 || r0 + k >= op.arows;
+        if constexpr (DIRECT) {
+            fused[k] = r0 + k < op.arows && na > 0;  // products are counted once staged
+            P[k] = 0;
+        } else {
+            const int64_t ub = lane64(ubl, k);
+            fused[k] = r0 + k < op.arows && ub > 0 && ub <= kFusedCap && na <= kFusedCap;
+            joint &= fused[k] || ub == 0 || r0 + k >= op.arows;
+            P[k] = (int)ub;
+            if (!fused[k] && r0 + k < op.arows && ub > 0) wave_total += lane64(snz, k);  // reserved for the numeric kernels
+        }
         cnt[k] = 0;
         out[k].exnz = 0;
         out[k].tot = 0;
-        if (!fused[k] && r0 + k < op.arows && ub > 0) wave_total += lane64(snz, k);  // reserved for the numeric kernels
     }
     const int64_t E = lane64(ap, R) - lane64(ap, 0);
     joint &= E <= kFusedCap;
     // First 64 entries of the wave's rows: row 0's metadata on either path.
     MetaReg<SMALLB> meta0;
-    if (fused[0] || joint) meta0 = load_meta<SMALLB>(op, ws, lane64(ap, 0), (int)min(E, (int64_t)kFusedCap), 0);
+    if (E > 0 && (fused[0] || joint))
+        meta0 = load_meta<SMALLB, DIRECT>(op, ws, lane64(ap, 0), (int)min(E, (int64_t)kFusedCap), 0, flags);
     if (joint) {
         // All rows' metadata staged at once and all rows' gathers in flight
         // together: one memory round trip for the rows of this wave.
         if (E > 0) {
-            stage_meta<SMALLB>(op, ws, w, lane64(ap, 0), (int)E, meta0);
+            stage_meta<SMALLB, DIRECT>(op, ws, w, lane64(ap, 0), (int)E, meta0, flags);
             RowItems it[R];
 #pragma unroll
-            for (int k = 0; k < R; ++k)
-                if (fused[k])
-                    expand_row<SMALLB>(op, w, (int)(lane64(ap, k) - lane64(ap, 0)),
-                                       (int)(lane64(ap, k + 1) - lane64(ap, k)), (int)lane64(ubl, k), it[k], flags);
+            for (int k = 0; k < R; ++k) {
+                const int m = (int)(lane64(ap, k) - lane64(ap, 0)), na = (int)(lane64(ap, k + 1) - lane64(ap, k));
+                if constexpr (DIRECT) {
+                    if (fused[k]) P[k] = uni(w.off[m + na] - w.off[m]);
+                    fused[k] = fused[k] && P[k] > 0;
+                }
+                if (fused[k]) expand_row<SMALLB>(op, w, m, na, P[k], it[k], flags);
+            }
 #pragma unroll
             for (int k = 0; k < R; ++k)
                 if (fused[k]) form_products(it[k]);
@@ -552,7 +582,7 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op,
             for (int k = 0; k < R; ++k) {
                 if (fused[k]) {
                     int dist = 0;
-                    cnt[k] = sort_fold<SMALLB>(w, (int)lane64(ubl, k), it[k], out[k], dist, op.keep_zeros);
+                    cnt[k] = sort_fold<SMALLB>(w, P[k], it[k], out[k], dist, op.keep_zeros);
                     distinct_sum += dist;
                     wave_total += cnt[k];
                 }
@@ -565,25 +595,45 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op,
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             if (k + 1 < R && fused[k + 1])
-                meta[k + 1] =
-                    load_meta<SMALLB>(op, ws, lane64(ap, k + 1), (int)(lane64(ap, k + 2) - lane64(ap, k + 1)), 0);
+                meta[k + 1] = load_meta<SMALLB, DIRECT>(op, ws, lane64(ap, k + 1),
+                                                        (int)(lane64(ap, k + 2) - lane64(ap, k + 1)), 0, flags);
             if (fused[k]) {
                 const int na = (int)(lane64(ap, k + 1) - lane64(ap, k));
-                stage_meta<SMALLB>(op, ws, w, lane64(ap, k), na, meta[k]);
-                RowItems it;
-                expand_row<SMALLB>(op, w, 0, na, (int)lane64(ubl, k), it, flags);
-                form_products(it);
-                wave_sync();
-                int dist = 0;
-                cnt[k] = sort_fold<SMALLB>(w, (int)lane64(ubl, k), it, out[k], dist, op.keep_zeros);
-                distinct_sum += dist;
-                wave_total += cnt[k];
+                stage_meta<SMALLB, DIRECT>(op, ws, w, lane64(ap, k), na, meta[k], flags);
+                if constexpr (DIRECT) {
+                    P[k] = uni(w.off[na] - w.off[0]);
+                    fused[k] = P[k] > 0;
+                }
+                if (fused[k]) {
+                    RowItems it;
+                    expand_row<SMALLB>(op, w, 0, na, P[k], it, flags);
+                    form_products(it);
+                    wave_sync();
+                    int dist = 0;
+                    cnt[k] = sort_fold<SMALLB>(w, P[k], it, out[k], dist, op.keep_zeros);
+                    distinct_sum += dist;
+                    wave_total += cnt[k];
+                } else {
+                    wave_sync();  // the staged metadata is dead before the next row stages
+                }
             }
         }
     }
     if (lane == 0) {
         s_wtot[wv] = wave_total;
         s_dist[wv] = distinct_sum;
+        if constexpr (DIRECT) {
+            int64_t st[NS] = {0, 0, 0, 0};  // products, rows without products, rows of W1, W2
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (r0 + k >= op.arows) continue;
+                const int64_t na = lane64(ap, k + 1) - lane64(ap, k);
+                st[0] += P[k];
+                st[P[k] == 0 ? 1 : (P[k] <= kCapW1 && na <= kCapW1 ? 2 : 3)] += 1;
+            }
+#pragma unroll
+            for (int f = 0; f < NS; ++f) s_dstat[wv][f] = st[f];
+        }
     }
     __syncthreads();
     if (wv == 0) {
@@ -593,7 +643,18 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op,
             agg += s_wtot[i];
             dist += s_dist[i];
         }
-        if (lane == 0) fo.distinct[t] = dist;
+        if (lane == 0) {
+            fo.distinct[t] = dist;
+            if constexpr (DIRECT) {
+#pragma unroll
+                for (int f = 0; f < NS; ++f) {
+                    int64_t v = 0;
+#pragma unroll
+                    for (int i = 0; i < NW; ++i) v += s_dstat[i][f];
+                    fo.distinct[(int64_t)(f + 1) * fo.ntiles + t] = v;
+                }
+            }
+        }
         const int64_t excl = tile_lookback(fo.status, t, agg, flags);
         if (lane == 0) {
             s_base = excl;
@@ -633,13 +694,13 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op,
                 before += (out[k].tot >> (8 * j)) & 0xffu;
             }
             rp += cnt[k];
-        } else if (r0 + k < op.arows && lane64(ubl, k) > 0) {
+        } else if (!DIRECT && r0 + k < op.arows && lane64(ubl, k) > 0) {
             rp += lane64(snz, k);
         }
     }
     if (lane < R && r0 + lane < op.arows) {
         fo.rowptr[r0 + lane] = my_rp;
-        bool fl = false;
+        bool fl = DIRECT;
 #pragma unroll
         for (int k = 0; k < R; ++k) fl |= (lane == k) & fused[k];
         if (fl) ws.annz[r0 + lane] = my_cnt;
@@ -721,7 +782,7 @@ __global__ __launch_bounds__(kFusedDealTpb) void k_fused_dealt(Operands op, Work
     MetaReg<SMALLB> meta0;
     if (fused[0]) {
         if (joint) meta0 = load_meta2<SMALLB>(op, ws, a0[0], na[0], a0[1], E, 0);
-        else meta0 = load_meta<SMALLB>(op, ws, a0[0], na[0], 0);
+        else meta0 = load_meta<SMALLB>(op, ws, a0[0], na[0], 0, flags);
     }
     if (joint) {
         stage_meta2<SMALLB>(op, ws, w, a0[0], na[0], a0[1], E, meta0);
@@ -746,9 +807,9 @@ __global__ __launch_bounds__(kFusedDealTpb) void k_fused_dealt(Operands op, Work
         meta[0] = meta0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            if (k + 1 < R && fused[k + 1]) meta[k + 1] = load_meta<SMALLB>(op, ws, a0[k + 1], na[k + 1], 0);
+            if (k + 1 < R && fused[k + 1]) meta[k + 1] = load_meta<SMALLB>(op, ws, a0[k + 1], na[k + 1], 0, flags);
             if (fused[k]) {
-                stage_meta<SMALLB>(op, ws, w, a0[k], na[k], meta[k]);
+                stage_meta<SMALLB>(op, ws, w, a0[k], na[k], meta[k], flags);
                 RowItems it;
                 expand_row<SMALLB>(op, w, 0, na[k], ubk[k], it, flags);
                 form_products(it);
@@ -949,7 +1010,7 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused_packed(Opera
     MetaReg<SMALLB> meta0;
     if (fused[0]) {
         if (joint) meta0 = load_meta2<SMALLB>(op, ws, a0[0], na[0], a0[1], E, 0);
-        else meta0 = load_meta<SMALLB>(op, ws, a0[0], na[0], 0);
+        else meta0 = load_meta<SMALLB>(op, ws, a0[0], na[0], 0, flags);
     }
     if (joint) {
         stage_meta2<SMALLB>(op, ws, w, a0[0], na[0], a0[1], E, meta0);
@@ -974,9 +1035,9 @@ __global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused_packed(Opera
         meta[0] = meta0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            if (k + 1 < R && fused[k + 1]) meta[k + 1] = load_meta<SMALLB>(op, ws, a0[k + 1], na[k + 1], 0);
+            if (k + 1 < R && fused[k + 1]) meta[k + 1] = load_meta<SMALLB>(op, ws, a0[k + 1], na[k + 1], 0, flags);
             if (fused[k]) {
-                stage_meta<SMALLB>(op, ws, w, a0[k], na[k], meta[k]);
+                stage_meta<SMALLB>(op, ws, w, a0[k], na[k], meta[k], flags);
                 RowItems it;
                 expand_row<SMALLB>(op, w, 0, na[k], ubk[k], it, flags);
                 form_products(it);
@@ -1065,6 +1126,19 @@ void launch_pack_tiles(const int64_t *part, int64_t nblocks, int64_t *tcnt, int6
     hipLaunchKernelGGL(k_pack_counts, dim3(g), dim3(256), 0, stream, part, nblocks, tcnt);
     launch_scan(tcnt, toff, nblocks, tmp, &ctrl->packed_tiles, stream);
     hipLaunchKernelGGL(k_pack_map, dim3(g), dim3(256), 0, stream, tcnt, toff, nblocks, tmap);
+}
+
+void launch_fused_direct(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b,
+                         int rows_per_wave, hipStream_t stream) {
+    if (op.arows <= 0 || fo.ntiles <= 0) return;
+    const dim3 grid((unsigned)fo.ntiles), block(kFusedTpb);
+    if (rows_per_wave == 1) {
+        if (small_b) hipLaunchKernelGGL((k_fused<true, 1, true>), grid, block, 0, stream, op, ws, fo);
+        else hipLaunchKernelGGL((k_fused<false, 1, true>), grid, block, 0, stream, op, ws, fo);
+    } else {
+        if (small_b) hipLaunchKernelGGL((k_fused<true, 2, true>), grid, block, 0, stream, op, ws, fo);
+        else hipLaunchKernelGGL((k_fused<false, 2, true>), grid, block, 0, stream, op, ws, fo);
+    }
 }
 
 int64_t fused_tiles(int32_t rows, int rows_per_wave) {

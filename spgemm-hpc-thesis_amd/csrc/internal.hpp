@@ -125,6 +125,8 @@ struct Control {
     int large_next[3];              // next large row to take: count, HBM-window values, compact values
     int pad_;
     unsigned long long packed_tiles;  // tiles of the packed single-pass kernel
+    unsigned long long max_a_row;     // direct single pass: longest row of A
+    unsigned long long max_b_row;     // direct single pass: longest row of B
 };
 
 // Per-block partial sums of the analyze pass, reduced by one small kernel
@@ -200,6 +202,12 @@ void launch_zero_control(Control *ctrl, hipStream_t stream);
 int64_t analyze_blocks(int32_t rows);
 void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream);
 void launch_dir32(const int64_t *bptr, int64_t n, uint32_t *out, hipStream_t stream);
+// The longest rows of A (arows rows) and B (brows rows) into
+// ctrl->max_a_row / max_b_row (cleared with the control block), and B's row
+// offsets in 32 bits into dir32 (brows + 1 entries; B has fewer than 2^32
+// entries).
+void launch_rowmax(const int64_t *aptr, int32_t arows, const int64_t *bptr, int32_t brows, uint32_t *dir32,
+                   Control *ctrl, hipStream_t stream);
 // dst = sum of in[0..n) (one workgroup; scan.hip).
 // *dst += sum of in[0, n) (dst is cleared by the caller, e.g. with the control block).
 void launch_sum(const int64_t *in, int64_t n, unsigned long long *dst, hipStream_t stream);
@@ -233,6 +241,15 @@ void launch_pack_tiles(const int64_t *part, int64_t nblocks, int64_t *tcnt, int6
                        int64_t *tmp, Control *ctrl, hipStream_t stream);
 void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b, int rows_per_wave,
                   hipStream_t stream);
+// Direct single pass (no analysis): every row has at most kFusedCap products
+// and A entries, which the host checked from the longest rows of A and B.
+// Each tile reads its A entries' columns and B's 32-bit row offsets
+// (ws.bdir32) itself.  fo.distinct holds kDirectFields x ntiles counters:
+// field f of tile t at [f * ntiles + t] (distinct columns, scalar products,
+// rows without products, rows of bin W1, rows of bin W2).
+constexpr int kDirectFields = 5;
+void launch_fused_direct(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b,
+                         int rows_per_wave, hipStream_t stream);
 // Exclusive scan of n int64 counts into out[0..n]; out[n] = total; total also
 // stored to *total_dst when non-null (scan.hip).
 size_t scan_tmp_elems(int64_t n);

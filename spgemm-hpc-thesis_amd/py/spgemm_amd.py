@@ -84,13 +84,15 @@ class Stats(C.Structure):
     _fields_ = [("products", C.c_int64), ("structural_nnz", C.c_int64), ("nnz", C.c_int64),
                 ("rows_per_bin", C.c_int64 * 8), ("compacted", C.c_int32), ("bins", C.c_int32),
                 ("single_pass", C.c_int32), ("wide_format", C.c_int32),
-                ("ms_total", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double)]
+                ("ms_total", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double),
+                ("direct", C.c_int32), ("reserved_", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {"products": self.products, "structural_nnz": self.structural_nnz, "nnz": self.nnz,
                 "rows_per_bin": list(self.rows_per_bin)[: self.bins], "compacted": bool(self.compacted),
                 "single_pass": bool(self.single_pass), "wide_format": bool(self.wide_format),
-                "ms_total": self.ms_total, "ms_symbolic": self.ms_symbolic, "ms_numeric": self.ms_numeric}
+                "ms_total": self.ms_total, "ms_symbolic": self.ms_symbolic, "ms_numeric": self.ms_numeric,
+                "direct": bool(self.direct)}
 
 
 class FullExample(C.Structure):

@@ -67,6 +67,13 @@ def test_golden_products_bit_exact(path_engine, golden):
         assert c.nnz == g["c_nnz"], name
         assert digest(c) == g["digest"], name
         assert st["wide_format"] == (engine._format == S.FORMAT_WIDE), name
+        # AUTO skips the per-row analysis exactly when the longest rows of A
+        # and B prove every row single-pass (and A has at most 2^20 entries)
+        ma = int(np.diff(a.row_ptr).max()) if a.rows else 0
+        mb = int(np.diff(b.row_ptr).max()) if b.rows else 0
+        want_direct = (engine._path == S.PATH_AUTO and a.rows > 0 and b.rows > 0 and ma <= 256
+                       and ma * mb <= 256 and a.nnz <= (1 << 20))
+        assert st["direct"] == want_direct, name
         if a.nnz:
             # small products take the single pass unless two-phase is pinned
             assert st["single_pass"] == (engine._path != S.PATH_TWO_PHASE), name
