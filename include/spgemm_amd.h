@@ -251,6 +251,15 @@ spg_status spg_batch_create_chained(spg_context *ctx, const spg_batch *src, int3
 spg_status spg_batch_refresh(spg_batch *b);
 /* Run every product; *nnz_total receives the entries of all C_i. */
 spg_status spg_batch_run(spg_batch *b, int64_t *nnz_total);
+/* Run only the count phase (analyze and symbolic kernels) of the next run:
+ * *value_bytes receives the device bytes its value phase will add to the
+ * batch's buffers (C's entries at 12 B each, exact for rows the symbolic
+ * kernels count, the work bound for the single-pass rows, plus the large
+ * rows' accumulator windows and the single-pass tile state, less what the
+ * buffers already hold from earlier runs).  The next spg_batch_run continues
+ * from these counts (eagerly, without a captured graph).  For admission by
+ * C's size before it is allocated. */
+spg_status spg_batch_count(spg_batch *b, uint64_t *value_bytes);
 /* After a first run, replay the planned pipeline from a captured HIP graph
  * (no host synchronisation inside the pipeline). */
 spg_status spg_batch_set_graph(spg_batch *b, int enabled);
@@ -269,6 +278,21 @@ spg_status spg_batch_last_stats(const spg_batch *b, spg_stats *out);
 spg_status spg_batch_destroy(spg_batch *b);
 /* Run the batch on `stream` (NULL: the creating context's stream). */
 spg_status spg_batch_set_stream(spg_batch *b, void *stream);
+/* Device bytes the batch holds: its arena, workspace and result buffers
+ * (buffers grow and are kept across runs and rebuilds). */
+spg_status spg_batch_device_bytes(const spg_batch *b, uint64_t *bytes);
+/* Free every device buffer of the batch; it holds no products afterwards
+ * and is reused with spg_batch_rebuild (not for chained batches). */
+spg_status spg_batch_trim(spg_batch *b);
+/* Device bytes that building products of these total sizes into `reuse`
+ * (NULL: a new batch of ctx) and running them with C's capacity at c_entries
+ * would add to the batch's buffers: its arena (rows and a_nnz of the stacked
+ * A_i, b_rows and b_nnz of the stacked B_i), its count workspace (cols: the
+ * largest C column count) and C (none when c_entries is 0: the count phase
+ * alone).  The large rows' accumulator windows, the
+ * single-pass tile state and compaction buffers are not included. */
+spg_status spg_batch_estimate(spg_context *ctx, const spg_batch *reuse, int64_t rows, int64_t a_nnz, int64_t b_rows,
+                              int64_t b_nnz, int32_t cols, int64_t c_entries, uint64_t *bytes);
 /* Message of the batch's last failure (the batch computes on its own
  * context, so its errors are not reported on the creating one). */
 const char *spg_batch_last_error(const spg_batch *b);
@@ -276,17 +300,22 @@ const char *spg_batch_last_error(const spg_batch *b);
 /* ---- served products ------------------------------------------------------
  * No counterpart in the reference; the serving mode of the engine.  Requests
  * (A, B) are submitted from any host thread and copied; a builder thread
- * admits them in arrival order while their memory estimate (operands + 12 B
- * per scalar product, the symbolic phase's bound on C) fits budget_bytes,
- * packs them into batch arenas and uploads them on its own stream; a runner
- * thread computes the batches on a second stream (continuous batching: the
- * next batch is admitted and uploaded while the current one computes).
- * Results stay in device memory, charged to the budget, until released; each
- * is bit-identical to computing the product alone. */
+ * admits them in arrival order while the bytes known at submission (operand
+ * copies and the count workspace) fit budget_bytes, packs them into batch
+ * arenas, uploads them and runs the count phase (analyze and symbolic kernels)
+ * on its own stream, then charges C's bytes as the counts give them before C
+ * is allocated, waiting for releases until they fit (a batch that cannot fit
+ * is split; a single product that cannot fails with SPG_ENOMEM).  A runner
+ * thread computes the values on a second stream (continuous batching: the
+ * next batch is admitted, uploaded and counted while the current one
+ * computes).  Results stay in device memory, charged to the budget, until
+ * released; each is bit-identical to computing the product alone. */
 typedef struct spg_server spg_server;
 spg_status spg_server_create(int device, uint64_t budget_bytes, int max_batch, spg_server **out);
-/* Host views (copied); SPG_ENOMEM when the request alone exceeds the budget,
- * SPG_ERANGE when an A column lies outside B's rows. */
+/* Host views (copied); SPG_ENOMEM when the request's operands and count
+ * workspace alone exceed the budget, SPG_ERANGE when an A column lies outside
+ * B's rows.  spg_server_wait reports SPG_ENOMEM for a request whose counted
+ * result cannot fit the budget. */
 spg_status spg_server_submit(spg_server *s, const spg_csr_view *A, const spg_csr_view *B, int64_t *request_id);
 /* Register a matrix once (uploaded, resident, charged to the budget) and
  * submit products of registered matrices by handle. */

@@ -99,6 +99,8 @@ struct spg_batch {
     DevBuf src_row, col_shift, bad;
     ProductPlan plan;
     bool planned = false;
+    int counted = 0;  // 1: the count phase ran (counts in `counts`); 2: it finished the product (direct pass)
+    Control counts{};
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -133,6 +135,9 @@ spg_status new_batch(spg_context *ctx, spg_batch **out) {
         return set_err(ctx, s, "batch context");
     }
     b->parent = ctx;
+    b->own->for_each_buffer([](DevBuf &d) { d.headroom = true; });
+    DevBuf *arena[] = {&b->a_ptr, &b->a_col, &b->a_val, &b->b_ptr, &b->b_col, &b->b_val};
+    for (DevBuf *d : arena) d->headroom = true;
     b->own->stream = ctx->stream;
     b->own->path = ctx->path;
     b->own->fused_rows = ctx->fused_rows;
@@ -315,6 +320,7 @@ namespace {
 spg_status build_arena(spg_context *ctx, spg_batch *b, int32_t n, const spg_csr_view *A, const spg_csr_view *B) {
     b->drop_graph();
     b->planned = false;
+    b->counted = 0;
     b->plan = ProductPlan{};
     b->n = n;
     b->a_row0.clear();
@@ -462,6 +468,7 @@ spg_status spg_batch_refresh(spg_batch *b) {
     if (!b->src) return set_err(ctx, SPG_ESTATE, "not a chained batch");
     const spg_context *sc = b->src->own;
     if (!sc->have_result) return set_err(ctx, SPG_ESTATE, "the source batch has not run");
+    b->counted = 0;  // A's columns change: counts from before are void
     launch_zero_bytes(b->bad.p, 8, ctx->stream);
     if (b->total_rows > 0) {
         const int64_t blocks = std::min<int64_t>((b->total_rows * 64 + 255) / 256, 65536);
@@ -482,7 +489,12 @@ spg_status spg_batch_run(spg_batch *b, int64_t *nnz_total) {
     if (!b || !nnz_total) return SPG_EINVAL;
     spg_context *ctx = b->own;
     spg_status s;
-    if (!b->planned || !b->use_graph) {
+    if (b->counted) {
+        const int counted = b->counted;
+        b->counted = 0;
+        if (counted == 1 && (s = value_product(ctx, b->plan, b->counts)) != SPG_OK) return s;
+        b->planned = true;
+    } else if (!b->planned || !b->use_graph) {
         if ((s = run_product(ctx, b->plan)) != SPG_OK) return s;
         b->planned = true;
     } else {
@@ -501,6 +513,87 @@ spg_status spg_batch_run(spg_batch *b, int64_t *nnz_total) {
         }
     }
     *nnz_total = ctx->res_nnz;
+    return SPG_OK;
+}
+
+spg_status spg_batch_count(spg_batch *b, uint64_t *value_bytes_out) {
+    if (!b || !value_bytes_out) return SPG_EINVAL;
+    spg_context *ctx = b->own;
+    b->counted = 0;
+    bool done = false;
+    spg_status s = count_product(ctx, b->plan, b->counts, done);
+    if (s != SPG_OK) return s;
+    b->counted = done ? 2 : 1;
+    *value_bytes_out = done ? 0 : value_bytes(ctx, b->plan, b->counts);
+    return SPG_OK;
+}
+
+spg_status spg_batch_device_bytes(const spg_batch *b, uint64_t *bytes) {
+    if (!b || !bytes) return SPG_EINVAL;
+    uint64_t n = b->own->device_bytes();
+    const DevBuf *bufs[] = {&b->a_ptr, &b->a_col, &b->a_val, &b->b_ptr, &b->b_col, &b->b_val,
+                            &b->src_row, &b->col_shift, &b->bad};
+    for (const DevBuf *d : bufs) n += d->bytes;
+    *bytes = n;
+    return SPG_OK;
+}
+
+spg_status spg_batch_estimate(spg_context *ctx, const spg_batch *reuse, int64_t rows, int64_t a_nnz, int64_t b_rows,
+                              int64_t b_nnz, int32_t cols, int64_t c_entries, uint64_t *bytes) {
+    if (!ctx) return SPG_EINVAL;
+    if (!bytes || rows < 0 || rows > INT32_MAX || a_nnz < 0 || b_rows < 0 || b_rows > INT32_MAX || b_nnz < 0 ||
+        cols < 0 || c_entries < 0)
+        return set_err(ctx, SPG_EINVAL, "invalid batch sizes");
+    spg_batch *b = const_cast<spg_batch *>(reuse);  // read only: the buffers' current sizes
+    spg_context fresh;                               // a new batch's (empty) buffers
+    fresh.total_mem = ctx->total_mem;
+    fresh.cus = ctx->cus;
+    spg_context *own = b ? b->own : &fresh;
+    DevBuf none[6];
+    DevBuf *arena[6] = {&none[0], &none[1], &none[2], &none[3], &none[4], &none[5]};
+    if (b) {
+        DevBuf *mine[6] = {&b->a_ptr, &b->a_col, &b->a_val, &b->b_ptr, &b->b_col, &b->b_val};
+        std::copy(mine, mine + 6, arena);
+    }
+    BufferNeeds v;
+    v.add(*arena[0], (size_t)(rows + 1) * 8, "");
+    v.add(*arena[1], (size_t)(a_nnz ? a_nnz : 1) * 4, "");
+    v.add(*arena[2], (size_t)(a_nnz ? a_nnz : 1) * 8, "");
+    v.add(*arena[3], (size_t)(b_rows + 1) * 8, "");
+    v.add(*arena[4], (size_t)(b_nnz ? b_nnz : 1) * 4, "");
+    v.add(*arena[5], (size_t)(b_nnz ? b_nnz : 1) * 8, "");
+    int64_t lwidth = 0;
+    int lblocks = 0;
+    workspace_needs(own, (int32_t)rows, a_nnz, (int32_t)b_rows, b_nnz < (int64_t(1) << 32) && b_rows > 0, cols, v,
+                    lwidth, lblocks);
+    if (c_entries > 0) {
+        v.add(own->ccol[0], (size_t)c_entries * 4, "");
+        v.add(own->cval[0], (size_t)c_entries * 8, "");
+    }
+    *bytes = v.growth();
+    return SPG_OK;
+}
+
+spg_status spg_batch_trim(spg_batch *b) {
+    if (!b) return SPG_EINVAL;
+    if (b->src) return set_err(b->own, SPG_ESTATE, "a chained batch cannot be trimmed");
+    spg_context *ctx = b->own;
+    SPG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    SPG_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    b->drop_graph();
+    DevBuf *bufs[] = {&b->a_ptr, &b->a_col, &b->a_val, &b->b_ptr, &b->b_col, &b->b_val};
+    for (DevBuf *d : bufs) d->release();
+    ctx->for_each_buffer([](DevBuf &d) { d.release(); });
+    ctx->lbits_width = 0;
+    ctx->have_result = false;
+    b->n = 0;
+    b->a_row0.clear();
+    b->rows.clear();
+    b->cols.clear();
+    b->total_rows = b->a_nnz = b->b_rows = b->b_nnz = 0;
+    b->planned = false;
+    b->counted = 0;
+    b->plan = ProductPlan{};
     return SPG_OK;
 }
 
@@ -533,6 +626,7 @@ spg_status spg_batch_autotune(spg_batch *b, int repeats, int *choice) {
                                 {SPG_PATH_SINGLE_PASS, kFusedPackRows}};
     constexpr int kVariants = (int)(sizeof(variants) / sizeof(variants[0]));
     const bool was_graph = b->use_graph;
+    b->counted = 0;
     b->use_graph = false;
     b->drop_graph();
     float best = 0;

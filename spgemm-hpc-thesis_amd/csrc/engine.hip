@@ -64,21 +64,55 @@ spg_status flag_error(spg_context *ctx, unsigned long long flags) {
     return SPG_OK;
 }
 
+void workspace_needs(spg_context *ctx, int32_t rows, int64_t nnz_a, int32_t ndir, bool dir32, int64_t cols,
+                     BufferNeeds &v, int64_t &lwidth, int &lblocks) {
+    const size_t R = rows > 0 ? (size_t)rows : 1;
+    const size_t E = nnz_a > 0 ? (size_t)nnz_a : 1;
+    v.add(ctx->ub, R * 8, "hipMalloc(ub)");
+    v.add(ctx->ebst, E * 8, "hipMalloc(entry starts)");
+    v.add(ctx->elen, E * 4, "hipMalloc(entry lengths)");
+    v.add(ctx->bin_rows, R * sizeof(RowRec) * kNumBins, "hipMalloc(bin lists)");
+    v.add(ctx->snnz, R * 8, "hipMalloc(snnz)");
+    v.add(ctx->annz, R * 8, "hipMalloc(annz)");
+    v.add(ctx->scan_tmp, scan_tmp_elems(rows) * 8, "hipMalloc(scan)");
+    v.add(ctx->ctrl, sizeof(Control), "hipMalloc(ctrl)");
+    v.add(ctx->part, (size_t)(analyze_blocks(rows) + 1) * kAnalyzeFields * 8, "hipMalloc(partials)");
+    v.add(ctx->rowptr[0], ((size_t)rows + 1) * 8, "hipMalloc(row_ptr C)");
+    if (dir32) v.add(ctx->dir32, ((size_t)ndir + 1) * 4, "hipMalloc(B directory)");
+    // Large-row path: one column window and one presence bitmap per resident
+    // workgroup (kept all-zero between rows); the accumulators are allocated
+    // only when large rows exist.
+    int64_t width = (cols + 31) / 32 * 32;
+    const int64_t kMaxWindow = int64_t(1) << 22;
+    if (width > kMaxWindow) width = kMaxWindow;
+    if (width < 32) width = 32;
+    const size_t per_block = (size_t)width * 8 + (size_t)width / 8;
+    size_t budget = ctx->total_mem / 16;
+    if (budget > (size_t(4) << 30)) budget = size_t(4) << 30;
+    size_t nb = budget / per_block;
+    const size_t cap = (size_t)ctx->cus * 2;
+    if (nb > cap) nb = cap;
+    if (nb < 1) nb = 1;
+    v.add(ctx->lbits, nb * (size_t)(width / 32) * 4, "hipMalloc(large bitmaps)");
+    lwidth = width;
+    lblocks = (int)nb;
+}
+
 spg_status plan_workspace(spg_context *ctx, ProductPlan &p) {
     hipStream_t st = ctx->stream;
-    const int32_t rows = p.rows;
-    const size_t R = rows > 0 ? (size_t)rows : 1;
-    SPG_HIP(ctx->ub.ensure(R * 8, st), "hipMalloc(ub)");
-    const size_t E = p.op.nnz_a > 0 ? (size_t)p.op.nnz_a : 1;
-    SPG_HIP(ctx->ebst.ensure(E * 8, st), "hipMalloc(entry starts)");
-    SPG_HIP(ctx->elen.ensure(E * 4, st), "hipMalloc(entry lengths)");
-    SPG_HIP(ctx->bin_rows.ensure(R * sizeof(RowRec) * kNumBins, st), "hipMalloc(bin lists)");
-    SPG_HIP(ctx->snnz.ensure(R * 8, st), "hipMalloc(snnz)");
-    SPG_HIP(ctx->annz.ensure(R * 8, st), "hipMalloc(annz)");
-    SPG_HIP(ctx->scan_tmp.ensure(scan_tmp_elems(rows) * 8, st), "hipMalloc(scan)");
-    SPG_HIP(ctx->ctrl.ensure(sizeof(Control), st), "hipMalloc(ctrl)");
-    SPG_HIP(ctx->part.ensure((size_t)(analyze_blocks(rows) + 1) * kAnalyzeFields * 8, st), "hipMalloc(partials)");
-    SPG_HIP(ctx->rowptr[0].ensure(((size_t)rows + 1) * 8, st), "hipMalloc(row_ptr C)");
+    // B's row_ptr as the directory (bend = bstart + 1) with 32-bit entry
+    // indices: analyze gathers a 4-byte copy of it (half the cache lines).
+    const bool dir32 = p.b_span < (int64_t(1) << 32) && p.op.ndir > 0 && p.op.bend == p.op.bstart + 1;
+    BufferNeeds v;
+    int64_t width = 0;
+    int nb = 0;
+    workspace_needs(ctx, p.rows, p.op.nnz_a, p.op.ndir, dir32, p.cols, v, width, nb);
+    ctx->lbits.zero_on_grow = true;
+    if (ctx->lbits_width != width) {  // bitmaps are laid out per window width
+        ctx->lbits.release();
+        ctx->lbits_width = width;
+    }
+    for (int i = 0; i < v.n; ++i) SPG_HIP(v.buf[i]->ensure(v.bytes[i], st), v.what[i]);
 
     Workspace &ws = p.ws;
     ws = Workspace{};
@@ -91,40 +125,15 @@ spg_status plan_workspace(spg_context *ctx, ProductPlan &p) {
     ws.elen = ctx->elen.as<int32_t>();
     ws.ub = ctx->ub.as<int64_t>();
     ws.bin_recs = ctx->bin_rows.as<RowRec>();
-    ws.list_stride = (int64_t)R;
+    ws.list_stride = (int64_t)(p.rows > 0 ? p.rows : 1);
     ws.snnz = ctx->snnz.as<int64_t>();
     ws.annz = ctx->annz.as<int64_t>();
     ws.scan_tmp = ctx->scan_tmp.as<int64_t>();
     ws.ctrl = ctx->ctrl.as<Control>();
     ws.part = ctx->part.as<int64_t>();
-    // B's row_ptr as the directory (bend = bstart + 1) with 32-bit entry
-    // indices: analyze gathers a 4-byte copy of it (half the cache lines).
-    if (p.b_span < (int64_t(1) << 32) && p.op.ndir > 0 && p.op.bend == p.op.bstart + 1) {
-        SPG_HIP(ctx->dir32.ensure(((size_t)p.op.ndir + 1) * 4, st), "hipMalloc(B directory)");
-        ws.bdir32 = ctx->dir32.as<uint32_t>();
-    }
-    // Large-row path: one column window and one presence bitmap per resident
-    // workgroup (kept all-zero between rows); the accumulators are allocated
-    // only when large rows exist.
-    int64_t width = ((int64_t)p.cols + 31) / 32 * 32;
-    const int64_t kMaxWindow = int64_t(1) << 22;
-    if (width > kMaxWindow) width = kMaxWindow;
-    if (width < 32) width = 32;
+    if (dir32) ws.bdir32 = ctx->dir32.as<uint32_t>();
     ws.lwidth = width;
-    const size_t per_block = (size_t)width * 8 + (size_t)width / 8;
-    size_t budget = ctx->total_mem / 16;
-    if (budget > (size_t(4) << 30)) budget = size_t(4) << 30;
-    size_t nb = budget / per_block;
-    const size_t cap = (size_t)ctx->cus * 2;
-    if (nb > cap) nb = cap;
-    if (nb < 1) nb = 1;
-    ws.lblocks = (int)nb;
-    ctx->lbits.zero_on_grow = true;
-    if (ctx->lbits_width != width) {  // bitmaps are laid out per window width
-        ctx->lbits.release();
-        ctx->lbits_width = width;
-    }
-    SPG_HIP(ctx->lbits.ensure(nb * (size_t)(width / 32) * 4, st), "hipMalloc(large bitmaps)");
+    ws.lblocks = nb;
     ws.lbits = ctx->lbits.as<uint32_t>();
     ws.lacc = ctx->lacc.as<double>();
     return SPG_OK;
@@ -166,38 +175,63 @@ spg_status counts_compact(spg_context *ctx, ProductPlan &p, unsigned bins, bool 
     return SPG_OK;
 }
 
-spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h) {
-    hipStream_t st = ctx->stream;
-    for (int b = 0; b < 8; ++b) p.bin_count[b] = h.bin_count[b];
-    p.capacity = p.fused ? (int64_t)(h.ub_small + h.sym_total) : (int64_t)h.structural;
-    SPG_HIP(ctx->ccol[0].ensure((size_t)p.capacity * 4, st), "hipMalloc(C col)");
-    SPG_HIP(ctx->cval[0].ensure((size_t)p.capacity * 8, st), "hipMalloc(C val)");
-    if (h.bin_count[kBinLarge] > 0) {
-        SPG_HIP(ctx->lacc.ensure((size_t)p.ws.lblocks * (size_t)p.ws.lwidth * 8, st), "hipMalloc(large accumulators)");
-        p.ws.lacc = ctx->lacc.as<double>();
-    }
-    p.fo = FusedOut{};
+namespace {
+
+// The buffers the value phase of a counted plan needs, with their sizes.
+struct ValueBuffers : BufferNeeds {
+    int64_t capacity = 0, ntiles = 0, pack_blocks = 0;
+};
+
+ValueBuffers value_buffers(spg_context *ctx, const ProductPlan &p, const Control &h) {
+    ValueBuffers v;
+    v.capacity = p.fused ? (int64_t)(h.ub_small + h.sym_total) : (int64_t)h.structural;
+    v.add(ctx->ccol[0], (size_t)v.capacity * 4, "hipMalloc(C col)");
+    v.add(ctx->cval[0], (size_t)v.capacity * 8, "hipMalloc(C val)");
+    if (h.bin_count[kBinLarge] > 0)
+        v.add(ctx->lacc, (size_t)p.ws.lblocks * (size_t)p.ws.lwidth * 8, "hipMalloc(large accumulators)");
     if (p.fused) {
         if (p.fused_rows == kFusedPackRows) {
             // Packed tiles: at most one per block plus one per kFusedPackCap
             // single-pass rows; the device counts them exactly.
-            p.pack_blocks = analyze_blocks(p.rows);
+            v.pack_blocks = analyze_blocks(p.rows);
             const int64_t nf = (int64_t)h.bin_rows[kBinW1] + (int64_t)h.bin_rows[kBinW2];
-            p.fo.ntiles = p.rows > 0 ? p.pack_blocks + (nf + kFusedPackCap - 1) / kFusedPackCap : 0;
-            const size_t nb = (size_t)(p.pack_blocks > 0 ? p.pack_blocks : 1);
-            SPG_HIP(ctx->pack_cnt.ensure(nb * 8, st), "hipMalloc(packed tile counts)");
-            SPG_HIP(ctx->pack_off.ensure(nb * 8, st), "hipMalloc(packed tile offsets)");
-            SPG_HIP(ctx->pack_map.ensure((size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1) * 8, st),
-                    "hipMalloc(packed tile map)");
+            v.ntiles = p.rows > 0 ? v.pack_blocks + (nf + kFusedPackCap - 1) / kFusedPackCap : 0;
+            const size_t nb = (size_t)(v.pack_blocks > 0 ? v.pack_blocks : 1);
+            v.add(ctx->pack_cnt, nb * 8, "hipMalloc(packed tile counts)");
+            v.add(ctx->pack_off, nb * 8, "hipMalloc(packed tile offsets)");
+            v.add(ctx->pack_map, (size_t)(v.ntiles > 0 ? v.ntiles : 1) * 8, "hipMalloc(packed tile map)");
+        } else {
+            v.ntiles = fused_tiles(p.rows, p.fused_rows);
+        }
+        const size_t nt = (size_t)(v.ntiles > 0 ? v.ntiles : 1);
+        v.add(ctx->tile_status, nt * kStatusStride * 8, "hipMalloc(tile status)");
+        v.add(ctx->tile_distinct, nt * 8, "hipMalloc(tile counts)");
+    }
+    return v;
+}
+
+}  // namespace
+
+uint64_t value_bytes(spg_context *ctx, const ProductPlan &p, const Control &h) {
+    return value_buffers(ctx, p, h).growth();
+}
+
+spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h) {
+    hipStream_t st = ctx->stream;
+    for (int b = 0; b < 8; ++b) p.bin_count[b] = h.bin_count[b];
+    const ValueBuffers v = value_buffers(ctx, p, h);
+    for (int i = 0; i < v.n; ++i) SPG_HIP(v.buf[i]->ensure(v.bytes[i], st), v.what[i]);
+    p.capacity = v.capacity;
+    if (h.bin_count[kBinLarge] > 0) p.ws.lacc = ctx->lacc.as<double>();
+    p.fo = FusedOut{};
+    if (p.fused) {
+        p.fo.ntiles = v.ntiles;
+        if (p.fused_rows == kFusedPackRows) {
+            p.pack_blocks = v.pack_blocks;
             p.pack_cnt = ctx->pack_cnt.as<int64_t>();
             p.pack_off = ctx->pack_off.as<int64_t>();
             p.fo.tmap = ctx->pack_map.as<int64_t>();
-        } else {
-            p.fo.ntiles = fused_tiles(p.rows, p.fused_rows);
         }
-        const size_t nt = (size_t)(p.fo.ntiles > 0 ? p.fo.ntiles : 1);
-        SPG_HIP(ctx->tile_status.ensure(nt * kStatusStride * 8, st), "hipMalloc(tile status)");
-        SPG_HIP(ctx->tile_distinct.ensure(nt * 8, st), "hipMalloc(tile counts)");
         p.fo.rowptr = ctx->rowptr[0].as<int64_t>();
         p.fo.ccol = ctx->ccol[0].as<int32_t>();
         p.fo.cval = ctx->cval[0].as<double>();
@@ -353,16 +387,13 @@ spg_status try_direct(spg_context *ctx, ProductPlan &p, bool &done) {
     return SPG_OK;
 }
 
-// The whole product on the device.  On success the result is
-// rowptr[cur], ccol[cur], cval[cur] with res_nnz entries.
-spg_status run_product(spg_context *ctx, ProductPlan &p) {
+spg_status count_product(spg_context *ctx, ProductPlan &p, Control &h, bool &done) {
     hipStream_t st = ctx->stream;
     ctx->have_result = false;
-    spg_stats stats{};
     spg_status s;
     p.fused_rows = ctx->fused_rows;
     if ((s = plan_workspace(ctx, p)) != SPG_OK) return s;
-    bool done = false;
+    done = false;
     if ((s = try_direct(ctx, p, done)) != SPG_OK || done) return s;
     if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
     // Single-pass path: only the rows of the big bins are listed and counted
@@ -373,7 +404,7 @@ spg_status run_product(spg_context *ctx, ProductPlan &p) {
     // fold in LDS; their count then saves their bitmaps.
     p.compact = compact_large_available(p.ws.lwidth) && (int64_t)p.cols <= p.ws.lwidth;
     p.lsave = nullptr;
-    Control h{};
+    h = Control{};
     if (p.compact) {
         if ((s = counts_compact(ctx, p, try_fused ? kBigBins : kAllBins, !try_fused, h)) != SPG_OK) return s;
         if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
@@ -405,12 +436,19 @@ spg_status run_product(spg_context *ctx, ProductPlan &p) {
             if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
         }
     }
-    if ((s = plan_values(ctx, p, h)) != SPG_OK) return s;
+    return SPG_OK;
+}
+
+spg_status value_product(spg_context *ctx, ProductPlan &p, const Control &counted) {
+    hipStream_t st = ctx->stream;
+    spg_stats stats{};
+    spg_status s;
+    if ((s = plan_values(ctx, p, counted)) != SPG_OK) return s;
     if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");
     enqueue_values(ctx, p);
     if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[3], st), "hipEventRecord");
     if ((s = read_control(ctx)) != SPG_OK) return s;
-    h = *ctx->h_ctrl;
+    const Control h = *ctx->h_ctrl;
     if ((s = finish_product(ctx, p, h, stats)) != SPG_OK) return s;
     if (ctx->timing) {
         float t = 0;
@@ -422,6 +460,16 @@ spg_status run_product(spg_context *ctx, ProductPlan &p) {
         ctx->stats.ms_numeric = t;
     }
     return SPG_OK;
+}
+
+// The whole product on the device.  On success the result is
+// rowptr[cur], ccol[cur], cval[cur] with res_nnz entries.
+spg_status run_product(spg_context *ctx, ProductPlan &p) {
+    Control h{};
+    bool done = false;
+    spg_status s = count_product(ctx, p, h, done);
+    if (s != SPG_OK || done) return s;
+    return value_product(ctx, p, h);
 }
 
 }  // namespace spg
@@ -516,11 +564,7 @@ spg_status spg_create(spg_context **out, int device) {
 spg_status spg_destroy(spg_context *ctx) {
     if (!ctx) return SPG_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->lsave, &ctx->a_ptr, &ctx->a_col, &ctx->a_val, &ctx->b_ptr, &ctx->b_col, &ctx->b_val,
-                      &ctx->dir_start, &ctx->dir_end, &ctx->ebst, &ctx->elen, &ctx->ub, &ctx->bin_rows, &ctx->snnz,
-                      &ctx->annz, &ctx->scan_tmp, &ctx->ctrl, &ctx->lbits, &ctx->lacc, &ctx->tile_status, &ctx->tile_distinct, &ctx->part, &ctx->dir32, &ctx->halo_err, &ctx->halo_cnt, &ctx->halo_tmp, &ctx->pack_cnt, &ctx->pack_off, &ctx->pack_map, &ctx->rowptr[0],
-                      &ctx->rowptr[1], &ctx->ccol[0], &ctx->ccol[1], &ctx->cval[0], &ctx->cval[1]};
-    for (DevBuf *b : bufs) b->release();
+    ctx->for_each_buffer([](DevBuf &b) { b.release(); });
     for (hipEvent_t &e : ctx->ev)
         if (e) hipEventDestroy(e);
     if (ctx->h_ctrl) hipHostFree(ctx->h_ctrl);

@@ -9,13 +9,26 @@
 
 namespace spg {
 
-// Growth-only device buffer (no allocation once a context is warm).
+// Growth-only device buffer (no allocation once a context is warm).  With
+// `headroom` (batch buffers, rebuilt for products of varying sizes), a buffer
+// that has to grow again takes 1/8 more than asked, so that slowly growing
+// sizes reallocate rarely: freeing and allocating tens of GB stalls the
+// device for about a second.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
     bool zero_on_grow = false;
+    bool headroom = false;
+    // Bytes held after ensure(want).
+    size_t size_for(size_t want) const {
+        if (want <= bytes && p) return bytes;
+        size_t n = p && headroom ? want + want / 8 : want;
+        n = n < 256 ? 256 : n;
+        return (n + 255) & ~size_t(255);
+    }
     hipError_t ensure(size_t want, hipStream_t stream) {
         if (want <= bytes && p) return hipSuccess;
+        const size_t n = size_for(want);
         if (p) {
             hipError_t e = hipStreamSynchronize(stream);
             if (e != hipSuccess) return e;
@@ -23,8 +36,6 @@ struct DevBuf {
             p = nullptr;
             bytes = 0;
         }
-        size_t n = want < 256 ? 256 : want;
-        n = (n + 255) & ~size_t(255);
         hipError_t e = hipMalloc(&p, n);
         if (e != hipSuccess) {
             p = nullptr;
@@ -42,6 +53,31 @@ struct DevBuf {
     }
     template <typename T>
     T *as() const { return static_cast<T *>(p); }
+};
+
+// Buffers a phase needs, with their sizes: what ensure() on each would add to
+// the device bytes held (a growing buffer is freed before it is allocated
+// again).
+struct BufferNeeds {
+    static constexpr int kMax = 32;
+    int n = 0;
+    DevBuf *buf[kMax] = {};
+    size_t bytes[kMax] = {};
+    const char *what[kMax] = {};
+    void add(DevBuf &b, size_t sz, const char *w) {
+        buf[n] = &b;
+        bytes[n] = sz;
+        what[n] = w;
+        ++n;
+    }
+    uint64_t growth() const {
+        uint64_t g = 0;
+        for (int i = 0; i < n; ++i) {
+            const uint64_t after = buf[i]->size_for(bytes[i]);
+            if (after > buf[i]->bytes) g += after - buf[i]->bytes;
+        }
+        return g;
+    }
 };
 
 // Everything one product needs on the device, fixed once its counts are
@@ -98,6 +134,21 @@ struct spg_context {
     int32_t res_rows = 0, res_cols = 0;
     int64_t res_nnz = 0;
     spg_stats stats{};
+    // Every device buffer the context owns.
+    template <typename F>
+    void for_each_buffer(F f) {
+        spg::DevBuf *bufs[] = {&lsave, &a_ptr, &a_col, &a_val, &b_ptr, &b_col, &b_val, &dir_start, &dir_end,
+                               &ebst, &elen, &ub, &bin_rows, &snnz, &annz, &scan_tmp, &ctrl, &lbits, &lacc,
+                               &tile_status, &tile_distinct, &part, &dir32, &halo_err, &halo_cnt, &halo_tmp,
+                               &pack_cnt, &pack_off, &pack_map, &rowptr[0], &rowptr[1], &ccol[0], &ccol[1],
+                               &cval[0], &cval[1]};
+        for (spg::DevBuf *b : bufs) f(*b);
+    }
+    uint64_t device_bytes() {
+        uint64_t n = 0;
+        for_each_buffer([&](spg::DevBuf &b) { n += b.bytes; });
+        return n;
+    }
 };
 
 namespace spg {
@@ -130,6 +181,11 @@ spg_status flag_error(spg_context *ctx, unsigned long long flags);
         if (e_ != hipSuccess) return spg::hip_err(ctx, e_, what); \
     } while (0)
 
+// The count workspace of a product with `rows` A rows, nnz_a A entries, a
+// directory of ndir B rows (the 32-bit copy when dir32) and `cols` C columns:
+// its buffers with their sizes, and the large-row windows' width and count.
+void workspace_needs(spg_context *ctx, int32_t rows, int64_t nnz_a, int32_t ndir, bool dir32, int64_t cols,
+                     BufferNeeds &v, int64_t &lwidth, int &lblocks);
 // Workspace buffers for p.op / p.rows / p.cols; fills p.ws.
 spg_status plan_workspace(spg_context *ctx, ProductPlan &p);
 // Zero the control block, analyze, symbolic kernels of `bins` (and the scan
@@ -141,8 +197,21 @@ spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h);
 void enqueue_values(spg_context *ctx, const ProductPlan &p);
 // Compaction when exact zeros left gaps; publishes the result on the context.
 spg_status finish_product(spg_context *ctx, const ProductPlan &p, const Control &h, spg_stats &stats);
-// The whole product with its two host synchronisations; leaves the plan
-// filled for replays.
+// The count phase of a product: workspace, analyze and the symbolic kernels,
+// up to the point where C's size is known (the counts are left in h).  `done`
+// when the direct single pass has already produced the result.
+spg_status count_product(spg_context *ctx, ProductPlan &p, Control &h, bool &done);
+// Device bytes the value phase of a counted plan adds to the context's
+// buffers: C's entries at the plan's capacity (exact structural counts on the
+// two-phase path; the small rows' work bounds plus the big rows' exact counts
+// on the single pass), the large rows' accumulator windows and the
+// single-pass tile state, less what the buffers already hold.  Compaction
+// buffers (results with exact zeros) are not included.
+uint64_t value_bytes(spg_context *ctx, const ProductPlan &p, const Control &h);
+// The value phase of a counted plan: C buffers, value kernels, compaction.
+spg_status value_product(spg_context *ctx, ProductPlan &p, const Control &counted);
+// The whole product with its two host synchronisations (count_product, then
+// value_product); leaves the plan filled for replays.
 spg_status run_product(spg_context *ctx, ProductPlan &p);
 
 }  // namespace spg

@@ -34,9 +34,11 @@ EXPORTED = [
     "spg_matrix_last_error", "spg_matrix_upload", "spg_device_matrix_view", "spg_device_matrix_free",
     "spg_synchronize", "spg_set_path", "spg_set_format", "spg_multiply_rows", "spg_result_rows",
     "spg_multiply_noalloc",
-    "spg_batch_create", "spg_batch_create_chained", "spg_batch_refresh", "spg_batch_run", "spg_batch_set_graph",
+    "spg_batch_create", "spg_batch_create_chained", "spg_batch_refresh", "spg_batch_run", "spg_batch_count",
+    "spg_batch_set_graph",
     "spg_batch_autotune", "spg_batch_product_nnz", "spg_batch_copy", "spg_batch_info", "spg_batch_last_stats",
-    "spg_batch_destroy", "spg_batch_rebuild", "spg_batch_set_stream", "spg_batch_last_error", "spg_context_stream", "spg_context_bind",
+    "spg_batch_destroy", "spg_batch_rebuild", "spg_batch_set_stream", "spg_batch_device_bytes", "spg_batch_trim",
+    "spg_batch_estimate", "spg_batch_last_error", "spg_context_stream", "spg_context_bind",
     "spg_server_create", "spg_server_submit", "spg_server_register", "spg_server_unregister",
     "spg_server_submit_resident", "spg_server_wait", "spg_server_result", "spg_server_release", "spg_server_stats",
     "spg_server_destroy",
@@ -165,6 +167,12 @@ def lib():
         L.spg_batch_create_chained.argtypes = [vp, vp, C.c_int32, P(C.c_int32), P(CsrView), P(vp)]
         L.spg_batch_refresh.argtypes = [vp]
         L.spg_batch_run.argtypes = [vp, P(C.c_int64)]
+        L.spg_batch_count.argtypes = [vp, P(C.c_uint64)]
+        L.spg_batch_rebuild.argtypes = [vp, C.c_int32, P(CsrView), P(CsrView)]
+        L.spg_batch_device_bytes.argtypes = [vp, P(C.c_uint64)]
+        L.spg_batch_trim.argtypes = [vp]
+        L.spg_batch_estimate.argtypes = [vp, vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_int64,
+                                         P(C.c_uint64)]
         L.spg_batch_set_graph.argtypes = [vp, C.c_int]
         L.spg_batch_autotune.argtypes = [vp, C.c_int, P(C.c_int)]
         L.spg_batch_product_nnz.argtypes = [vp, C.c_int32, P(C.c_int64)]
@@ -566,6 +574,42 @@ class Batch:
         n = C.c_int64()
         self._bcheck(self._L.spg_batch_run(self._h, C.byref(n)))
         return int(n.value)
+
+    def count(self) -> int:
+        """Run the count phase only; returns the device bytes the value phase
+        will allocate (the next run() continues from these counts)."""
+        n = C.c_uint64()
+        self._bcheck(self._L.spg_batch_count(self._h, C.byref(n)))
+        return int(n.value)
+
+    def rebuild(self, a_list, b_list):
+        """Replace the products, reusing the batch's device buffers."""
+        n = len(a_list)
+        av = (CsrView * max(n, 1))()
+        bv = (CsrView * max(n, 1))()
+        for i, (a, b) in enumerate(zip(a_list, b_list)):
+            av[i] = host_view(a)
+            bv[i] = host_view(b)
+        self._bcheck(self._L.spg_batch_rebuild(self._h, n, av, bv))
+        self.n = n
+
+    def device_bytes(self) -> int:
+        n = C.c_uint64()
+        self._bcheck(self._L.spg_batch_device_bytes(self._h, C.byref(n)))
+        return int(n.value)
+
+    def estimate(self, rows: int, a_nnz: int, b_rows: int, b_nnz: int, cols: int, c_entries: int) -> int:
+        """Device bytes rebuilding this batch with products of these total
+        sizes and running it with C's capacity at c_entries would add."""
+        n = C.c_uint64()
+        self._e._check(self._L.spg_batch_estimate(self._e._h, self._h, rows, a_nnz, b_rows, b_nnz, cols, c_entries,
+                                                  C.byref(n)))
+        return int(n.value)
+
+    def trim(self):
+        """Free the batch's device buffers (it holds no products afterwards)."""
+        self._bcheck(self._L.spg_batch_trim(self._h))
+        self.n = 0
 
     def set_graph(self, on: bool):
         self._e._check(self._L.spg_batch_set_graph(self._h, 1 if on else 0))

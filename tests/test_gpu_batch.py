@@ -168,3 +168,40 @@ def test_batch_big_bins_and_wide_format(engine, path, fmt):
     finally:
         engine.set_path(S.PATH_AUTO)
         engine.set_format(S.FORMAT_AUTO)
+
+
+def test_batch_count_then_run_and_trim(engine):
+    """The count phase alone (spg_batch_count) then the run gives the same
+    products as one run; the reported value bytes are the buffer growth the
+    run then shows; a trimmed batch holds no device bytes and rebuilds."""
+    jobs = small_jobs(12, seed=23)
+    _, _, a1, b1 = waves(jobs)
+    ref = S.Batch(engine, a1, b1)
+    ref.run()
+    want = [digest(ref.result(i)) for i in range(len(a1))]
+    batch = S.Batch(engine, a1, b1)
+    before = batch.device_bytes()
+    grow = batch.count()
+    counted = batch.device_bytes()
+    assert counted >= before and grow > 0
+    total = batch.run()
+    after = batch.device_bytes()
+    # the value phase grows the buffers by the reported bytes, plus the
+    # compaction buffers when a product has exact zeros (not in the bound)
+    assert counted + grow <= after <= counted + grow + 12 * total + 8 * (sum(a.rows for a in a1) + 1) + 768
+    assert [digest(batch.result(i)) for i in range(len(a1))] == want
+    # counted again on warm buffers: nothing more to grow
+    assert batch.count() == 0
+    batch.run()
+    assert [digest(batch.result(i)) for i in range(len(a1))] == want
+    batch.trim()
+    assert batch.device_bytes() == 0
+    # the estimate of a rebuild's arena and count workspace is what they take
+    a5, b5 = a1[:5], b1[:5]
+    est = batch.estimate(sum(a.rows for a in a5), sum(a.nnz for a in a5), sum(b.rows for b in b5),
+                         sum(b.nnz for b in b5), max(b.cols for b in b5), 0)
+    batch.rebuild(a5, b5)
+    batch.count()
+    assert batch.device_bytes() == est
+    batch.run()
+    assert [digest(batch.result(i)) for i in range(5)] == want[:5]

@@ -88,3 +88,95 @@ def test_served_admission_and_errors():
         srv.release(rid)
     finally:
         srv.close()
+
+
+def _dup_heavy_pair(seed):
+    """A (256 x 512, 24 distinct columns per row) times B (512 x 1024, every
+    row holding columns 0..15): 384 scalar products per row (fewer than C's
+    1024 columns, so no structural bound is tighter than the products) but
+    only 16 entries per row of C, so C is 24x smaller than its bound."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rows, inner, cols, per, dense = 256, 512, 1024, 24, 16
+    acol = np.concatenate([np.sort(rng.choice(inner, per, replace=False)) for _ in range(rows)])
+    a = S.Csr.make(rows, inner, np.arange(rows + 1) * per, acol, rng.uniform(-1, 1, rows * per))
+    bcol = np.tile(np.arange(dense), inner)
+    b = S.Csr.make(inner, cols, np.arange(inner + 1) * dense, bcol, rng.uniform(-1, 1, inner * dense))
+    return a, b
+
+
+def _base_bytes(a, b):
+    """An upper estimate of a product's arena and count workspace: 12 B per
+    entry + 8 B per row of each operand, 12 B per A entry (entry stream),
+    160 B per A row (row bins, counts, C's row offsets), 4 B per B row."""
+    def operand(m):
+        return 12 * int(m.row_ptr[-1]) + 8 * (m.rows + 1)
+    return operand(a) + operand(b) + 12 * int(a.row_ptr[-1]) + 160 * (a.rows + 1) + 4 * (b.rows + 1)
+
+
+SLACK = 256 << 10  # large-row bitmaps, small buffers and allocation rounding
+
+
+def test_served_admission_by_counts(engine):
+    a, b = _dup_heavy_pair(11)
+    products = int(sum(b.row_ptr[c + 1] - b.row_ptr[c] for c in a.col))
+    alone = engine.multiply(a, b)
+    nnz = int(alone.row_ptr[-1])
+    assert nnz == a.rows * 16 and products == 384 * a.rows
+    base = _base_bytes(a, b)
+    budget = base + 12 * nnz + SLACK
+    assert base + 12 * products > budget  # the work bound would not admit it
+    srv = S.Server(0, budget_bytes=budget, max_batch=8)
+    try:
+        rid = srv.submit(a, b)
+        got = srv.result(rid)
+        assert digest(got) == digest(alone)
+        # held: the batch's measured buffers, C at its counted size
+        held = srv.stats()["charged_bytes"]
+        print("held bytes", held, "budget", budget, "work bound", 12 * products, flush=True)
+        assert 12 * nnz <= held <= budget
+        srv.release(rid)
+        assert srv.stats()["charged_bytes"] == 0
+    finally:
+        srv.close()
+    # in a mix: two such requests and small ones (single-pass rows, charged by
+    # their work bound).  Every operand fits the budget at once but C of only
+    # one of the two big requests does: the builder splits the batch or waits
+    # for the first big request's release.
+    smalls = [(W.uniform_random(40, 40, 3, seed=70 + k), W.uniform_random(40, 40, 3, seed=80 + k)) for k in range(4)]
+    a2, b2 = _dup_heavy_pair(12)
+    small_products = sum(int(sum(y.row_ptr[c + 1] - y.row_ptr[c] for c in x.col)) for x, y in smalls)
+    budget = 2 * base + 12 * nnz + sum(_base_bytes(x, y) for x, y in smalls) + 12 * small_products + SLACK
+    assert base + 12 * products > budget
+    srv = S.Server(0, budget_bytes=budget, max_batch=8)
+    try:
+        ids = [srv.submit(a, b)] + [srv.submit(x, y) for x, y in smalls] + [srv.submit(a2, b2)]
+        expect = [alone] + [engine.multiply(x, y) for x, y in smalls] + [engine.multiply(a2, b2)]
+        for k, (rid, want) in enumerate(zip(ids, expect)):
+            print("mixed request", k, flush=True)
+            assert digest(srv.result(rid)) == digest(want), k
+            srv.release(rid)
+        assert srv.stats()["charged_bytes"] == 0
+        assert srv.stats()["completed"] == len(ids)
+    finally:
+        srv.close()
+
+
+def test_served_result_too_large_for_budget():
+    # one A column of ones times one dense B row: 64 x 4096 entries of C (3 MiB)
+    # from operands of a few KiB; the count phase finds C does not fit 1 MiB
+    a = S.Csr.make(64, 1, np.arange(65), np.zeros(64), np.ones(64))
+    b = S.Csr.make(1, 4096, [0, 4096], np.arange(4096), np.ones(4096))
+    srv = S.Server(0, budget_bytes=1 << 20, max_batch=8)
+    try:
+        rid = srv.submit(a, b)
+        with pytest.raises(S.SpgError) as e:
+            srv.wait(rid)
+        assert e.value.name == "SPG_ENOMEM"
+        srv.release(rid)
+        assert srv.stats()["charged_bytes"] == 0
+        small = S.Csr.make(2, 2, [0, 1, 2], [0, 1], [2.0, 3.0])
+        rid = srv.submit(small, small)
+        assert srv.result(rid).val.tolist() == [4.0, 9.0]
+        srv.release(rid)
+    finally:
+        srv.close()
