@@ -193,10 +193,11 @@ def cpu_baseline(a, b, products_total, target_s=20.0):
     threads, physical = host_cores()
     runs = 1 + CPU_WARMUP + CPU_REPEATS * CPU_TRIALS
     # leading rows of A holding about target_s / runs seconds of the
-    # reference's work per run (~8e6 products per second per thread)
-    per_row = products_total / max(a.rows, 1)
+    # reference's work per run (~8e6 products per second per thread), by the
+    # rows' cumulative products (skewed matrices put their heaviest rows first)
     budget_products = 8e6 * threads * target_s / runs
-    sample_rows = int(max(256, min(a.rows, budget_products / max(per_row, 1))))
+    cum = np.concatenate([[0], np.cumsum(np.diff(b.row_ptr)[a.col], dtype=np.int64)])[a.row_ptr]
+    sample_rows = int(min(a.rows, max(1, np.searchsorted(cum, budget_products, side="left"))))
     sub_p = int(a.row_ptr[sample_rows])
     asub = O.Csr.make(sample_rows, a.cols, a.row_ptr[: sample_rows + 1], a.col[:sub_p], a.val[:sub_p])
     bo = O.Csr.make(b.rows, b.cols, b.row_ptr, b.col, b.val)
@@ -539,6 +540,17 @@ def run_batched(args, wl, steps=10, warmup=2, cpu=True):
                  for k, i in enumerate(rap))
     gpu_mean = statistics.mean(gpu_ms) if gpu_ms else tf["mean_ms"]
 
+    if args.batch_only:  # counter profiles: the batch's dispatches are each kernel's last
+        for o in (w2, w1, eng):
+            o.close()
+        p90_s = tf["p90_ms"] * 1e-3
+        return {"metric": "batched pipeline only (profiling run: no sequential leg, parity or CPU baseline)",
+                "value": round(njobs / p90_s, 2), "unit": "jobs/s", "steps": steps, "warmup": warmup,
+                "ms_per_step": tf["mean_ms"],
+                "config": {"workload": "batched_4096", **wl["desc"], "jobs": njobs, "products": int(P_total),
+                           "c_nnz": int(nnz), "gpu_ms_mean": round(gpu_mean, 3)},
+                "roofline": None, "cpu_baseline": None}
+
     # sequential plain launches: one drop-in multiply per product on resident
     # operands; the triples' second product takes the first one's device result
     log("bench: sequential plain launches")
@@ -781,6 +793,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="batched workload: launch the pipeline eagerly instead of replaying a HIP graph "
                          "(for kernel traces)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batched workload: time the batch pipeline only, so that each kernel's last dispatch "
+                         "is the batch's (for counter profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this engine computes one rank's local product, and only a "

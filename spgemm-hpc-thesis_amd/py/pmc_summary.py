@@ -1,14 +1,19 @@
 """Reduce rocprofv3 output to the per-kernel tables kept under profiles/.
 
     python spgemm-hpc-thesis_amd/py/pmc_summary.py counters <counter_collection.csv> [--kernel SUBSTR]
-        [--dispatch last|all|N] [--factor COUNTER=F ...]
+        [--dispatch last|all|median|N] [--segment MARKER:K] [--factor COUNTER=F ...]
     python spgemm-hpc-thesis_amd/py/pmc_summary.py trace <kernel_stats.csv> [--top N]
 
 counters: one row per (kernel, counter) with the value per dispatch of the
   chosen dispatches (`last`: the last dispatch of each kernel, which in a
   bench run is the last timed product; `all`: the mean over every dispatch;
   `median`: the lower median over every dispatch; N: the N-th dispatch,
-  0-based).  Counters with a calibration factor
+  0-based; negative N counts from the end).  --segment MARKER:K first keeps
+  only the dispatches of segment K (0-based, negative from the end) of the
+  dispatch sequence cut before every dispatch of the kernel MARKER (for
+  example one product's pipeline, which starts with k_zero_control), and
+  reports each kernel's sum over its dispatches in that segment.  Counters
+  with a calibration factor
   (--factor FETCH_SIZE=2.0, from a plain device copy of known size) are also
   printed in bytes (KiB x 1024 x factor).  Derived ratios are printed when
   their counters are present: SQ_LDS_BANK_CONFLICT / SQ_ACTIVE_INST_LDS and
@@ -31,15 +36,24 @@ def _short(name: str) -> str:
     return n.split("(")[0].strip()
 
 
-def counters(path, kernel=None, dispatch="last", factors=None):
+def counters(path, kernel=None, dispatch="last", factors=None, segment=None):
     factors = factors or {}
     rows = defaultdict(lambda: defaultdict(list))  # kernel -> counter -> [(dispatch id, value)]
     with open(path, newline="") as f:
-        for r in csv.DictReader(f):
-            k = _short(r["Kernel_Name"])
-            if kernel and kernel not in k:
-                continue
-            rows[k][r["Counter_Name"]].append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+        recs = list(csv.DictReader(f))
+    if segment:
+        marker, k = segment.rsplit(":", 1)
+        starts = sorted({int(r["Dispatch_Id"]) for r in recs if marker in _short(r["Kernel_Name"])})
+        lo = starts[int(k)]
+        nxt = [d for d in starts if d > lo]
+        hi = nxt[0] if nxt else float("inf")
+        recs = [r for r in recs if lo <= int(r["Dispatch_Id"]) < hi]
+        dispatch = "sum"
+    for r in recs:
+        k = _short(r["Kernel_Name"])
+        if kernel and kernel not in k:
+            continue
+        rows[k][r["Counter_Name"]].append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
     out = []
     for k, cs in rows.items():
         vals = OrderedDict()
@@ -51,6 +65,8 @@ def counters(path, kernel=None, dispatch="last", factors=None):
                 v = lst[-1][1]
             elif dispatch == "all":
                 v = sum(x for _, x in lst) / len(lst)
+            elif dispatch == "sum":
+                v = sum(x for _, x in lst)
             elif dispatch == "median":
                 xs = sorted(x for _, x in lst)
                 v = xs[(len(xs) - 1) // 2]
@@ -60,7 +76,8 @@ def counters(path, kernel=None, dispatch="last", factors=None):
         out.append((k, ndisp, vals))
     lines = []
     for k, ndisp, vals in out:
-        lines.append(f"### `{k}` ({ndisp} dispatches; {dispatch})")
+        what = f"segment {segment}, summed" if segment else dispatch
+        lines.append(f"### `{k}` ({ndisp} dispatches; {what})")
         lines.append("")
         lines.append("| counter | per dispatch | bytes |")
         lines.append("|---|---|---|")
@@ -94,6 +111,7 @@ def main():
     c.add_argument("csv")
     c.add_argument("--kernel", default=None)
     c.add_argument("--dispatch", default="last")
+    c.add_argument("--segment", default=None)
     c.add_argument("--factor", action="append", default=[])
     t = sub.add_parser("trace")
     t.add_argument("csv")
@@ -104,7 +122,7 @@ def main():
         for item in a.factor:
             name, val = item.split("=")
             fac[name] = float(val)
-        print(counters(a.csv, a.kernel, a.dispatch, fac))
+        print(counters(a.csv, a.kernel, a.dispatch, fac, a.segment))
     else:
         print(trace(a.csv, a.top))
 
