@@ -100,7 +100,7 @@ typedef struct spg_stats {
     double ms_numeric;         /* device time of the numeric phase, if timing on */
     int32_t direct;            /* 1 if every row fit the single pass by the longest rows of A
                                   and B alone, so the per-row analysis was skipped */
-    int32_t reserved_;
+    int32_t partitioned;       /* 1 if large rows were partitioned by column window (spg_set_large) */
 } spg_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -147,6 +147,20 @@ spg_status spg_set_path(spg_context *ctx, int path);
  * from the context inherit the setting.  No counterpart in the reference. */
 enum { SPG_FORMAT_AUTO = 0, SPG_FORMAT_WIDE = 1 };
 spg_status spg_set_format(spg_context *ctx, int format);
+/* Value kernels of the rows with more than 2048 scalar products:
+ *   SPG_LARGE_AUTO     rows whose distinct columns fit an LDS accumulator
+ *                      fold there (C with at most 2^16 columns); the others
+ *                      are partitioned by windows of 4096 columns into HBM
+ *                      scratch and each window folds in LDS, when C has at
+ *                      most 2^20 columns and the scratch (10 B per product)
+ *                      fits a quarter of the device; else they fold onto
+ *                      dense accumulator windows in HBM
+ *   SPG_LARGE_WINDOWS  never partitioned (the HBM windows)
+ * Results are identical.  Batches and accumulators created from the context
+ * inherit the setting; spg_stats.partitioned reports when rows were
+ * partitioned.  No counterpart in the reference. */
+enum { SPG_LARGE_AUTO = 0, SPG_LARGE_WINDOWS = 1 };
+spg_status spg_set_large(spg_context *ctx, int mode);
 /* Record HIP events around the phases of each multiply (spg_stats.ms_*). */
 spg_status spg_set_timing(spg_context *ctx, int enabled);
 spg_status spg_last_stats(const spg_context *ctx, spg_stats *out);

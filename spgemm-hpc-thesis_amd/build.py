@@ -35,7 +35,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-Wextra", f"-I{INCLUDE}"]
 
 HIP_SOURCES = ["analyze.hip", "symbolic.hip", "numeric.hip", "fused.hip", "scan.hip", "engine.hip", "batch.hip", "accumulate.hip",
-               "halo.hip", "noalloc.hip"]
+               "halo.hip", "noalloc.hip", "large_part.hip"]
 CXX_SOURCES = ["host_matrix.cpp", "server.cpp", "validate.cpp"]
 
 

@@ -196,6 +196,7 @@ spg_status spg_accumulator_add(spg_accumulator *acc, const spg_csr_view *A, cons
     spg_context *own = acc->own;
     own->path = ctx->path;
     own->force_wide = ctx->force_wide;
+    own->large_mode = ctx->large_mode;
     own->allow_direct = false;
     plan.small_b = narrow_b(own, bn);
     plan.b_span = bn;

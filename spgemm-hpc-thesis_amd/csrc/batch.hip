@@ -142,6 +142,7 @@ spg_status new_batch(spg_context *ctx, spg_batch **out) {
     b->own->path = ctx->path;
     b->own->fused_rows = ctx->fused_rows;
     b->own->force_wide = ctx->force_wide;
+    b->own->large_mode = ctx->large_mode;
     b->own->allow_direct = false;  // batch plans are replayed as analysis + values
     b->own->timing = ctx->timing;
     *out = b;

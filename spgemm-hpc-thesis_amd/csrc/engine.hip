@@ -144,20 +144,27 @@ void enqueue_counts(spg_context *ctx, const ProductPlan &p, unsigned bins, bool 
     launch_zero_control(p.ws.ctrl, st);
     if (p.ws.bdir32) launch_dir32(p.op.bstart, (int64_t)p.op.ndir + 1, p.ws.bdir32, st);
     launch_analyze(p.op, p.ws, st);
-    launch_symbolic(p.op, p.ws, bins, ctx->cus, st, p.lsave);
+    // replays of a plan whose large rows are partitioned recompute the
+    // batches of the large rows (the control block was cleared)
+    const bool part = p.part_counts && (bins & (1u << kBinLarge)) && p.bin_count[kBinLarge] > 0 && p.ps.pseg;
+    if (part) launch_part_batches(p.op, p.ws, p.ps, p.bin_count[kBinLarge], st);
+    launch_symbolic(p.op, p.ws, bins, ctx->cus, st, p.lsave, part ? &p.ps : nullptr);
     if (scan) launch_scan(p.ws.snnz, ctx->rowptr[0].as<int64_t>(), p.rows, p.ws.scan_tmp, &p.ws.ctrl->structural, st);
 }
 
-// The counting phase of a plan whose large rows take the compact value
-// kernel: their count saves each row's presence bitmap into a buffer sized by
-// the number of large rows, which is known only after the other bins are
-// counted, so the large rows are counted after a first read of the control
-// block (then C's row offsets are scanned when `scan`).  Leaves the control
-// block's current state in h.
-spg_status counts_compact(spg_context *ctx, ProductPlan &p, unsigned bins, bool scan, Control &h) {
+// The counting phase of a plan whose large-row count writes per-row arrays
+// sized by the number of large rows (the compact value kernel's saved
+// bitmaps, the partition's per-window counts), which is known only after the
+// other bins are counted: the large rows are counted after a first read of the
+// control block (then C's row offsets are scanned when `scan`).  Leaves the
+// control block's current state in h.
+spg_status counts_deferred(spg_context *ctx, ProductPlan &p, unsigned bins, bool scan, Control &h) {
     hipStream_t st = ctx->stream;
     p.lsave = nullptr;
+    const bool part_counts = p.part_counts;
+    p.part_counts = false;  // the first pass counts no large rows
     enqueue_counts(ctx, p, bins & ~(1u << kBinLarge), false);
+    p.part_counts = part_counts;
     spg_status s;
     if ((s = read_control(ctx)) != SPG_OK) return s;
     h = *ctx->h_ctrl;
@@ -165,9 +172,26 @@ spg_status counts_compact(spg_context *ctx, ProductPlan &p, unsigned bins, bool 
     const int nlarge = (bins & (1u << kBinLarge)) ? h.bin_count[kBinLarge] : 0;
     if (nlarge == 0 && !scan) return SPG_OK;
     if (nlarge > 0) {
-        SPG_HIP(ctx->lsave.ensure((size_t)nlarge * (size_t)(p.ws.lwidth / 8), st), "hipMalloc(large-row bitmaps)");
-        p.lsave = ctx->lsave.as<uint32_t>();
-        launch_symbolic(p.op, p.ws, 1u << kBinLarge, ctx->cus, st, p.lsave);
+        if (p.compact) {
+            SPG_HIP(ctx->lsave.ensure((size_t)nlarge * (size_t)(p.ws.lwidth / 8), st), "hipMalloc(large-row bitmaps)");
+            p.lsave = ctx->lsave.as<uint32_t>();
+        }
+        if (p.part_counts) {
+            // per large row and window, and per batch of kCapLarge A entries
+            // (at most one per large row plus one per kCapLarge A entries)
+            const size_t cells = (size_t)nlarge * (size_t)p.ps.nwin;
+            const size_t batches = (size_t)nlarge + (size_t)(p.op.nnz_a / kCapLarge) + 1;
+            SPG_HIP(ctx->ppw.ensure(cells * 8, st), "hipMalloc(window products)");
+            SPG_HIP(ctx->ppd.ensure(cells * 4, st), "hipMalloc(window columns)");
+            SPG_HIP(ctx->psegb.ensure(2 * ((size_t)nlarge + 1) * 8, st), "hipMalloc(batch offsets)");
+            SPG_HIP(ctx->pseg.ensure(batches * (size_t)p.ps.nwin * 4, st), "hipMalloc(batch window counts)");
+            p.ps.pw = ctx->ppw.as<int64_t>();
+            p.ps.pd = ctx->ppd.as<int32_t>();
+            p.ps.segbase = ctx->psegb.as<int64_t>();
+            p.ps.pseg = ctx->pseg.as<uint32_t>();
+            launch_part_batches(p.op, p.ws, p.ps, nlarge, st);
+        }
+        launch_symbolic(p.op, p.ws, 1u << kBinLarge, ctx->cus, st, p.lsave, p.part_counts ? &p.ps : nullptr);
     }
     if (scan) launch_scan(p.ws.snnz, ctx->rowptr[0].as<int64_t>(), p.rows, p.ws.scan_tmp, &p.ws.ctrl->structural, st);
     if ((s = read_control(ctx)) != SPG_OK) return s;
@@ -179,7 +203,7 @@ namespace {
 
 // The buffers the value phase of a counted plan needs, with their sizes.
 struct ValueBuffers : BufferNeeds {
-    int64_t capacity = 0, ntiles = 0, pack_blocks = 0;
+    int64_t capacity = 0, ntiles = 0, pack_blocks = 0, max_tasks = 0;
 };
 
 ValueBuffers value_buffers(spg_context *ctx, const ProductPlan &p, const Control &h) {
@@ -187,8 +211,23 @@ ValueBuffers value_buffers(spg_context *ctx, const ProductPlan &p, const Control
     v.capacity = p.fused ? (int64_t)(h.ub_small + h.sym_total) : (int64_t)h.structural;
     v.add(ctx->ccol[0], (size_t)v.capacity * 4, "hipMalloc(C col)");
     v.add(ctx->cval[0], (size_t)v.capacity * 8, "hipMalloc(C val)");
-    if (h.bin_count[kBinLarge] > 0)
+    const int nlarge = h.bin_count[kBinLarge];
+    if (nlarge > 0 && p.part) {
+        // partitioned large rows: per-window stored counts, gap flags,
+        // scratch offsets, the scratch and the fold tasks (every task but a
+        // row's last holds at least kPartTaskProducts products)
+        const size_t cells = (size_t)nlarge * (size_t)p.ps.nwin;
+        const size_t P = (size_t)h.part_products;
+        v.max_tasks = (int64_t)nlarge + (int64_t)(P / (size_t)kPartTaskProducts) + 1;
+        v.add(ctx->ppst, cells * 4, "hipMalloc(window stored counts)");
+        v.add(ctx->ppgap, (size_t)nlarge * 4, "hipMalloc(window gaps)");
+        v.add(ctx->ppoff, 2 * ((size_t)nlarge + 1) * 8, "hipMalloc(scratch offsets)");
+        v.add(ctx->pscol, P * 2, "hipMalloc(scratch columns)");
+        v.add(ctx->psval, P * 8, "hipMalloc(scratch products)");
+        v.add(ctx->ptask, (size_t)v.max_tasks * sizeof(PartTask), "hipMalloc(fold tasks)");
+    } else if (nlarge > 0) {
         v.add(ctx->lacc, (size_t)p.ws.lblocks * (size_t)p.ws.lwidth * 8, "hipMalloc(large accumulators)");
+    }
     if (p.fused) {
         if (p.fused_rows == kFusedPackRows) {
             // Packed tiles: at most one per block plus one per kFusedPackCap
@@ -222,7 +261,16 @@ spg_status plan_values(spg_context *ctx, ProductPlan &p, const Control &h) {
     const ValueBuffers v = value_buffers(ctx, p, h);
     for (int i = 0; i < v.n; ++i) SPG_HIP(v.buf[i]->ensure(v.bytes[i], st), v.what[i]);
     p.capacity = v.capacity;
-    if (h.bin_count[kBinLarge] > 0) p.ws.lacc = ctx->lacc.as<double>();
+    if (h.bin_count[kBinLarge] > 0 && !p.part) p.ws.lacc = ctx->lacc.as<double>();
+    if (h.bin_count[kBinLarge] > 0 && p.part) {
+        p.ps.pst = ctx->ppst.as<int32_t>();
+        p.ps.pgap = ctx->ppgap.as<int32_t>();
+        p.ps.poff = ctx->ppoff.as<int64_t>();
+        p.ps.scol = ctx->pscol.as<uint16_t>();
+        p.ps.sval = ctx->psval.as<double>();
+        p.ps.task = ctx->ptask.as<PartTask>();
+        p.ps.max_tasks = v.max_tasks;
+    }
     p.fo = FusedOut{};
     if (p.fused) {
         p.fo.ntiles = v.ntiles;
@@ -252,7 +300,8 @@ void enqueue_values(spg_context *ctx, const ProductPlan &p) {
         launch_fused(p.op, p.ws, p.fo, p.small_b, p.fused_rows, st);
     }
     launch_numeric(p.op, p.ws, p.bin_count, p.fused ? kBigBins : kAllBins, ctx->cus, ctx->rowptr[0].as<int64_t>(),
-                   ctx->ccol[0].as<int32_t>(), ctx->cval[0].as<double>(), st, p.lsave, ctx->side, ctx->fork, ctx->join);
+                   ctx->ccol[0].as<int32_t>(), ctx->cval[0].as<double>(), st, p.lsave, ctx->side, ctx->fork, ctx->join,
+                   p.part ? &p.ps : nullptr);
     if (p.fused && p.fo.ntiles > 0) launch_sum(p.fo.distinct, p.fo.ntiles, &p.ws.ctrl->distinct, st);
 }
 
@@ -271,6 +320,7 @@ spg_status finish_product(spg_context *ctx, const ProductPlan &p, const Control 
     stats.structural_nnz = p.fused ? (int64_t)(h.distinct + h.sym_total) : (int64_t)h.structural;
     stats.single_pass = p.fused ? 1 : 0;
     stats.wide_format = p.small_b ? 0 : 1;
+    stats.partitioned = p.part && h.bin_count[kBinLarge] > 0 ? 1 : 0;
     const int64_t nnz = p.fused ? (int64_t)(h.reserved - h.sym_total + h.actual) : (int64_t)h.actual;
     int result = 0;
     if (nnz != layout) {
@@ -404,9 +454,17 @@ spg_status count_product(spg_context *ctx, ProductPlan &p, Control &h, bool &don
     // fold in LDS; their count then saves their bitmaps.
     p.compact = compact_large_available(p.ws.lwidth) && (int64_t)p.cols <= p.ws.lwidth;
     p.lsave = nullptr;
+    // Large rows the compact kernel leaves are partitioned by column window
+    // when the count can count each window; the value phase decides once the
+    // scratch size is known.
+    p.part = false;
+    p.ps = PartState{};
+    p.ps.skip_le = p.compact ? compact_take_le(p.ws.lwidth) : -1;
+    p.part_counts = ctx->large_mode == SPG_LARGE_AUTO && part_available(p.op.bcols, p.ws.lwidth);
+    p.ps.nwin = p.part_counts ? part_windows(p.op.bcols) : 0;
     h = Control{};
-    if (p.compact) {
-        if ((s = counts_compact(ctx, p, try_fused ? kBigBins : kAllBins, !try_fused, h)) != SPG_OK) return s;
+    if (p.compact || p.part_counts) {
+        if ((s = counts_deferred(ctx, p, try_fused ? kBigBins : kAllBins, !try_fused, h)) != SPG_OK) return s;
         if (ctx->timing) SPG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
     } else {
         enqueue_counts(ctx, p, try_fused ? kBigBins : kAllBins, !try_fused);
@@ -426,8 +484,8 @@ spg_status count_product(spg_context *ctx, ProductPlan &p, Control &h, bool &don
             // Two-phase after all: list the small rows too (analyze again) and
             // count them.
             p.ws.mode |= kWsListSmall;
-            if (p.compact) {
-                if ((s = counts_compact(ctx, p, kAllBins, true, h)) != SPG_OK) return s;
+            if (p.compact || p.part_counts) {
+                if ((s = counts_deferred(ctx, p, kAllBins, true, h)) != SPG_OK) return s;
             } else {
                 enqueue_counts(ctx, p, kAllBins, true);
                 if ((s = read_control(ctx)) != SPG_OK) return s;
@@ -436,6 +494,11 @@ spg_status count_product(spg_context *ctx, ProductPlan &p, Control &h, bool &don
             if ((s = flag_error(ctx, h.flags)) != SPG_OK) return s;
         }
     }
+    // The large rows the compact kernel leaves are partitioned when their
+    // scratch (10 B per product) fits a quarter of the device and no row has
+    // 2^31 products or more (the count's window counters are 32-bit).
+    if (p.part_counts && h.bin_count[kBinLarge] > 0 && h.part_products > 0)
+        p.part = h.part_max_row < (1ull << 31) && 10ull * h.part_products <= (uint64_t)(ctx->total_mem / 4);
     return SPG_OK;
 }
 
@@ -607,6 +670,13 @@ spg_status spg_set_format(spg_context *ctx, int format) {
     if (!ctx) return SPG_EINVAL;
     if (format != SPG_FORMAT_AUTO && format != SPG_FORMAT_WIDE) return set_err(ctx, SPG_EINVAL, "unknown format");
     ctx->force_wide = format == SPG_FORMAT_WIDE;
+    return SPG_OK;
+}
+
+spg_status spg_set_large(spg_context *ctx, int mode) {
+    if (!ctx) return SPG_EINVAL;
+    if (mode != SPG_LARGE_AUTO && mode != SPG_LARGE_WINDOWS) return set_err(ctx, SPG_EINVAL, "unknown large-row mode");
+    ctx->large_mode = mode;
     return SPG_OK;
 }
 

@@ -99,6 +99,9 @@ struct ProductPlan {
     int64_t *pack_off = nullptr;
     bool compact = false;                // large rows take the compact value kernel (B columns <= kCompactCols)
     uint32_t *lsave = nullptr;           // compact: the large rows' saved presence bitmaps, sized for this plan
+    bool part_counts = false;            // the large-row count writes per-window counts (part_available)
+    bool part = false;                   // the large rows the compact kernel leaves are partitioned
+    PartState ps{};                      // their state (pw, pd from the count; the rest for the values)
 };
 
 }  // namespace spg
@@ -122,10 +125,12 @@ struct spg_context {
     spg::DevBuf halo_err, halo_cnt, halo_tmp;  // device halo directory and packing
     spg::DevBuf pack_cnt, pack_off, pack_map;  // packed single-pass tiles
     spg::DevBuf lsave;                          // large rows' presence bitmaps (compact value kernel)
+    spg::DevBuf ppw, ppd, ppst, ppgap, ppoff, pscol, psval, ptask, psegb, pseg;  // partitioned large rows
     int path = SPG_PATH_AUTO;
     int fused_rows = spg::kFusedRows;
     bool force_wide = false;  // SPG_FORMAT_WIDE: 64-bit entry stream and B addressing whatever B's size
     bool allow_direct = true; // products whose rows all fit the single pass skip the analysis (not for batch plans)
+    int large_mode = SPG_LARGE_AUTO;  // SPG_LARGE_WINDOWS: no partitioned large rows
     int64_t lbits_width = 0;
     // results (ping-pong for compaction)
     spg::DevBuf rowptr[2], ccol[2], cval[2];
@@ -141,7 +146,8 @@ struct spg_context {
                                &ebst, &elen, &ub, &bin_rows, &snnz, &annz, &scan_tmp, &ctrl, &lbits, &lacc,
                                &tile_status, &tile_distinct, &part, &dir32, &halo_err, &halo_cnt, &halo_tmp,
                                &pack_cnt, &pack_off, &pack_map, &rowptr[0], &rowptr[1], &ccol[0], &ccol[1],
-                               &cval[0], &cval[1]};
+                               &cval[0], &cval[1], &ppw, &ppd, &ppst, &ppgap, &ppoff, &pscol, &psval, &ptask,
+                               &psegb, &pseg};
         for (spg::DevBuf *b : bufs) f(*b);
     }
     uint64_t device_bytes() {

@@ -65,6 +65,21 @@ constexpr int kTpbLargeHbm = 512;
 // distinct columns, from the presence bitmap the count kernel saves per row
 // (kCompactCols / 8 bytes each).
 constexpr int64_t kCompactCols = int64_t(1) << 16;
+// Partitioned large rows: the count kernel also counts each large row's
+// products and distinct columns per window of kPartCols columns; a partition
+// kernel moves the row's products (window-local column and product value) to
+// HBM scratch, window by window and in serial order inside each window; fold
+// tasks of about kPartTaskProducts products then fold each window into an LDS
+// accumulator.  Used when the count keeps every column of B in its LDS bitmap
+// (at most kPartMaxWin windows) and the scratch fits the device.
+constexpr int kPartLog = 12;
+constexpr int64_t kPartCols = int64_t(1) << kPartLog;
+constexpr int kPartMaxWin = 256;
+constexpr int64_t kPartTaskProducts = int64_t(1) << 15;
+constexpr int kTpbPart = 256;
+constexpr int kCapPart = 1024;
+constexpr int kTpbFold = 256;
+constexpr int kCapFold = 1024;
 constexpr int kCapLargeCompact = 1024;
 constexpr int kTpbLargeCompact = 512;
 // Single-pass path: rows with at most kFusedCap products and A entries are
@@ -127,12 +142,44 @@ struct Control {
     unsigned long long packed_tiles;  // tiles of the packed single-pass kernel
     unsigned long long max_a_row;     // direct single pass: longest row of A
     unsigned long long max_b_row;     // direct single pass: longest row of B
+    unsigned long long ntasks;        // partitioned large rows: fold tasks written
+    unsigned long long part_products; // partitioned large rows: their products (scratch entries)
+    unsigned long long part_max_row;  // partitioned large rows: the largest product count of one
+    unsigned long long part_units;    // partitioned large rows: batches of kCapLarge A entries of all large rows
+    int part_next;                    // next batch of the partition kernel
+    int fold_next;                    // next fold task
 };
 
 // Per-block partial sums of the analyze pass, reduced by one small kernel
 // (a device-scope atomic on one address costs tens of nanoseconds, so the
 // hot kernels never add into the control block per wave).
 constexpr int kAnalyzeFields = 2 + kNumBins;  // products, ub of bins W1/W2, rows per bin
+
+// A fold task: windows [w0, w1) of large row i (list index).
+struct PartTask {
+    int64_t s0;    // scratch position of window w0's products
+    int64_t out0;  // C position of window w0's first entry (structural layout)
+    int32_t i;
+    int16_t w0, w1;
+};
+
+// State of the partitioned large-row path (large row i of the list at
+// i * nwin in the per-window arrays).
+struct PartState {
+    int64_t *pw;     // [nlarge * nwin] products per window (count kernel)
+    int32_t *pd;     // [nlarge * nwin] distinct columns per window (count kernel)
+    int64_t *segbase;  // [2 * (nlarge + 1)] batches of kCapLarge A entries per large row, then their exclusive scan
+    uint32_t *pseg;    // [batches * nwin] products per window before each batch of its row (count kernel)
+    int32_t *pst;    // [nlarge * nwin] stored entries per window (fold kernel)
+    int32_t *pgap;   // [nlarge] nonzero when a window of the row dropped exact zeros
+    int64_t *poff;   // [2 * (nlarge + 1)] products of each partitioned row, then their exclusive scan
+    uint16_t *scol;  // [scratch] window-local columns
+    double *sval;    // [scratch] products
+    PartTask *task;  // [max_tasks]
+    int64_t max_tasks;
+    int64_t skip_le;  // rows with snnz <= skip_le belong to the compact kernel (-1: none)
+    int nwin;
+};
 
 // Outputs of the single-pass kernel.
 struct FusedOut {
@@ -220,8 +267,28 @@ constexpr unsigned kAllBins = kSmallBins | kBigBins;
 // With `lsave`, the large-row count also saves each row's presence bitmap
 // (row i of the large-row list at word i * lwidth/32; B's columns must fit
 // one window) for the compact value kernel.
+// With `part` (partition counts: part_available) the large-row count also
+// writes each large row's products and distinct columns per window of
+// kPartCols columns (pw, pd), the products per window before each batch of
+// kCapLarge A entries (pseg), and sums the products and the largest product
+// count of the rows with snnz > part->skip_le into the control block.
 void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream,
-                     uint32_t *lsave = nullptr);
+                     uint32_t *lsave = nullptr, const PartState *part = nullptr);
+// Batches of kCapLarge A entries of every large row and their exclusive scan
+// (ps.segbase; total into ctrl->part_units), before the large-row count
+// (large_part.hip).
+void launch_part_batches(const Operands &op, const Workspace &ws, const PartState &ps, int nlarge,
+                         hipStream_t stream);
+// Whether the large rows of a product with `bcols` B columns can be
+// partitioned when their count runs over windows of `lwidth` columns: the
+// count keeps all of B's columns in its LDS bitmap, in at most kPartMaxWin
+// windows.
+bool part_available(int64_t bcols, int64_t lwidth);
+inline int part_windows(int64_t bcols) { return (int)((bcols + kPartCols - 1) >> kPartLog); }
+// Rows with snnz at most this are the compact value kernel's (its LDS
+// accumulator holds them in one pass), or -1 when it does not run for windows
+// of `lwidth` columns.
+int64_t compact_take_le(int64_t lwidth);
 // Per-process kernel attributes (dynamic LDS limits); called when a context
 // is created, with the device's LDS bytes per workgroup.
 hipError_t prepare_kernels(size_t lds_per_block);
@@ -262,10 +329,18 @@ void launch_scan(const int64_t *in, int64_t *out, int64_t n, int64_t *tmp, unsig
 // to `stream` with the events fork/join), beside the HBM-window kernel: the
 // window kernel's heaviest rows bound its duration while its other workgroups
 // finish early, and the compact kernel's workgroups fill those CUs.
+// With `part`, the large rows that the compact kernel does not take are
+// partitioned and folded (launch_part) instead of folded onto HBM windows.
 void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, unsigned bins, int cus,
                     const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream,
                     const uint32_t *lsave = nullptr, hipStream_t side = nullptr, hipEvent_t fork = nullptr,
-                    hipEvent_t join = nullptr);
+                    hipEvent_t join = nullptr, const PartState *part = nullptr);
+// The partitioned large-row path (large_part.hip): scratch offsets of the
+// partitioned rows, the partition kernel (which also writes the fold tasks),
+// the fold kernel and the move of windows left apart by dropped zeros.
+// nlarge: rows in the large-row list.
+void launch_part(const Operands &op, const Workspace &ws, const PartState &ps, int nlarge, int cus,
+                 const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream);
 // Whether the compact large-row value kernel runs for windows of `lwidth`
 // columns on this device (its LDS accumulator holds at least one chunk).
 bool compact_large_available(int64_t lwidth);

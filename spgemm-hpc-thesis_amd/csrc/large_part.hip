@@ -1,0 +1,523 @@
+// large_part.hip -- large rows by partition: products moved to HBM scratch by
+// column window, then every window folded in an LDS accumulator.
+//
+// Semantics are the reference's (src/baselines/spgemm_exchange.cpp:274-309):
+// per output row, products a*b in serial order (A entries in stored order,
+// then B entries in stored order), a column's first product starts its sum and
+// later ones are added left to right, columns ascending, sums equal to 0.0
+// dropped.
+//
+// A large row's columns are cut into windows of kPartCols columns.  The count
+// kernel has already counted each window's products and distinct columns.
+//   1. k_large_part (one workgroup per row, heaviest rows first) streams the
+//      row's products in serial order and moves each one -- its column inside
+//      the window (16 bits) and its product value -- to the window's region of
+//      the scratch.  The split is stable: each wave takes a contiguous range of
+//      the serial order, and a lane's position among the wave's items of the
+//      same window comes from ballots over the window id's bits; waves are then
+//      ordered by a scan per window.  So every window's products stay in serial
+//      order.  The kernel also cuts the row into fold tasks of about
+//      kPartTaskProducts products (whole windows).
+//   2. k_large_fold (tasks on demand) folds a task's windows one at a time into
+//      a dense LDS accumulator of kPartCols sums that start at -0.0
+//      (-0.0 + p == p for every p, which is "the first product starts the sum").
+//      Each chunk of the window's products is split the same stable way by
+//      column modulo the block size, so that thread g owns the columns
+//      congruent to g and walks their products in serial order, keeping a
+//      column's running sum in a register while its products follow one
+//      another.  No sort, no atomics on values.  The window's sums are then
+//      written in column order at the window's place in the structural layout.
+//   3. k_large_gapfix moves windows together in the rare rows where a window
+//      dropped exact zeros (the stored count of the row goes to annz, and the
+//      usual compaction closes the gaps between rows).
+#include "device_common.hpp"
+
+namespace spg {
+namespace {
+
+using namespace dev;
+
+constexpr uint32_t kPartMask = (uint32_t)(kPartCols - 1);
+
+// Bits that tell n keys apart.
+__device__ __forceinline__ int key_bits(int n) { return n <= 1 ? 0 : 32 - __clz(n - 1); }
+
+// Lanes of the wave holding the same key as this lane, among lanes where
+// `valid` (nbits is uniform).
+__device__ __forceinline__ uint64_t same_key_lanes(uint32_t key, bool valid, int nbits) {
+    uint64_t m = __ballot(valid);
+    for (int b = 0; b < nbits; ++b) {
+        const bool bit = (key >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+// Stable split of the wave's ITEMS items per lane (the wave's serial order:
+// item j of lane l is item 64 j + l) by key: rank[j] is the item's position
+// among the wave's earlier items of the same key; cnt[key * stride] (zeroed
+// by the caller) receives the wave's count per key.  Only this wave touches
+// its counters.
+template <int ITEMS>
+__device__ __forceinline__ void wave_multisplit(const uint32_t (&key)[ITEMS], const bool (&valid)[ITEMS], int nbits,
+                                                int *cnt, int stride, int (&rank)[ITEMS]) {
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t m = same_key_lanes(key[j], valid[j], nbits);
+        const int before = __popcll(m & lt);
+        const int base = valid[j] ? cnt[key[j] * stride] : 0;
+        rank[j] = base + before;
+        wave_sync();  // every lane has read its counter before the group's first lane updates it
+        if (valid[j] && before == 0) cnt[key[j] * stride] = base + __popcll(m);
+        wave_sync();
+    }
+}
+
+// Batches of kCapLarge A entries of every large row (the count kernel's
+// batches, which the partition kernel takes one at a time).
+__global__ void k_part_batches(Operands op, Workspace ws, PartState ps, int count) {
+    const RowRec *list = large_list(ws);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const int row = list[i].row;
+        ps.segbase[i] = (op.aptr[row + 1] - op.aptr[row] + kCapLarge - 1) / kCapLarge;
+    }
+}
+
+// Scratch sizes of the large rows: a partitioned row's products, 0 for the
+// rows the compact kernel takes.
+__global__ void k_part_sizes(Workspace ws, PartState ps, int count) {
+    const RowRec *list = large_list(ws);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const int row = list[i].row;
+        ps.poff[i] = ws.snnz[row] > ps.skip_le ? ws.ub[row] : 0;
+    }
+}
+
+template <int TPB, int CAP>
+__global__ __launch_bounds__(TPB) void k_large_part(Operands op, Workspace ws, PartState ps, const int64_t *crowptr) {
+    static_assert(TPB >= kPartMaxWin, "one thread per window");
+    static_assert(CAP == kCapLarge, "a unit of work is one batch of the count kernel");
+    constexpr int NW = TPB / 64;
+    constexpr int ITEMS = CAP / TPB;
+    __shared__ int64_t s_bst[CAP];
+    __shared__ double s_av[CAP];
+    __shared__ int64_t s_off[CAP + 1];
+    __shared__ int64_t s_ws64[NW + 1];
+    __shared__ int64_t s_wb[kPartMaxWin];   // next scratch position of each window
+    __shared__ int64_t s_np[kPartMaxWin];   // products per window
+    __shared__ int64_t s_pex[kPartMaxWin];  // their exclusive scan
+    __shared__ int64_t s_dex[kPartMaxWin];  // exclusive scan of the distinct columns per window
+    __shared__ int s_cnt[NW][kPartMaxWin];  // per-wave counts, then the waves' offsets inside a chunk
+    __shared__ int s_tot[kPartMaxWin];      // a chunk's items per window
+    __shared__ int s_next;
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    const RowRec *list = large_list(ws);
+    const int count = ws.ctrl->bin_count[kBinLarge];
+    const int64_t units = (int64_t)ws.ctrl->part_units;
+    const int64_t *ubase = ps.segbase + count + 1;  // first batch of each large row
+    const int nwin = ps.nwin;
+    const int nbits = key_bits(nwin);
+    unsigned flags = 0;
+    // One unit of work is one batch of kCapLarge A entries of a row: the
+    // count kernel recorded each window's products before every batch, so
+    // the batches of one row go to their windows' places independently.
+    for (int u; (u = next_large_row(&ws.ctrl->part_next, &s_next)) < units;) {
+        const int i = upper_entry(ubase, count, (int64_t)u);
+        const int64_t bidx = (int64_t)u - ubase[i];
+        const int row = list[i].row;
+        if (ws.snnz[row] <= ps.skip_le) continue;  // the compact kernel's
+        const size_t rb = (size_t)i * (size_t)nwin;
+        const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
+        const int64_t e0 = a0 + bidx * CAP;
+        const bool last = e0 + CAP >= a1;
+        const int64_t np = tid < nwin ? ps.pw[rb + tid] : 0;
+        const int64_t nd = tid < nwin ? (int64_t)ps.pd[rb + tid] : 0;
+        int64_t ptot, dtot;
+        const int64_t pex = block_excl_scan<TPB>(np, s_ws64, ptot);
+        const int64_t dex = block_excl_scan<TPB>(nd, s_ws64, dtot);
+        const int64_t sbase = ps.poff[count + 1 + i];
+        const int64_t send = ps.poff[count + 2 + i];
+        if (tid < nwin) {
+            const size_t seg = (size_t)u * (size_t)nwin + tid;
+            s_np[tid] = np;
+            s_pex[tid] = pex;
+            s_dex[tid] = dex;
+            s_wb[tid] = sbase + pex + ps.pseg[seg];
+        }
+        if (ptot != ws.ub[row] || dtot != ws.snnz[row] || send - sbase != ptot || e0 >= a1) {
+            // the counts disagree: nothing of the row is written, and the
+            // fold kernel (which reads the flags) folds nothing
+            flags |= kErrInternal;
+            continue;
+        }
+        __syncthreads();
+        if (tid == 0 && bidx == 0) {
+            ws.annz[row] = 0;  // the fold tasks add their stored counts
+            // Fold tasks: consecutive windows until a task holds
+            // kPartTaskProducts products; empty windows are skipped.
+            int nt = 0;
+            int64_t acc = 0;
+            bool open = false;
+            for (int k = 0; k < nwin; ++k) {
+                const int64_t n = s_np[k];
+                if (n == 0) continue;
+                if (!open || acc >= kPartTaskProducts) {
+                    ++nt;
+                    acc = 0;
+                    open = true;
+                }
+                acc += n;
+            }
+            const unsigned long long t0 = atomicAdd(&ws.ctrl->ntasks, (unsigned long long)nt);
+            if ((int64_t)(t0 + nt) > ps.max_tasks) {
+                flags |= kErrInternal;
+            } else {
+                int64_t t = (int64_t)t0;
+                PartTask cur{};
+                acc = 0;
+                open = false;
+                for (int k = 0; k < nwin; ++k) {
+                    const int64_t n = s_np[k];
+                    if (n == 0) continue;
+                    if (!open || acc >= kPartTaskProducts) {
+                        if (open) {
+                            cur.w1 = (int16_t)k;
+                            ps.task[t++] = cur;
+                        }
+                        cur.s0 = sbase + s_pex[k];
+                        cur.out0 = crowptr[row] + s_dex[k];
+                        cur.i = i;
+                        cur.w0 = (int16_t)k;
+                        acc = 0;
+                        open = true;
+                    }
+                    acc += n;
+                }
+                if (open) {
+                    cur.w1 = (int16_t)nwin;
+                    ps.task[t++] = cur;
+                }
+            }
+        }
+        // The batch's products, in serial order, to their windows.
+        const int nbt = (int)min((int64_t)CAP, a1 - e0);
+        {
+            // each thread a run of consecutive entries: B row start, A value,
+            // and the exclusive offset of the entry's products
+            const int ipt = (nbt + TPB - 1) / TPB;
+            int64_t run = 0;
+            for (int j = 0; j < ipt; ++j) {
+                const int e = tid * ipt + j;
+                if (e < nbt) {
+                    const int64_t len = ws.elen[e0 + e];
+                    s_bst[e] = entry_bstart_any(ws, e0 + e);
+                    s_av[e] = op.aval[e0 + e];
+                    s_off[e] = len;
+                    run += len;
+                }
+            }
+            int64_t Pb;
+            run = block_excl_scan<TPB>(run, s_ws64, Pb);
+            for (int j = 0; j < ipt; ++j) {
+                const int e = tid * ipt + j;
+                if (e < nbt) {
+                    const int64_t len = s_off[e];
+                    s_off[e] = run;
+                    run += len;
+                }
+            }
+            __syncthreads();
+            for (int64_t c0 = 0; c0 < Pb; c0 += CAP) {
+                uint32_t key[ITEMS];
+                bool ok[ITEMS];
+                uint16_t lc[ITEMS];
+                double v[ITEMS];
+                int rank[ITEMS];
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    const int64_t s = c0 + (int64_t)wv * (64 * ITEMS) + j * 64 + lane;
+                    ok[j] = s < Pb;
+                    key[j] = 0u;
+                    lc[j] = 0;
+                    v[j] = 0.0;
+                    if (ok[j]) {
+                        const int e = upper_entry(s_off, nbt, s);
+                        const int64_t idx = s_bst[e] + (s - s_off[e]);
+                        const uint32_t c = (uint32_t)op.bcol[idx];
+                        v[j] = s_av[e] * op.bval[idx];
+                        key[j] = c >> kPartLog;
+                        lc[j] = (uint16_t)(c & kPartMask);
+                        if ((int)key[j] >= nwin) {  // outside B's columns: cannot be counted
+                            ok[j] = false;
+                            flags |= kErrInternal;
+                        }
+                    }
+                }
+                for (int q = lane; q < nwin; q += 64) s_cnt[wv][q] = 0;
+                wave_sync();
+                wave_multisplit<ITEMS>(key, ok, nbits, s_cnt[wv], 1, rank);
+                __syncthreads();
+                if (tid < nwin) {
+                    int r = 0;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) {
+                        const int c = s_cnt[w][tid];
+                        s_cnt[w][tid] = r;
+                        r += c;
+                    }
+                    s_tot[tid] = r;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    if (ok[j]) {
+                        const int64_t dst = s_wb[key[j]] + s_cnt[wv][key[j]] + rank[j];
+                        if (dst < send) {
+                            ps.scol[dst] = lc[j];
+                            ps.sval[dst] = v[j];
+                        } else {
+                            flags |= kErrInternal;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (tid < nwin) s_wb[tid] += s_tot[tid];
+                __syncthreads();
+            }
+        }
+        // every window filled exactly up to where the next batch starts
+        if (tid < nwin) {
+            const int64_t want = sbase + s_pex[tid] +
+                                 (last ? s_np[tid] : (int64_t)ps.pseg[((size_t)u + 1) * (size_t)nwin + tid]);
+            if (s_wb[tid] != want) flags |= kErrInternal;
+        }
+        __syncthreads();  // window state is rewritten by the next unit
+    }
+    if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
+}
+
+template <int TPB, int CAP>
+__global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, PartState ps, int32_t *ccol,
+                                                    double *cval) {
+    constexpr int NW = TPB / 64;
+    constexpr int ITEMS = CAP / TPB;
+    constexpr int CPT = (int)(kPartCols / TPB);  // columns per thread at extraction
+    static_assert(kPartCols % TPB == 0 && (TPB & (TPB - 1)) == 0, "column owners");
+    __shared__ double s_acc[kPartCols];
+    __shared__ uint32_t s_pres[kPartCols / 4];  // one presence byte per column
+    __shared__ uint16_t s_lc[CAP];
+    __shared__ double s_lv[CAP];
+    __shared__ int s_gc[TPB * NW];  // per (owner, wave) counts, then their offsets; owner-major
+    __shared__ int s_ws[NW + 1];
+    __shared__ int64_t s_ws64[NW + 1];
+    __shared__ int s_next;
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    const RowRec *list = large_list(ws);
+    // Tasks of a partition that reported an inconsistency are not folded.
+    const unsigned long long nt = ws.ctrl->ntasks;
+    const int ntasks = (ws.ctrl->flags & kErrInternal) || nt > (unsigned long long)ps.max_tasks ? 0 : (int)nt;
+    const int nwin = ps.nwin;
+    const int obits = key_bits(TPB);
+    uint8_t *pres = reinterpret_cast<uint8_t *>(s_pres);
+    unsigned flags = 0;
+    unsigned long long stored = 0;
+    for (int t; (t = next_large_row(&ws.ctrl->fold_next, &s_next)) < ntasks;) {
+        const PartTask task = ps.task[t];
+        const int i = task.i;
+        const int row = list[i].row;
+        const size_t rb = (size_t)i * (size_t)nwin;
+        int64_t s = task.s0, out = task.out0;
+        unsigned long long task_stored = 0;
+        for (int w = task.w0; w < task.w1; ++w) {
+            const int64_t n = ps.pw[rb + w];
+            const int d = ps.pd[rb + w];
+            if (n == 0) continue;
+            for (int q = tid; q < (int)kPartCols; q += TPB) s_acc[q] = -0.0;
+            for (int q = tid; q < (int)(kPartCols / 4); q += TPB) s_pres[q] = 0u;
+            // each chunk's products are loaded one chunk ahead (while the
+            // previous chunk is walked)
+            uint16_t nlc[ITEMS];
+            double nv[ITEMS];
+            auto load = [&](int64_t c0) {
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    const int64_t y = c0 + (int64_t)wv * (64 * ITEMS) + j * 64 + lane;
+                    nlc[j] = y < n ? ps.scol[s + y] : (uint16_t)0;
+                    nv[j] = y < n ? ps.sval[s + y] : 0.0;
+                }
+            };
+            load(0);
+            __syncthreads();
+            for (int64_t c0 = 0; c0 < n; c0 += CAP) {
+                uint32_t key[ITEMS];
+                bool ok[ITEMS];
+                uint16_t lc[ITEMS];
+                double v[ITEMS];
+                int rank[ITEMS];
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    const int64_t y = c0 + (int64_t)wv * (64 * ITEMS) + j * 64 + lane;
+                    ok[j] = y < n;
+                    lc[j] = nlc[j];
+                    v[j] = nv[j];
+                    key[j] = (uint32_t)lc[j] & (uint32_t)(TPB - 1);
+                }
+                for (int g = lane; g < TPB; g += 64) s_gc[g * NW + wv] = 0;
+                wave_sync();
+                wave_multisplit<ITEMS>(key, ok, obits, &s_gc[wv], NW, rank);
+                __syncthreads();
+                const int total = block_scan_runs<TPB>(s_gc, TPB * NW, s_ws);
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    if (ok[j]) {
+                        const int dst = s_gc[key[j] * NW + wv] + rank[j];
+                        s_lc[dst] = lc[j];
+                        s_lv[dst] = v[j];
+                    }
+                }
+                if (c0 + CAP < n) load(c0 + CAP);
+                __syncthreads();
+                // thread tid owns the columns congruent to tid: their products
+                // in serial order, eight read ahead at a time; a column's sum
+                // stays in a register while its products follow one another
+                {
+                    constexpr int kAhead = 8;
+                    const int y0 = s_gc[tid * NW];
+                    const int y1 = tid + 1 < TPB ? s_gc[(tid + 1) * NW] : total;
+                    int cur = -1;
+                    double cv = 0.0;
+                    for (int y = y0; y < y1; y += kAhead) {
+                        int cc[kAhead];
+                        double xx[kAhead];
+#pragma unroll
+                        for (int q = 0; q < kAhead; ++q) {
+                            const bool in = y + q < y1;
+                            cc[q] = in ? (int)s_lc[y + q] : -1;
+                            xx[q] = in ? s_lv[y + q] : 0.0;
+                        }
+#pragma unroll
+                        for (int q = 0; q < kAhead; ++q) {
+                            const int c = cc[q];
+                            if (c < 0) continue;
+                            if (c == cur) {
+                                cv = cv + xx[q];
+                            } else {
+                                if (cur >= 0) s_acc[cur] = cv;
+                                cur = c;
+                                cv = s_acc[c] + xx[q];
+                                pres[c] = 1;
+                            }
+                        }
+                    }
+                    if (cur >= 0) s_acc[cur] = cv;
+                }
+                __syncthreads();
+            }
+            // the window's sums in column order, exact zeros dropped unless kept
+            int mine = 0;
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) {
+                const int c = tid * CPT + u;
+                mine += pres[c] && (op.keep_zeros || s_acc[c] != 0.0) ? 1 : 0;
+            }
+            int64_t tot;
+            int64_t o = block_excl_scan<TPB>((int64_t)mine, s_ws64, tot);
+            if (tot > d) {
+                flags |= kErrInternal;
+            } else {
+                const int32_t cbase = (int32_t)((int64_t)w << kPartLog);
+#pragma unroll
+                for (int u = 0; u < CPT; ++u) {
+                    const int c = tid * CPT + u;
+                    if (pres[c] && (op.keep_zeros || s_acc[c] != 0.0)) {
+                        ccol[out + o] = cbase + c;
+                        cval[out + o] = s_acc[c];
+                        ++o;
+                    }
+                }
+                if (tid == 0) {
+                    ps.pst[rb + w] = (int32_t)tot;
+                    if (tot < d) ps.pgap[i] = 1;
+                }
+                task_stored += (unsigned long long)tot;
+            }
+            out += d;
+            s += n;
+            __syncthreads();  // the accumulator is cleared for the next window
+        }
+        if (tid == 0 && task_stored)
+            atomicAdd(reinterpret_cast<unsigned long long *>(&ws.annz[row]), task_stored);
+        stored += task_stored;
+    }
+    if (tid == 0 && stored) atomicAdd(&ws.ctrl->actual, stored);
+    if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
+}
+
+// Rows where a window dropped exact zeros: the windows' stored runs move
+// together to the row's start, in column order (each run moves left, copied
+// in ascending blocks, so no entry is overwritten before it is read).
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_large_gapfix(Workspace ws, PartState ps, const int64_t *crowptr,
+                                                      int32_t *ccol, double *cval) {
+    const RowRec *list = large_list(ws);
+    const int count = ws.ctrl->bin_count[kBinLarge];
+    const int nwin = ps.nwin;
+    for (int i = blockIdx.x; i < count; i += gridDim.x) {
+        if (!ps.pgap[i]) continue;
+        const int row = list[i].row;
+        const size_t rb = (size_t)i * (size_t)nwin;
+        int64_t dst = crowptr[row], src = crowptr[row];
+        for (int w = 0; w < nwin; ++w) {
+            const int d = ps.pd[rb + w];
+            const int st = ps.pw[rb + w] > 0 ? ps.pst[rb + w] : 0;
+            if (dst != src) {
+                for (int q0 = 0; q0 < st; q0 += TPB) {
+                    const int q = q0 + threadIdx.x;
+                    int32_t c = 0;
+                    double v = 0.0;
+                    if (q < st) {
+                        c = ccol[src + q];
+                        v = cval[src + q];
+                    }
+                    __syncthreads();
+                    if (q < st) {
+                        ccol[dst + q] = c;
+                        cval[dst + q] = v;
+                    }
+                    __syncthreads();
+                }
+            }
+            dst += st;
+            src += d;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_part_batches(const Operands &op, const Workspace &ws, const PartState &ps, int nlarge,
+                         hipStream_t stream) {
+    if (nlarge <= 0) return;
+    const int gs = (nlarge + 255) / 256 < 1024 ? (nlarge + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_part_batches, dim3(gs), dim3(256), 0, stream, op, ws, ps, nlarge);
+    launch_scan(ps.segbase, ps.segbase + nlarge + 1, nlarge, ws.scan_tmp, &ws.ctrl->part_units, stream);
+}
+
+void launch_part(const Operands &op, const Workspace &ws, const PartState &ps, int nlarge, int cus,
+                 const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream) {
+    if (nlarge <= 0) return;
+    launch_zero_bytes(ps.pgap, (size_t)nlarge * 4, stream);
+    const int gs = (nlarge + 255) / 256 < 1024 ? (nlarge + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_part_sizes, dim3(gs), dim3(256), 0, stream, ws, ps, nlarge);
+    launch_scan(ps.poff, ps.poff + nlarge + 1, nlarge, ws.scan_tmp, nullptr, stream);
+    const int gp = cus * 4;
+    hipLaunchKernelGGL((k_large_part<kTpbPart, kCapPart>), dim3(gp), dim3(kTpbPart), 0, stream, op, ws, ps, crowptr);
+    hipLaunchKernelGGL((k_large_fold<kTpbFold, kCapFold>), dim3(cus * 3), dim3(kTpbFold), 0, stream, op, ws, ps,
+                       ccol, cval);
+    hipLaunchKernelGGL((k_large_gapfix<256>), dim3(gp), dim3(256), 0, stream, ws, ps, crowptr, ccol, cval);
+}
+
+}  // namespace spg

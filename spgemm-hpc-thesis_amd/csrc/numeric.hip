@@ -849,6 +849,12 @@ bool compact_large_available(int64_t lwidth) {
     return acc >= kCapLargeCompact;
 }
 
+int64_t compact_take_le(int64_t lwidth) {
+    if (!compact_large_available(lwidth)) return -1;
+    const int dyn = g_compact_dyn.load(std::memory_order_relaxed);
+    return (int64_t)((int)(((int64_t)dyn - lwidth / 4) / 8) & ~63);
+}
+
 bool prepare_numeric_kernels(size_t lds_per_block) {
     {
         // The compact large-row kernel's accumulator takes whatever LDS its
@@ -882,7 +888,7 @@ bool prepare_numeric_kernels(size_t lds_per_block) {
 
 void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_counts, unsigned bins, int cus,
                     const int64_t *crowptr, int32_t *ccol, double *cval, hipStream_t stream, const uint32_t *lsave,
-                    hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+                    hipStream_t side, hipEvent_t fork, hipEvent_t join, const PartState *part) {
     if (bins & (1u << kBinW1))
         launch_esc<64, kCapW1>(op, ws, kBinW1, bin_counts[kBinW1], cus, crowptr, ccol, cval, stream);
     if (bins & (1u << kBinW2))
@@ -895,11 +901,11 @@ void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_coun
         launch_esc<kTpbBlock, kCapBlock>(op, ws, kBinBlock, bin_counts[kBinBlock], cus, crowptr, ccol, cval, stream);
     if ((bins & (1u << kBinLarge)) && bin_counts[kBinLarge] > 0) {
         // Rows whose distinct columns fit the compact kernel's accumulator
-        // (bitmaps saved by the count) fold in LDS; wider rows fold onto the
-        // HBM windows.  Each kernel skips the other's rows.
+        // (bitmaps saved by the count) fold in LDS; the others are
+        // partitioned by column window (`part`) or fold onto the HBM
+        // windows.  Each kernel skips the other's rows.
         const bool compact = lsave && compact_large_available(ws.lwidth);
-        const int dyn = g_compact_dyn.load(std::memory_order_relaxed);
-        const int acc_cap = compact ? (int)(((int64_t)dyn - ws.lwidth / 4) / 8) & ~63 : 0;
+        const int acc_cap = compact ? (int)compact_take_le(ws.lwidth) : 0;
         const int64_t skip_le = compact ? acc_cap : -1;
         const bool forked = compact && side && fork && join;
         if (forked) {
@@ -907,7 +913,9 @@ void launch_numeric(const Operands &op, const Workspace &ws, const int *bin_coun
             (void)hipStreamWaitEvent(side, fork, 0);
         }
         const int g = bin_counts[kBinLarge] < ws.lblocks ? bin_counts[kBinLarge] : ws.lblocks;
-        if (ws.lwidth <= kLdsBitsCols)
+        if (part)
+            launch_part(op, ws, *part, bin_counts[kBinLarge], cus, crowptr, ccol, cval, stream);
+        else if (ws.lwidth <= kLdsBitsCols)
             hipLaunchKernelGGL((k_large_numeric<kTpbLarge, kCapLarge, true>), dim3(g), dim3(kTpbLarge),
                                (unsigned)(ws.lwidth / 8), stream, op, ws, crowptr, ccol, cval, skip_le);
         else if (ws.lwidth <= kLdsBitsColsWide && g_num_wide.load(std::memory_order_relaxed))

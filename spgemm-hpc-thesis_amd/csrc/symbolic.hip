@@ -227,14 +227,18 @@ __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Wo
 // claims are LDS atomics; else it is the workgroup's bitmap in HBM.  With
 // `lsave` (LDSB, B's columns in one window) each row's bitmap is also saved,
 // row i of the large-row list at word i * lwidth/32, for the compact value
-// kernel.
+// kernel.  With `pw` (LDSB, B's columns in one window; partitioned large
+// rows) each row's products and distinct columns per window of kPartCols
+// columns go to pw[i * nwin + k] and pd[i * nwin + k], and the rows with
+// snnz > part_skip_le add their product counts into the control block.
 template <int TPB, int CAP, bool LDSB>
-__global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace ws, uint32_t *lsave) {
+__global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace ws, uint32_t *lsave, PartState ps) {
     constexpr int UNROLL = LDSB ? 2 * kLargeUnroll : kLargeUnroll;
     extern __shared__ uint32_t s_lbits[];
     __shared__ int64_t s_bst[CAP];
     __shared__ int64_t s_off[CAP + 1];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
+    __shared__ uint32_t s_pw[kPartMaxWin];  // products per partition window (32-bit: rows past 2^31 are not partitioned)
     const int tid = threadIdx.x;
     const RowRec *list = large_list(ws);
     const int count = ws.ctrl->bin_count[kBinLarge];
@@ -243,8 +247,15 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
     uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
     unsigned flags = 0;
     unsigned long long sym = 0;
+    unsigned long long part_sum = 0, part_max = 0;
+    const bool part = LDSB && ps.pw != nullptr;
+    const int nwin = ps.nwin;
+    static_assert(!LDSB || CAP == kCapLarge, "partition batches are the count's batches");
     if (LDSB) {
         for (int64_t q = tid; q < W / 32; q += TPB) s_lbits[q] = 0u;
+    }
+    if (part) {
+        for (int k = tid; k < kPartMaxWin; k += TPB) s_pw[k] = 0u;
     }
     __shared__ int s_next;
     for (int i; (i = next_large_row(&ws.ctrl->large_next[0], &s_next)) < count;) {
@@ -255,6 +266,10 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
             const int64_t wend = min(w0 + W, (int64_t)op.bcols);
             for (int64_t e0 = a0; e0 < a1; e0 += CAP) {
                 const int nbt = (int)min((int64_t)CAP, a1 - e0);
+                if (part) {  // the window counts before this batch (every claim of the previous one is done)
+                    const int64_t u = ps.segbase[count + 1 + i] + (e0 - a0) / CAP;
+                    for (int k = tid; k < nwin; k += TPB) ps.pseg[(size_t)u * (size_t)nwin + k] = s_pw[k];
+                }
                 const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, nullptr, s_off, s_ws64, false);
                 // UNROLL products per thread per step: every column load,
                 // then every bit claim, in flight together.  With the bitmap
@@ -295,10 +310,45 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                     }
 #pragma unroll
                     for (int u = 0; u < UNROLL; ++u) cnt += in[u] && !(old[u] & bit[u]);
+                    if (part) {
+                        // consecutive lanes hold consecutive products (a B
+                        // row's columns ascend): one add per run of lanes in
+                        // the same window
+                        // (lanes past the batch's last product have left the
+                        // loop: they end runs too)
+                        const int lane = threadIdx.x & 63;
+                        const uint64_t active = __ballot(true);
+#pragma unroll
+                        for (int u = 0; u < UNROLL; ++u) {
+                            const uint32_t wk = in[u] ? (uint32_t)(c[u] - w0) >> kPartLog : 0xffffffffu;
+                            const uint32_t prev = (uint32_t)__shfl_up((int)wk, 1, 64);
+                            const bool start = in[u] && (lane == 0 || prev != wk);
+                            const uint64_t ends = __ballot(start) | __ballot(!in[u]) | ~active;
+                            if (start) {
+                                const uint64_t rest = lane == 63 ? 0ull : ends >> (lane + 1);
+                                const uint32_t len = rest ? (uint32_t)__builtin_ctzll(rest) + 1u : (uint32_t)(64 - lane);
+                                atomicAdd(&s_pw[wk], len);
+                            }
+                        }
+                    }
                 }
                 __syncthreads();  // batch metadata is overwritten next
             }
             const int64_t nwords = (wend - w0 + 31) / 32;
+            if (part) {
+                // per window: its products, and its distinct columns (the set
+                // bits of its kPartCols / 32 bitmap words)
+                constexpr int64_t kWords = kPartCols / 32;
+                for (int k = tid; k < nwin; k += TPB) {
+                    int d = 0;
+                    const int64_t q1 = min((int64_t)(k + 1) * kWords, nwords);
+                    for (int64_t q = (int64_t)k * kWords; q < q1; ++q) d += __popc(s_lbits[q]);
+                    ps.pw[(size_t)i * (size_t)nwin + k] = (int64_t)s_pw[k];
+                    ps.pd[(size_t)i * (size_t)nwin + k] = d;
+                    s_pw[k] = 0u;
+                }
+                __syncthreads();  // bitmap read before it is cleared below
+            }
             uint32_t *save = LDSB && lsave ? lsave + (size_t)i * (size_t)(W / 32) : nullptr;
             for (int64_t q = tid; q < nwords; q += TPB) {
                 if (LDSB) {
@@ -314,8 +364,17 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
         const int64_t total = block_sum<TPB>(cnt, s_ws64);
         if (tid == 0) ws.snnz[row] = total;
         sym += (unsigned long long)total;
+        if (part && total > ps.skip_le) {
+            const unsigned long long p = (unsigned long long)ws.ub[row];
+            part_sum += p;
+            part_max = p > part_max ? p : part_max;
+        }
     }
     if (tid == 0 && sym) atomicAdd(&ws.ctrl->sym_total, sym);
+    if (tid == 0 && part_sum) {
+        atomicAdd(&ws.ctrl->part_products, part_sum);
+        atomicMax(&ws.ctrl->part_max_row, part_max);
+    }
     if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
 }
 
@@ -337,7 +396,8 @@ hipError_t prepare_kernels(size_t lds_per_block) {
     // dynamic LDS beside its static arrays; without that much LDS per
     // workgroup it is not used.
     const size_t dyn = (size_t)(kLdsBitsColsWide / 8);
-    const size_t stat = sizeof(int64_t) * (2 * kCapLarge + 1) + sizeof(int64_t) * (kTpbLargeWide / 64 + 1) + 64;
+    const size_t stat = sizeof(int64_t) * (2 * kCapLarge + 1) + sizeof(int64_t) * (kTpbLargeWide / 64 + 1) +
+                        sizeof(uint32_t) * kPartMaxWin + 64;
     bool ok = lds_per_block >= dyn + stat;
     if (ok) {
         const hipError_t e = hipFuncSetAttribute(
@@ -353,8 +413,13 @@ hipError_t prepare_kernels(size_t lds_per_block) {
     return hipSuccess;
 }
 
+bool part_available(int64_t bcols, int64_t lwidth) {
+    if (bcols <= 0 || bcols > lwidth || part_windows(bcols) > kPartMaxWin) return false;
+    return lwidth <= kLdsBitsCols || (lwidth <= kLdsBitsColsWide && g_wide_lds.load(std::memory_order_relaxed));
+}
+
 void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream,
-                     uint32_t *lsave) {
+                     uint32_t *lsave, const PartState *part) {
     if (op.arows <= 0) return;
     if (bins & (1u << kBinW1)) launch_hash<64, kCapW1>(op, ws, kBinW1, cus, stream);
     if (bins & (1u << kBinW2)) launch_hash<64, kCapW2>(op, ws, kBinW2, cus, stream);
@@ -362,16 +427,21 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
     if (bins & (1u << kBinW4)) launch_hash<kSymTpbW4, kCapW4>(op, ws, kBinW4, cus, stream);
     if (bins & (1u << kBinBlock)) launch_hash<kSymTpbBlock, kCapBlock>(op, ws, kBinBlock, cus, stream);
     if ((bins & (1u << kBinLarge)) && ws.lblocks > 0) {
+        PartState ps{};  // no partition counts unless asked for and possible
+        ps.skip_le = -1;
+        if (part && part->pw && part->pd && part->pseg && part->segbase && part_available(op.bcols, ws.lwidth))
+            ps = *part;
         if (ws.lwidth <= kLdsBitsCols)
             hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, true>), dim3(ws.lblocks), dim3(kTpbLarge),
-                               (unsigned)(ws.lwidth / 8), stream, op, ws, op.bcols <= ws.lwidth ? lsave : nullptr);
+                               (unsigned)(ws.lwidth / 8), stream, op, ws, op.bcols <= ws.lwidth ? lsave : nullptr,
+                               ps);
         else if (ws.lwidth <= kLdsBitsColsWide && g_wide_lds.load(std::memory_order_relaxed))
             hipLaunchKernelGGL((k_large_symbolic<kTpbLargeWide, kCapLarge, true>), dim3(ws.lblocks),
                                dim3(kTpbLargeWide), (unsigned)(ws.lwidth / 8), stream, op, ws,
-                               (uint32_t *)nullptr);
+                               (uint32_t *)nullptr, ps);
         else
             hipLaunchKernelGGL((k_large_symbolic<kTpbLargeHbm, kCapLargeHbm, false>), dim3(ws.lblocks),
-                               dim3(kTpbLargeHbm), 0, stream, op, ws, (uint32_t *)nullptr);
+                               dim3(kTpbLargeHbm), 0, stream, op, ws, (uint32_t *)nullptr, PartState{});
     }
 }
 

@@ -32,7 +32,7 @@ EXPORTED = [
     "spg_accumulator_create", "spg_accumulator_add", "spg_accumulator_finish", "spg_accumulator_copy",
     "spg_accumulator_destroy", "spg_matrix_read_mm", "spg_matrix_view", "spg_matrix_free",
     "spg_matrix_last_error", "spg_matrix_upload", "spg_device_matrix_view", "spg_device_matrix_free",
-    "spg_synchronize", "spg_set_path", "spg_set_format", "spg_multiply_rows", "spg_result_rows",
+    "spg_synchronize", "spg_set_path", "spg_set_format", "spg_set_large", "spg_multiply_rows", "spg_result_rows",
     "spg_multiply_noalloc",
     "spg_batch_create", "spg_batch_create_chained", "spg_batch_refresh", "spg_batch_run", "spg_batch_count",
     "spg_batch_set_graph",
@@ -51,6 +51,7 @@ EXPORTED = [
 
 PATH_AUTO, PATH_SINGLE_PASS, PATH_TWO_PHASE, PATH_SINGLE_PASS_DEALT, PATH_SINGLE_PASS_PACKED = 0, 1, 2, 3, 4
 FORMAT_AUTO, FORMAT_WIDE = 0, 1
+LARGE_AUTO, LARGE_WINDOWS = 0, 1
 
 
 @dataclass
@@ -87,14 +88,14 @@ class Stats(C.Structure):
                 ("rows_per_bin", C.c_int64 * 8), ("compacted", C.c_int32), ("bins", C.c_int32),
                 ("single_pass", C.c_int32), ("wide_format", C.c_int32),
                 ("ms_total", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double),
-                ("direct", C.c_int32), ("reserved_", C.c_int32)]
+                ("direct", C.c_int32), ("partitioned", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {"products": self.products, "structural_nnz": self.structural_nnz, "nnz": self.nnz,
                 "rows_per_bin": list(self.rows_per_bin)[: self.bins], "compacted": bool(self.compacted),
                 "single_pass": bool(self.single_pass), "wide_format": bool(self.wide_format),
                 "ms_total": self.ms_total, "ms_symbolic": self.ms_symbolic, "ms_numeric": self.ms_numeric,
-                "direct": bool(self.direct)}
+                "direct": bool(self.direct), "partitioned": bool(self.partitioned)}
 
 
 class FullExample(C.Structure):
@@ -161,6 +162,7 @@ def lib():
         L.spg_synchronize.argtypes = [vp]
         L.spg_set_path.argtypes = [vp, C.c_int]
         L.spg_set_format.argtypes = [vp, C.c_int]
+        L.spg_set_large.argtypes = [vp, C.c_int]
         L.spg_multiply_noalloc.argtypes = [vp, P(CsrView), P(CsrView), vp, vp, vp, C.c_int64, P(C.c_int64)]
         L.spg_result_rows.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, C.c_int64, P(C.c_int64)]
         L.spg_batch_create.argtypes = [vp, C.c_int32, P(CsrView), P(CsrView), P(vp)]
@@ -352,6 +354,11 @@ class Engine:
         whatever B's size; results are identical)."""
         self._check(self._L.spg_set_format(self._h, int(fmt)))
         self._format = int(fmt)
+
+    def set_large(self, mode: int):
+        """LARGE_AUTO or LARGE_WINDOWS (rows above 2048 products never
+        partitioned by column window; results are identical)."""
+        self._check(self._L.spg_set_large(self._h, int(mode)))
 
     def stats(self) -> dict:
         st = Stats()

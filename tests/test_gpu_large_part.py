@@ -1,0 +1,135 @@
+"""Rows above 2048 scalar products partitioned by windows of 4096 columns
+(SPG_LARGE_AUTO) and folded window by window in LDS: bit-exact against the
+serial-order oracle and identical to the HBM-window kernels
+(SPG_LARGE_WINDOWS), for rows whose windows cancel to exact zeros, rows cut
+into several fold tasks, hub columns taking most of a window's products,
+windows of many chunks, and C with more columns than the compact kernel
+takes."""
+import numpy as np
+import pytest
+
+import oracle as O
+import spgemm_amd as S
+from parity_util import digest, to_s
+
+pytestmark = pytest.mark.gpu
+
+
+def _csr(rows, cols, lens, col, val):
+    rp = np.zeros(rows + 1, dtype=np.int64)
+    np.cumsum(lens, out=rp[1:])
+    return O.Csr.make(rows, cols, rp, np.asarray(col, dtype=np.int64), np.asarray(val, dtype=np.float64))
+
+
+def _both_modes(engine, a, b, paths=(S.PATH_AUTO, S.PATH_TWO_PHASE)):
+    ref = O.csr_digest(O.product(a, b, threads=8))
+    for path in paths:
+        engine.set_path(path)
+        try:
+            for mode in (S.LARGE_AUTO, S.LARGE_WINDOWS):
+                print("path", path, "large mode", mode, flush=True)
+                engine.set_large(mode)
+                c = engine.multiply(to_s(a), to_s(b))
+                st = engine.stats()
+                assert digest(c) == ref, (path, mode)
+                if mode == S.LARGE_WINDOWS:
+                    assert not st["partitioned"]
+                yield mode, st, c
+        finally:
+            engine.set_large(S.LARGE_AUTO)
+            engine.set_path(S.PATH_AUTO)
+
+
+def test_partition_cancellation_tasks_hubs(engine):
+    rng = np.random.Generator(np.random.PCG64(71))
+    cols = 200_000  # 49 windows; above the compact kernel's 65,536 columns
+    nb = 4000
+    # B: rows of 40 random columns; rows 2k and 2k+1 are equal, so A entries
+    # (2k, x) and (2k+1, -x) cancel to exact zeros in whatever windows they hit
+    blen = np.full(nb, 40)
+    bcol = rng.integers(0, cols, size=nb * 40)
+    bval = rng.uniform(-1, 1, size=nb * 40)
+    bcol = bcol.reshape(nb, 40)
+    bval = bval.reshape(nb, 40)
+    bcol[1::2] = bcol[0::2]
+    bval[1::2] = bval[0::2]
+    b = _csr(nb, cols, blen, bcol.ravel(), bval.ravel())
+    # the same with every B row's first entry in column 7: a hub column that
+    # takes a large share of window 0's products
+    hub = O.Csr.make(nb, cols, b.row_ptr, np.where(np.arange(nb * 40) % 40 == 0, 7, b.col), b.val)
+    rows = []
+    # 0-7: cancelling pairs (some rows fully, some partly)
+    for r in range(8):
+        k = rng.choice(nb // 2, size=80, replace=False) * 2
+        x = rng.uniform(-1, 1, size=80)
+        cancel = 80 if r < 3 else 40
+        ac = np.concatenate([k, k[:cancel] + 1])
+        av = np.concatenate([x, -x[:cancel]])
+        rows.append((ac, av))
+    # 8-9: 2,000 entries each (80,000 products: several fold tasks and
+    # windows of several chunks)
+    for r in range(2):
+        ac = rng.integers(0, nb, size=2000)
+        rows.append((ac, rng.uniform(-1, 1, size=2000)))
+    # 10-19: rows of 60-120 entries (2,400-4,800 products)
+    for r in range(10):
+        n = int(rng.integers(60, 121))
+        rows.append((rng.integers(0, nb, size=n), rng.uniform(-1, 1, size=n)))
+    lens = [len(c) for c, _ in rows]
+    a = _csr(len(rows), nb, lens, np.concatenate([c for c, _ in rows]), np.concatenate([v for _, v in rows]))
+    for bb in (b, hub):
+        seen = {}
+        for mode, st, c in _both_modes(engine, a, bb):
+            seen[mode] = st["partitioned"]
+            assert st["rows_per_bin"][6] == len(rows)
+        assert seen[S.LARGE_AUTO]  # the partition ran
+    ref = O.product(a, b)
+    # rows 0-2 cancel but for columns two pairs share (the pairs' partial sums
+    # need not cancel exactly then): most of their ~3,200 columns drop
+    assert all(int(ref.row_ptr[r + 1] - ref.row_ptr[r]) < 300 for r in range(3))
+
+
+def test_partition_beside_compact_rows(engine):
+    # C of 40,000 columns: rows whose distinct columns fit the compact
+    # kernel's accumulator fold there, the wider ones are partitioned; both
+    # kinds in one product, through the batch pipeline too.
+    rng = np.random.Generator(np.random.PCG64(72))
+    cols, nb = 40_000, 3000
+    b = _csr(nb, cols, np.full(nb, 30), rng.integers(0, cols, size=nb * 30), rng.uniform(-1, 1, size=nb * 30))
+    lens = np.array([80] * 6 + [1500] * 3 + [120] * 6)  # 2,400 .. 45,000 products per row
+    a = _csr(len(lens), nb, lens, rng.integers(0, nb, size=int(lens.sum())), rng.uniform(-1, 1, size=int(lens.sum())))
+    seen = {}
+    for mode, st, c in _both_modes(engine, a, b):
+        seen[mode] = st["partitioned"]
+    assert seen[S.LARGE_AUTO]
+    want = O.csr_digest(O.product(a, b))
+    batch = S.Batch(engine, [to_s(a), to_s(a)], [to_s(b), to_s(b)])
+    batch.run()
+    assert digest(batch.result(0)) == want and digest(batch.result(1)) == want
+    assert batch.stats()["partitioned"]
+
+
+@pytest.mark.parametrize("fmt", [S.FORMAT_AUTO, S.FORMAT_WIDE])
+def test_partition_keep_zeros_accumulator(engine, fmt):
+    # The staged accumulator keeps structural zeros (keep_zeros): windows with
+    # exact zeros must keep them; equal under both large-row modes.
+    rng = np.random.Generator(np.random.PCG64(73))
+    cols, nb = 150_000, 2000
+    b = _csr(nb, cols, np.full(nb, 30), rng.integers(0, cols, size=nb * 30), rng.choice([-1.0, 1.0], size=nb * 30))
+    lens = np.full(6, 120)
+    a = _csr(6, nb, lens, rng.integers(0, nb, size=720), rng.choice([-1.0, 1.0], size=720))
+    out = {}
+    engine.set_format(fmt)
+    try:
+        for mode in (S.LARGE_AUTO, S.LARGE_WINDOWS):
+            engine.set_large(mode)
+            acc = S.Accumulator(engine, 6, cols)
+            acc.add_product(to_s(a), to_s(b))
+            acc.add_product(to_s(a), to_s(b))
+            out[mode] = digest(acc.finish())
+            acc.close()
+    finally:
+        engine.set_large(S.LARGE_AUTO)
+        engine.set_format(S.FORMAT_AUTO)
+    assert out[S.LARGE_AUTO] == out[S.LARGE_WINDOWS]
+    assert out[S.LARGE_AUTO] == O.csr_digest(O.accumulate(6, cols, [(a, b), (a, b)]))
