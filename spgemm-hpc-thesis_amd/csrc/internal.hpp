@@ -72,11 +72,14 @@ constexpr int64_t kCompactCols = int64_t(1) << 16;
 // tasks of about kPartTaskProducts products then fold each window into an LDS
 // accumulator.  Used when the count keeps every column of B in its LDS bitmap
 // (at most kPartMaxWin windows) and the scratch fits the device.
-constexpr int kPartLog = 12;
+#ifndef SPG_PART_LOG
+#define SPG_PART_LOG 12
+#endif
+constexpr int kPartLog = SPG_PART_LOG;
 constexpr int64_t kPartCols = int64_t(1) << kPartLog;
-constexpr int kPartMaxWin = 256;
+constexpr int kPartMaxWin = 1 << (20 - kPartLog);  // C of at most 2^20 columns
 constexpr int64_t kPartTaskProducts = int64_t(1) << 15;
-constexpr int kTpbPart = 256;
+constexpr int kTpbPart = kPartMaxWin > 256 ? kPartMaxWin : 256;
 constexpr int kCapPart = 1024;
 constexpr int kTpbFold = 256;
 constexpr int kCapFold = 1024;

@@ -21,12 +21,13 @@
 //   2. k_large_fold (tasks on demand) folds a task's windows one at a time into
 //      a dense LDS accumulator of kPartCols sums that start at -0.0
 //      (-0.0 + p == p for every p, which is "the first product starts the sum").
-//      Each chunk of the window's products is split the same stable way by
-//      column modulo the block size, so that thread g owns the columns
-//      congruent to g and walks their products in serial order, keeping a
-//      column's running sum in a register while its products follow one
-//      another.  No sort, no atomics on values.  The window's sums are then
-//      written in column order at the window's place in the structural layout.
+//      Each chunk of the window's products is sorted by its 12-bit window
+//      column with two stable 6-bit radix passes split the same way (so a
+//      column's products keep their serial order), and the first position of
+//      each column's run adds the run, left to right, to the column's sum.
+//      No value atomics.  The window's touched columns (a presence bitmap) are
+//      then written in column order at the window's place in the structural
+//      layout, and only their sums are reset.
 //   3. k_large_gapfix moves windows together in the rare rows where a window
 //      dropped exact zeros (the stored count of the row goes to annz, and the
 //      usual compaction closes the gaps between rows).
@@ -97,7 +98,7 @@ __global__ void k_part_sizes(Workspace ws, PartState ps, int count) {
 
 template <int TPB, int CAP>
 __global__ __launch_bounds__(TPB) void k_large_part(Operands op, Workspace ws, PartState ps, const int64_t *crowptr) {
-    static_assert(TPB >= kPartMaxWin, "one thread per window");
+    static_assert(TPB >= kPartMaxWin, "one thread per window");  // kTpbPart follows kPartMaxWin
     static_assert(CAP == kCapLarge, "a unit of work is one batch of the count kernel");
     constexpr int NW = TPB / 64;
     constexpr int ITEMS = CAP / TPB;
@@ -303,26 +304,57 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
                                                     double *cval) {
     constexpr int NW = TPB / 64;
     constexpr int ITEMS = CAP / TPB;
-    constexpr int CPT = (int)(kPartCols / TPB);  // columns per thread at extraction
-    static_assert(kPartCols % TPB == 0 && (TPB & (TPB - 1)) == 0, "column owners");
-    __shared__ double s_acc[kPartCols];
-    __shared__ uint32_t s_pres[kPartCols / 4];  // one presence byte per column
-    __shared__ uint16_t s_lc[CAP];
+    constexpr int NB = 64;                        // radix buckets of a pass (6 bits)
+    constexpr int WORDS = (int)(kPartCols / 32);  // presence bitmap of a window
+    static_assert(NB * NW == TPB && kPartLog > 6 && kPartLog <= 12,
+                  "two passes of at most 6 bits, one counter per thread");
+    static_assert(WORDS <= TPB, "one bitmap word per thread at extraction");
+    __shared__ double s_acc[kPartCols];  // running sums, -0.0 where untouched
+    __shared__ uint32_t s_bits[WORDS];   // columns touched in the window
+    __shared__ uint16_t s_lc[CAP];       // a chunk's products, sorted by column (then serial order)
     __shared__ double s_lv[CAP];
-    __shared__ int s_gc[TPB * NW];  // per (owner, wave) counts, then their offsets; owner-major
+    __shared__ int s_gc[NB * NW];        // per (bucket, wave) counts, then their offsets; bucket-major
     __shared__ int s_ws[NW + 1];
     __shared__ int64_t s_ws64[NW + 1];
     __shared__ int s_next;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
     const RowRec *list = large_list(ws);
     // Tasks of a partition that reported an inconsistency are not folded.
-    const unsigned long long nt = ws.ctrl->ntasks;
-    const int ntasks = (ws.ctrl->flags & kErrInternal) || nt > (unsigned long long)ps.max_tasks ? 0 : (int)nt;
+    const unsigned long long ntk = ws.ctrl->ntasks;
+    const int ntasks = (ws.ctrl->flags & kErrInternal) || ntk > (unsigned long long)ps.max_tasks ? 0 : (int)ntk;
     const int nwin = ps.nwin;
-    const int obits = key_bits(TPB);
-    uint8_t *pres = reinterpret_cast<uint8_t *>(s_pres);
+    for (int q = tid; q < (int)kPartCols; q += TPB) s_acc[q] = -0.0;
+    for (int q = tid; q < WORDS; q += TPB) s_bits[q] = 0u;
+    __syncthreads();
     unsigned flags = 0;
     unsigned long long stored = 0;
+    // One stable radix pass over a chunk's items (item j of lane l of wave
+    // wv is the chunk's item wv * 64 * ITEMS + 64 j + l), by `nbits` bits of
+    // the column from `shift`, into s_lc / s_lv.
+    auto radix_pass = [&](const uint16_t (&lc)[ITEMS], const double (&v)[ITEMS], const bool (&ok)[ITEMS], int shift,
+                          int nbits) {
+        uint32_t key[ITEMS];
+        int rank[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) key[j] = ((uint32_t)lc[j] >> shift) & (uint32_t)(NB - 1);
+        s_gc[lane * NW + wv] = 0;
+        wave_sync();
+        wave_multisplit<ITEMS>(key, ok, nbits, &s_gc[wv], NW, rank);
+        __syncthreads();
+        int tot;
+        const int ex = block_excl_scan<TPB>(s_gc[tid], s_ws, tot);
+        s_gc[tid] = ex;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            if (ok[j]) {
+                const int dst = s_gc[key[j] * NW + wv] + rank[j];
+                s_lc[dst] = lc[j];
+                s_lv[dst] = v[j];
+            }
+        }
+        __syncthreads();
+    };
     for (int t; (t = next_large_row(&ws.ctrl->fold_next, &s_next)) < ntasks;) {
         const PartTask task = ps.task[t];
         const int i = task.i;
@@ -334,10 +366,7 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
             const int64_t n = ps.pw[rb + w];
             const int d = ps.pd[rb + w];
             if (n == 0) continue;
-            for (int q = tid; q < (int)kPartCols; q += TPB) s_acc[q] = -0.0;
-            for (int q = tid; q < (int)(kPartCols / 4); q += TPB) s_pres[q] = 0u;
-            // each chunk's products are loaded one chunk ahead (while the
-            // previous chunk is walked)
+            // each chunk's products are loaded one chunk ahead
             uint16_t nlc[ITEMS];
             double nv[ITEMS];
             auto load = [&](int64_t c0) {
@@ -349,95 +378,71 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
                 }
             };
             load(0);
-            __syncthreads();
             for (int64_t c0 = 0; c0 < n; c0 += CAP) {
-                uint32_t key[ITEMS];
-                bool ok[ITEMS];
+                const int total = (int)min((int64_t)CAP, n - c0);
                 uint16_t lc[ITEMS];
                 double v[ITEMS];
-                int rank[ITEMS];
+                bool ok[ITEMS];
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
-                    const int64_t y = c0 + (int64_t)wv * (64 * ITEMS) + j * 64 + lane;
-                    ok[j] = y < n;
                     lc[j] = nlc[j];
                     v[j] = nv[j];
-                    key[j] = (uint32_t)lc[j] & (uint32_t)(TPB - 1);
-                }
-                for (int g = lane; g < TPB; g += 64) s_gc[g * NW + wv] = 0;
-                wave_sync();
-                wave_multisplit<ITEMS>(key, ok, obits, &s_gc[wv], NW, rank);
-                __syncthreads();
-                const int total = block_scan_runs<TPB>(s_gc, TPB * NW, s_ws);
-                __syncthreads();
-#pragma unroll
-                for (int j = 0; j < ITEMS; ++j) {
-                    if (ok[j]) {
-                        const int dst = s_gc[key[j] * NW + wv] + rank[j];
-                        s_lc[dst] = lc[j];
-                        s_lv[dst] = v[j];
-                    }
+                    ok[j] = wv * (64 * ITEMS) + j * 64 + lane < total;
                 }
                 if (c0 + CAP < n) load(c0 + CAP);
-                __syncthreads();
-                // thread tid owns the columns congruent to tid: their products
-                // in serial order, eight read ahead at a time; a column's sum
-                // stays in a register while its products follow one another
-                {
-                    constexpr int kAhead = 8;
-                    const int y0 = s_gc[tid * NW];
-                    const int y1 = tid + 1 < TPB ? s_gc[(tid + 1) * NW] : total;
-                    int cur = -1;
-                    double cv = 0.0;
-                    for (int y = y0; y < y1; y += kAhead) {
-                        int cc[kAhead];
-                        double xx[kAhead];
+                // stable sort by the window-local column: low 6 bits, then high
+                radix_pass(lc, v, ok, 0, 6);
 #pragma unroll
-                        for (int q = 0; q < kAhead; ++q) {
-                            const bool in = y + q < y1;
-                            cc[q] = in ? (int)s_lc[y + q] : -1;
-                            xx[q] = in ? s_lv[y + q] : 0.0;
-                        }
+                for (int j = 0; j < ITEMS; ++j) {
+                    const int x = wv * (64 * ITEMS) + j * 64 + lane;
+                    lc[j] = ok[j] ? s_lc[x] : (uint16_t)0;
+                    v[j] = ok[j] ? s_lv[x] : 0.0;
+                }
+                __syncthreads();  // read back before the second pass overwrites
+                radix_pass(lc, v, ok, 6, kPartLog - 6);
+                // each column's run folds, in serial order, onto its running sum
 #pragma unroll
-                        for (int q = 0; q < kAhead; ++q) {
-                            const int c = cc[q];
-                            if (c < 0) continue;
-                            if (c == cur) {
-                                cv = cv + xx[q];
-                            } else {
-                                if (cur >= 0) s_acc[cur] = cv;
-                                cur = c;
-                                cv = s_acc[c] + xx[q];
-                                pres[c] = 1;
-                            }
-                        }
-                    }
-                    if (cur >= 0) s_acc[cur] = cv;
+                for (int q = 0; q < ITEMS; ++q) {
+                    const int x = tid * ITEMS + q;
+                    if (x >= total) break;
+                    const int c = (int)s_lc[x];
+                    if (x > 0 && (int)s_lc[x - 1] == c) continue;
+                    double acc = s_acc[c];
+                    int y = x;
+                    do {
+                        acc = acc + s_lv[y];
+                        ++y;
+                    } while (y < total && (int)s_lc[y] == c);
+                    s_acc[c] = acc;
+                    atomicOr(&s_bits[c >> 5], 1u << (c & 31));
                 }
                 __syncthreads();
             }
-            // the window's sums in column order, exact zeros dropped unless kept
+            // the window's sums in column order, exact zeros dropped unless
+            // kept; the touched sums go back to -0.0 and the bitmap to zero
+            const uint32_t m = tid < WORDS ? s_bits[tid] : 0u;
             int mine = 0;
-#pragma unroll
-            for (int u = 0; u < CPT; ++u) {
-                const int c = tid * CPT + u;
-                mine += pres[c] && (op.keep_zeros || s_acc[c] != 0.0) ? 1 : 0;
+            for (uint32_t r = m; r; r &= r - 1u) {
+                const int c = tid * 32 + __ffs((int)r) - 1;
+                mine += op.keep_zeros || s_acc[c] != 0.0 ? 1 : 0;
             }
             int64_t tot;
             int64_t o = block_excl_scan<TPB>((int64_t)mine, s_ws64, tot);
-            if (tot > d) {
-                flags |= kErrInternal;
-            } else {
-                const int32_t cbase = (int32_t)((int64_t)w << kPartLog);
-#pragma unroll
-                for (int u = 0; u < CPT; ++u) {
-                    const int c = tid * CPT + u;
-                    if (pres[c] && (op.keep_zeros || s_acc[c] != 0.0)) {
-                        ccol[out + o] = cbase + c;
-                        cval[out + o] = s_acc[c];
-                        ++o;
-                    }
+            const bool fits = tot <= d;
+            if (!fits) flags |= kErrInternal;
+            const int32_t cbase = (int32_t)((int64_t)w << kPartLog);
+            for (uint32_t r = m; r; r &= r - 1u) {
+                const int c = tid * 32 + __ffs((int)r) - 1;
+                const double x = s_acc[c];
+                if (fits && (op.keep_zeros || x != 0.0)) {
+                    ccol[out + o] = cbase + c;
+                    cval[out + o] = x;
+                    ++o;
                 }
+                s_acc[c] = -0.0;
+            }
+            if (tid < WORDS) s_bits[tid] = 0u;
+            if (fits) {
                 if (tid == 0) {
                     ps.pst[rb + w] = (int32_t)tot;
                     if (tot < d) ps.pgap[i] = 1;
@@ -446,7 +451,7 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
             }
             out += d;
             s += n;
-            __syncthreads();  // the accumulator is cleared for the next window
+            __syncthreads();  // the accumulator is clean for the next window
         }
         if (tid == 0 && task_stored)
             atomicAdd(reinterpret_cast<unsigned long long *>(&ws.annz[row]), task_stored);
