@@ -9,15 +9,17 @@
 //
 // A large row's columns are cut into windows of kPartCols columns.  The count
 // kernel has already counted each window's products and distinct columns.
-//   1. k_large_part (one workgroup per row, heaviest rows first) streams the
-//      row's products in serial order and moves each one -- its column inside
-//      the window (16 bits) and its product value -- to the window's region of
-//      the scratch.  The split is stable: each wave takes a contiguous range of
-//      the serial order, and a lane's position among the wave's items of the
-//      same window comes from ballots over the window id's bits; waves are then
-//      ordered by a scan per window.  So every window's products stay in serial
-//      order.  The kernel also cuts the row into fold tasks of about
-//      kPartTaskProducts products (whole windows).
+//   1. k_large_part takes units of one batch of kCapLarge A entries of a row
+//      (rows heaviest first; the count kernel recorded every window's products
+//      before each batch, so a batch knows where its products go).  It streams
+//      the batch's products in serial order and moves each one -- its column
+//      inside the window (16 bits) and its product value -- to the window's
+//      region of the scratch.  The split is stable: each wave takes a
+//      contiguous range of the serial order, and a lane's position among the
+//      wave's items of the same window comes from ballots over the window id's
+//      bits; waves are then ordered by a scan per window.  So every window's
+//      products stay in serial order.  A row's first batch also cuts the row
+//      into fold tasks of about kPartTaskProducts products (whole windows).
 //   2. k_large_fold (tasks on demand) folds a task's windows one at a time into
 //      a dense LDS accumulator of kPartCols sums that start at -0.0
 //      (-0.0 + p == p for every p, which is "the first product starts the sum").
