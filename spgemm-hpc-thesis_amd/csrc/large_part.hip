@@ -310,14 +310,13 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
     constexpr int WORDS = (int)(kPartCols / 32);  // presence bitmap of a window
     static_assert(NB * NW == TPB && kPartLog > 6 && kPartLog <= 12,
                   "two passes of at most 6 bits, one counter per thread");
-    static_assert(WORDS <= TPB, "one bitmap word per thread at extraction");
+    static_assert(WORDS <= TPB && kPartCols == 16 * TPB, "extraction: half a bitmap word per thread");
     __shared__ double s_acc[kPartCols];  // running sums, -0.0 where untouched
     __shared__ uint32_t s_bits[WORDS];   // columns touched in the window
     __shared__ uint16_t s_lc[CAP];       // a chunk's products, sorted by column (then serial order)
     __shared__ double s_lv[CAP];
     __shared__ int s_gc[NB * NW];        // per (bucket, wave) counts, then their offsets; bucket-major
     __shared__ int s_ws[NW + 1];
-    __shared__ int64_t s_ws64[NW + 1];
     __shared__ int s_next;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
     const RowRec *list = large_list(ws);
@@ -402,54 +401,93 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
                 }
                 __syncthreads();  // read back before the second pass overwrites
                 radix_pass(lc, v, ok, 6, kPartLog - 6);
-                // each column's run folds, in serial order, onto its running sum
+                // Each column's run folds, in serial order, onto its running
+                // sum at the run's first position.  Position x = q * TPB + tid:
+                // a wave reads 64 consecutive sorted positions (no bank
+                // conflicts), and a run that ends inside them has its end from
+                // a ballot of the heads, so its values are read without waiting
+                // on each next column; only the wave's last run reads on.
 #pragma unroll
                 for (int q = 0; q < ITEMS; ++q) {
-                    const int x = tid * ITEMS + q;
-                    if (x >= total) break;
-                    const int c = (int)s_lc[x];
-                    if (x > 0 && (int)s_lc[x - 1] == c) continue;
-                    double acc = s_acc[c];
-                    int y = x;
-                    do {
-                        acc = acc + s_lv[y];
-                        ++y;
-                    } while (y < total && (int)s_lc[y] == c);
-                    s_acc[c] = acc;
-                    atomicOr(&s_bits[c >> 5], 1u << (c & 31));
+                    const int x = q * TPB + tid;
+                    const bool in = x < total;
+                    const int c = in ? (int)s_lc[x] : -1;
+                    const int pc = in && x > 0 ? (int)s_lc[x - 1] : -2;
+                    const bool head = in && pc != c;
+                    const uint64_t hm = __ballot(head);
+                    if (head) {
+                        const uint64_t above = lane == 63 ? 0ull : hm & (~0ull << (lane + 1));
+                        const int end = above ? x - lane + (__ffsll((long long)above) - 1) : min(x - lane + 64, total);
+                        double acc = s_acc[c];
+                        int y = x;
+                        for (; y + 2 <= end; y += 2) {
+                            const double p0 = s_lv[y], p1 = s_lv[y + 1];
+                            acc = acc + p0;
+                            acc = acc + p1;
+                        }
+                        if (y < end) acc = acc + s_lv[y];
+                        if (!above)
+                            for (y = end; y < total && (int)s_lc[y] == c; ++y) acc = acc + s_lv[y];
+                        s_acc[c] = acc;
+                        atomicOr(&s_bits[c >> 5], 1u << (c & 31));
+                    }
                 }
                 __syncthreads();
             }
-            // the window's sums in column order, exact zeros dropped unless
-            // kept; the touched sums go back to -0.0 and the bitmap to zero
-            const uint32_t m = tid < WORDS ? s_bits[tid] : 0u;
-            int mine = 0;
-            for (uint32_t r = m; r; r &= r - 1u) {
-                const int c = tid * 32 + __ffs((int)r) - 1;
-                mine += op.keep_zeros || s_acc[c] != 0.0 ? 1 : 0;
-            }
-            int64_t tot;
-            int64_t o = block_excl_scan<TPB>((int64_t)mine, s_ws64, tot);
-            const bool fits = tot <= d;
+            // The window's sums in column order, exact zeros dropped unless
+            // kept.  The touched columns are listed in column order first
+            // (thread t reads columns 16t..16t+15 of the bitmap), so that
+            // entry i of the list is read by thread i mod TPB and written to
+            // C coalesced.  The list takes the chunk's dead value storage.
+            uint16_t *s_list = reinterpret_cast<uint16_t *>(s_lv);
+            const uint32_t hw = (s_bits[tid >> 1] >> ((tid & 1) * 16)) & 0xffffu;
+            int tot;
+            int o = block_excl_scan<TPB>((int)__popc(hw), s_ws, tot);
+            for (uint32_t r = hw; r; r &= r - 1u) s_list[o++] = (uint16_t)(tid * 16 + __ffs((int)r) - 1);
+            __syncthreads();
+            if (tid < WORDS) s_bits[tid] = 0u;
+            const bool fits = tot == d;  // touched columns are the counted distinct columns
             if (!fits) flags |= kErrInternal;
             const int32_t cbase = (int32_t)((int64_t)w << kPartLog);
-            for (uint32_t r = m; r; r &= r - 1u) {
-                const int c = tid * 32 + __ffs((int)r) - 1;
+            int zeros = 0;
+            for (int e = tid; e < tot; e += TPB) {
+                const int c = s_list[e];
                 const double x = s_acc[c];
-                if (fits && (op.keep_zeros || x != 0.0)) {
-                    ccol[out + o] = cbase + c;
-                    cval[out + o] = x;
-                    ++o;
+                zeros |= !op.keep_zeros && x == 0.0;
+                if (fits) {
+                    __builtin_nontemporal_store(cbase + c, &ccol[out + e]);
+                    __builtin_nontemporal_store(x, &cval[out + e]);
                 }
-                s_acc[c] = -0.0;
             }
-            if (tid < WORDS) s_bits[tid] = 0u;
+            int st = tot;
+            if (__syncthreads_or(zeros) && fits) {
+                // Rare: sums of exactly zero.  Rounds of TPB list entries move
+                // the nonzero ones down over the window's range (a round reads
+                // before it writes, and writes below where later rounds read);
+                // the barrier orders them after the stores above.
+                __syncthreads();
+                st = 0;
+                for (int e0 = 0; e0 < tot; e0 += TPB) {
+                    const int e = e0 + tid;
+                    const double x = e < tot ? s_acc[s_list[e]] : 0.0;
+                    const int32_t cc = e < tot ? cbase + (int)s_list[e] : 0;
+                    const int nz = e < tot && x != 0.0 ? 1 : 0;
+                    int rt;
+                    const int ex = block_excl_scan<TPB>(nz, s_ws, rt);
+                    if (nz) {
+                        ccol[out + st + ex] = cc;
+                        cval[out + st + ex] = x;
+                    }
+                    st += rt;
+                }
+            }
+            for (int e = tid; e < tot; e += TPB) s_acc[s_list[e]] = -0.0;
             if (fits) {
                 if (tid == 0) {
-                    ps.pst[rb + w] = (int32_t)tot;
-                    if (tot < d) ps.pgap[i] = 1;
+                    ps.pst[rb + w] = (int32_t)st;
+                    if (st < d) ps.pgap[i] = 1;
                 }
-                task_stored += (unsigned long long)tot;
+                task_stored += (unsigned long long)st;
             }
             out += d;
             s += n;
