@@ -233,6 +233,29 @@ __device__ __forceinline__ int upper_entry(const T *off, int n, T s) {
     return lo;
 }
 
+// upper_entry for ITEMS queries at once (off[0] <= s[j] for valid queries;
+// others get some e in [0, n)).  The searches advance in lockstep for the
+// ceil(log2 n) steps any of them needs, so a thread's LDS reads of one step
+// are in flight together instead of one dependent chain per query.
+template <int ITEMS, typename T>
+__device__ __forceinline__ void upper_entries(const T *off, int n, const T (&s)[ITEMS], int (&e)[ITEMS]) {
+    int hi[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        e[j] = 0;
+        hi[j] = n - 1;
+    }
+    const int steps = n <= 1 ? 0 : 32 - __clz(n - 1);
+    for (int st = 0; st < steps; ++st) {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int mid = (e[j] + hi[j] + 1) >> 1;
+            if (off[mid] <= s[j]) e[j] = mid;
+            else hi[j] = mid - 1;
+        }
+    }
+}
+
 // Where the products of a row go, slot by slot: slot j of thread tid holds the
 // product (A entry e[j], B offset t[j]) when ok[j].  Entry-major when every
 // referenced B row fits a power-of-two lane group (L lanes per A entry, no
@@ -262,13 +285,17 @@ __device__ __forceinline__ void slot_products(int na, int P, int L, const int *o
             t[j] = sub;
         }
     } else {
+        int sx[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
-            const int s = tid + j * TPB;
-            ok[j] = s < P;
-            const int ee = ok[j] ? upper_entry(off, na, s) : 0;
-            e[j] = ee;
-            t[j] = ok[j] ? s - off[ee] : 0;
+            sx[j] = tid + j * TPB;
+            ok[j] = sx[j] < P;
+        }
+        upper_entries<ITEMS>(off, na, sx, e);
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            if (!ok[j]) e[j] = 0;
+            t[j] = ok[j] ? sx[j] - off[e[j]] : 0;
         }
     }
 }

@@ -231,12 +231,19 @@ __device__ __forceinline__ void expand_row(const Operands &op, FusedWave<SMALLB>
             it.sidx[j] = ok[j] ? o - sbase + sub : -1;
         }
     } else {
-        // Serial order dealt out lane by lane; entry found by binary search.
+        // Serial order dealt out lane by lane; entries found by binary
+        // searches in lockstep.
+        int sx[kFItems], ex[kFItems];
+#pragma unroll
+        for (int j = 0; j < kFItems; ++j) {
+            ok[j] = lane + 64 * j < P;
+            sx[j] = lane + 64 * j + sbase;
+        }
+        upper_entries<kFItems>(w.off + m, na, sx, ex);
 #pragma unroll
         for (int j = 0; j < kFItems; ++j) {
             const int s = lane + 64 * j;
-            ok[j] = s < P;
-            const int e = m + (ok[j] ? upper_entry(w.off + m, na, s + sbase) : 0);
+            const int e = m + (ok[j] ? ex[j] : 0);
             bs[j] = w.bst[e];
             it.av[j] = w.av[e];
             t[j] = ok[j] ? s + sbase - w.off[e] : 0;

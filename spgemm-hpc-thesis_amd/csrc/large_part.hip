@@ -232,6 +232,31 @@ __global__ __launch_bounds__(TPB) void k_large_part(Operands op, Workspace ws, P
                 }
             }
             __syncthreads();
+            // Chunks of CAP products in serial order, software pipelined: a
+            // chunk's B gathers are issued before the previous chunk is split
+            // and stored.
+            int ncol[ITEMS];
+            double nbv[ITEMS], nav[ITEMS];
+            bool nok[ITEMS];
+            auto issue = [&](int64_t c0) {
+                int64_t sx[ITEMS];
+                int ent[ITEMS];
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    sx[j] = c0 + (int64_t)wv * (64 * ITEMS) + j * 64 + lane;
+                    nok[j] = sx[j] < Pb;
+                }
+                upper_entries<ITEMS>(s_off, nbt, sx, ent);
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    const int e = ent[j];
+                    const int64_t idx = nok[j] ? s_bst[e] + (sx[j] - s_off[e]) : 0;
+                    nav[j] = s_av[e];
+                    ncol[j] = op.bcol[idx];
+                    nbv[j] = op.bval[idx];
+                }
+            };
+            if (Pb > 0) issue(0);
             for (int64_t c0 = 0; c0 < Pb; c0 += CAP) {
                 uint32_t key[ITEMS];
                 bool ok[ITEMS];
@@ -240,24 +265,18 @@ __global__ __launch_bounds__(TPB) void k_large_part(Operands op, Workspace ws, P
                 int rank[ITEMS];
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
-                    const int64_t s = c0 + (int64_t)wv * (64 * ITEMS) + j * 64 + lane;
-                    ok[j] = s < Pb;
-                    key[j] = 0u;
-                    lc[j] = 0;
-                    v[j] = 0.0;
-                    if (ok[j]) {
-                        const int e = upper_entry(s_off, nbt, s);
-                        const int64_t idx = s_bst[e] + (s - s_off[e]);
-                        const uint32_t c = (uint32_t)op.bcol[idx];
-                        v[j] = s_av[e] * op.bval[idx];
-                        key[j] = c >> kPartLog;
-                        lc[j] = (uint16_t)(c & kPartMask);
-                        if ((int)key[j] >= nwin) {  // outside B's columns: cannot be counted
-                            ok[j] = false;
-                            flags |= kErrInternal;
-                        }
+                    ok[j] = nok[j];
+                    const uint32_t c = (uint32_t)ncol[j];
+                    v[j] = nav[j] * nbv[j];
+                    key[j] = ok[j] ? c >> kPartLog : 0u;
+                    lc[j] = (uint16_t)(c & kPartMask);
+                    if (ok[j] && (int)key[j] >= nwin) {  // outside B's columns: cannot be counted
+                        ok[j] = false;
+                        key[j] = 0u;
+                        flags |= kErrInternal;
                     }
                 }
+                if (c0 + CAP < Pb) issue(c0 + CAP);
                 for (int q = lane; q < nwin; q += 64) s_cnt[wv][q] = 0;
                 wave_sync();
                 wave_multisplit<ITEMS>(key, ok, nbits, s_cnt[wv], 1, rank);

@@ -298,18 +298,21 @@ template <int TPB, int CAP, int ITEMS>
 __device__ __forceinline__ void load_chunk(const Operands &op, const int64_t *s_bst, const int64_t *s_off, int nbt,
                                            int64_t Pb, int64_t c0, int (&col)[ITEMS], double (&bv)[ITEMS],
                                            int (&ent)[ITEMS]) {
+    int64_t g[ITEMS];
+    int e[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) g[j] = c0 + threadIdx.x + j * TPB;
+    upper_entries<ITEMS>(s_off, nbt, g, e);
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-        const int64_t g = c0 + threadIdx.x + j * TPB;
         col[j] = -1;
         bv[j] = 0.0;
         ent[j] = 0;
-        if (g < Pb) {
-            const int e = upper_entry(s_off, nbt, g);
-            const int64_t idx = s_bst[e] + (g - s_off[e]);
+        if (g[j] < Pb) {
+            const int64_t idx = s_bst[e[j]] + (g[j] - s_off[e[j]]);
             col[j] = op.bcol[idx];
             bv[j] = op.bval[idx];
-            ent[j] = e;
+            ent[j] = e[j];
         }
     }
 }

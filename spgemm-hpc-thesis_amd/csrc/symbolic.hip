@@ -276,14 +276,15 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                 // in LDS the next step's column loads are issued before this
                 // step's claims (LDS atomics do not wait on them).
                 auto load_step = [&](int64_t s0, int (&c)[UNROLL]) {
+                    int64_t sx[UNROLL];
+                    int e[UNROLL];
+#pragma unroll
+                    for (int u = 0; u < UNROLL; ++u) sx[u] = s0 + (int64_t)u * TPB;
+                    upper_entries<UNROLL>(s_off, nbt, sx, e);
 #pragma unroll
                     for (int u = 0; u < UNROLL; ++u) {
-                        const int64_t s = s0 + (int64_t)u * TPB;
                         c[u] = -1;
-                        if (s < Pb) {
-                            const int e = upper_entry(s_off, nbt, s);
-                            c[u] = op.bcol[s_bst[e] + (s - s_off[e])];
-                        }
+                        if (sx[u] < Pb) c[u] = op.bcol[s_bst[e[u]] + (sx[u] - s_off[e[u]])];
                     }
                 };
                 constexpr int64_t kStep = (int64_t)TPB * UNROLL;
