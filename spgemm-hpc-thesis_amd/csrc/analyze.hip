@@ -32,6 +32,13 @@ __global__ void k_zero_words(uint32_t *p, size_t n) {
 constexpr int kAnalyzeTpb = 256;
 static_assert(kAnalyzeTpb / 64 * 64 == kAnalyzeRows, "packed single-pass tiles take an analyze block's rows");
 constexpr int kAnChunks = 2;  // 64-entry chunks in flight per wave (1 and 4 measured no faster)
+// Heavy tiles are the critical path of products with power-law rows: they
+// are cut into pieces of about kAnPiece entries that a separate kernel takes
+// on demand, each piece walked by the 16 waves of a 1024-thread workgroup,
+// each wave keeping more chunks in flight.
+constexpr int kAnHeavyChunks = 4;
+constexpr int kAnHeavyTpb = 1024;
+constexpr int64_t kAnPiece = 32768;
 
 // Longest row of A and of B (block maxima, one atomic per block each) and
 // B's row offsets in 32 bits.
@@ -78,17 +85,23 @@ __global__ void k_dir32(const int64_t *bptr, int64_t n, uint32_t *out) {
 }
 
 // Row tiles of 64 rows, one per wave.  A tile with more than kAnHeavy
-// entries (R-MAT's first rows hold hundreds of thousands) is walked by all
-// waves of its workgroup together, each taking a contiguous share of its
-// entries, so no single wave carries it.
+// entries (R-MAT's first rows hold hundreds of thousands) is left to
+// k_analyze_heavy.
 constexpr int64_t kAnHeavy = 16384;
+
+// Pieces of a heavy tile of E entries and r rows: at most one per row, so
+// that the pieces of all heavy tiles fit a list of one entry per row of A.
+__device__ __forceinline__ int64_t heavy_pieces(int64_t E, int rows) {
+    const int64_t p = (E + kAnPiece - 1) / kAnPiece;
+    return p < rows ? p : rows;
+}
 
 // Entries [S, T) of a row tile (lane l holds the tile's row l, starting at
 // entry a0; rows 0..last exist).  DIR32: B row k is [bdir32[k], bdir32[k+1])
 // (B's own row_ptr, 32-bit copy).  Writes each entry's (B row start, length)
 // and adds the lengths into s_ub[row] (LDS atomics: several waves may walk
 // one tile).  s_mark is this wave's scratch.
-template <bool DIR32>
+template <bool DIR32, int kAnChunks>
 __device__ __forceinline__ unsigned tile_entries(const Operands &op, const Workspace &ws, int last, int64_t a0,
                                                  int64_t S, int64_t T, unsigned long long *s_ub, int *s_mark) {
     const int lane = lane_id();
@@ -164,7 +177,21 @@ __device__ __forceinline__ unsigned tile_entries(const Operands &op, const Works
             const int seed = wave_max_dpp(lane <= last && a0 <= cb ? lane : -1);
             wave_sync();
             const int row = max(seed, wave_incl_max_dpp(s_mark[lane]));
-            if (inv[h]) atomicAdd(&s_ub[row], (unsigned long long)lv[h]);
+            // The lengths of a row's lanes (consecutive) are summed in the
+            // wave first: one LDS atomic per row of the chunk, not one per
+            // entry (64 on one address inside a long row).
+            const int64_t v = inv[h] ? lv[h] : 0;
+            if (__ballot(v >= ((int64_t)1 << 24)) == 0) {
+                const int incl = wave_incl_sum_dpp((int)v);
+                const int nrow = __shfl_down(row, 1, 64);
+                const int64_t ar = __shfl(a0, row, 64);
+                const int st = (int)max((int64_t)0, ar - cb);  // the row's first lane in the chunk
+                const int before = __shfl(incl, max(st, 1) - 1, 64);
+                const int64_t sum = (int64_t)incl - (st > 0 ? before : 0);
+                if ((lane == 63 || nrow != row) && sum) atomicAdd(&s_ub[row], (unsigned long long)sum);
+            } else if (inv[h]) {
+                atomicAdd(&s_ub[row], (unsigned long long)lv[h]);
+            }
         }
     }
     return flags;
@@ -218,7 +245,6 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
     __shared__ unsigned long long s_ub[NW][64];
     __shared__ int s_mark[NW][64];
     __shared__ int64_t s_part[NW][kAnalyzeFields];
-    __shared__ int64_t s_span[NW][2];  // entry range of each wave's tile when heavy, else empty
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
     const int64_t tile = (int64_t)blockIdx.x * NW + w;
@@ -234,27 +260,33 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
     s_ub[w][lane] = 0ull;
     unsigned flags = 0;
     if (has && op.nnz_a <= 0 && E1 > E0) flags |= kErrInternal;  // row_ptr inconsistent with nnz(A)
+    // Heavy tiles (their rows and their share of the block's partial sums)
+    // are k_analyze_heavy's: the tile's rows start from 0 in ub, and its
+    // pieces are listed in the storage of the empty bin's list (which lists
+    // no rows; k_analyze_reduce reuses it after).  Record of piece k: row =
+    // tile, na = k, a0 = index of piece 0 (piece 0: its pieces done, from 0).
     const bool heavy = has && E1 - E0 > kAnHeavy;
-    if (lane == 0) {
-        s_span[w][0] = heavy ? E0 : 0;
-        s_span[w][1] = heavy ? E1 : 0;
+    if (heavy) {
+        if (valid) ws.ub[r] = 0;
+        const int np = (int)heavy_pieces(E1 - E0, last + 1);
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&ws.ctrl->heavy_units, np);
+        base = __shfl(base, 0, 64);
+        RowRec *hl = ws.bin_recs + (size_t)kBinEmpty * (size_t)ws.list_stride;
+        for (int k = lane; k < np; k += 64) {
+            RowRec rec;
+            rec.row = (int32_t)tile;
+            rec.na = k;
+            rec.a0 = k == 0 ? 0 : base;
+            hl[base + k] = rec;
+        }
     }
     wave_sync();
-    if (has && !heavy) flags |= tile_entries<DIR32>(op, ws, last, a0, E0, E1, s_ub[w], s_mark[w]);
-    __syncthreads();
-    for (int h = 0; h < NW; ++h) {  // heavy tiles of the workgroup, walked by every wave
-        const int64_t hE0 = s_span[h][0], hE1 = s_span[h][1];
-        if (hE1 <= hE0) continue;  // uniform across the workgroup
-        const int64_t htile = (int64_t)blockIdx.x * NW + h;
-        const int64_t hr = htile * 64 + lane;
-        const int hlast = (int)min((int64_t)63, (int64_t)op.arows - 1 - htile * 64);
-        const int64_t ha0 = hr < op.arows ? op.aptr[hr] : 0;
-        const int64_t per = ((hE1 - hE0 + NW - 1) / NW + 63) / 64 * 64;
-        const int64_t S = hE0 + (int64_t)w * per, T = min(S + per, hE1);
-        if (S < T) flags |= tile_entries<DIR32>(op, ws, hlast, ha0, S, T, s_ub[h], s_mark[w]);
+    if (has && !heavy) {
+        flags |= tile_entries<DIR32, kAnChunks>(op, ws, last, a0, E0, E1, s_ub[w], s_mark[w]);
+        wave_sync();
+        tile_finish(op, ws, tile, a0, a1, s_ub[w], s_part[w]);
     }
-    __syncthreads();
-    if (has) tile_finish(op, ws, tile, a0, a1, s_ub[w], s_part[w]);
     unsigned long long fl = flags;
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) fl |= __shfl_xor(fl, d, 64);
@@ -266,6 +298,72 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
         for (int i = 0; i < NW; ++i) v += s_part[i][threadIdx.x];
         ws.part[(size_t)blockIdx.x * kAnalyzeFields + threadIdx.x] = v;
     }
+}
+
+// Pieces of heavy tiles, taken on demand.  The 16 waves walk contiguous
+// shares of a piece's entries into LDS row sums, which go to ub with global
+// atomics; the workgroup that completes a tile's last piece classifies the
+// tile's rows and adds the tile's partial sums to its analyze block's (whose
+// other tiles k_analyze has stored).
+template <bool DIR32>
+__global__ __launch_bounds__(kAnHeavyTpb) void k_analyze_heavy(Operands op, Workspace ws) {
+    constexpr int NW = kAnHeavyTpb / 64;
+    constexpr int TPW = kAnalyzeTpb / 64;  // tiles per analyze block
+    __shared__ unsigned long long s_ub[64];
+    __shared__ int s_mark[NW][64];
+    __shared__ int64_t s_part[kAnalyzeFields];
+    __shared__ int s_next, s_last;
+    const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
+    RowRec *hl = ws.bin_recs + (size_t)kBinEmpty * (size_t)ws.list_stride;
+    const int units = ws.ctrl->heavy_units;
+    unsigned flags = 0;
+    for (int u; (u = next_large_row(&ws.ctrl->heavy_next, &s_next)) < units;) {
+        const RowRec rec = hl[u];
+        const int64_t tile = rec.row;
+        const int first = rec.na == 0 ? u : (int)rec.a0;
+        const int64_t r0 = tile * 64;
+        const int last = (int)min((int64_t)63, (int64_t)op.arows - 1 - r0);
+        const int64_t E0 = op.aptr[r0], E1 = op.aptr[r0 + last + 1];
+        const int64_t np = heavy_pieces(E1 - E0, last + 1);
+        const int64_t pp = ((E1 - E0 + np - 1) / np + 63) / 64 * 64;
+        const int64_t P0 = min(E0 + (int64_t)rec.na * pp, E1), P1 = min(P0 + pp, E1);
+        const bool valid = lane <= last;
+        const int64_t a0 = valid ? op.aptr[r0 + lane] : 0;
+        const int64_t a1 = valid ? op.aptr[r0 + lane + 1] : 0;
+        if (threadIdx.x < 64) s_ub[threadIdx.x] = 0ull;
+        __syncthreads();
+        const int64_t per = ((P1 - P0 + NW - 1) / NW + 63) / 64 * 64;
+        const int64_t S = min(P0 + (int64_t)w * per, P1), T = min(S + per, P1);
+        if (S < T) flags |= tile_entries<DIR32, kAnHeavyChunks>(op, ws, last, a0, S, T, s_ub, s_mark[w]);
+        __syncthreads();
+        if (w == 0 && valid && s_ub[lane])
+            atomicAdd(reinterpret_cast<unsigned long long *>(&ws.ub[r0 + lane]), s_ub[lane]);
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0)
+            s_last = atomicAdd(reinterpret_cast<unsigned long long *>(&hl[first].a0), 1ull) ==
+                     (unsigned long long)(np - 1);
+        __syncthreads();
+        if (s_last && w == 0) {
+            __threadfence();
+            s_ub[lane] = valid ? __hip_atomic_load(reinterpret_cast<const unsigned long long *>(&ws.ub[r0 + lane]),
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : 0ull;
+            if (lane < kAnalyzeFields) s_part[lane] = 0;
+            wave_sync();
+            tile_finish(op, ws, tile, a0, a1, s_ub, s_part);
+            wave_sync();
+            if (lane < kAnalyzeFields && s_part[lane])
+                atomicAdd(reinterpret_cast<unsigned long long *>(&ws.part[(size_t)(tile / TPW) * kAnalyzeFields + lane]),
+                          (unsigned long long)s_part[lane]);
+        }
+        __syncthreads();  // piece state is reset for the next piece
+    }
+    unsigned long long fl = flags;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) fl |= __shfl_xor(fl, d, 64);
+    if (lane == 0 && fl) atomicOr(&ws.ctrl->flags, fl);
 }
 
 // Large-row work classes: two per power of two of P_row (its leading bit
@@ -369,10 +467,14 @@ int64_t analyze_blocks(int32_t rows) {
 void launch_analyze(const Operands &op, const Workspace &ws, hipStream_t stream) {
     if (op.arows <= 0) return;
     const int64_t blocks = analyze_blocks(op.arows);
-    if (ws.bdir32)
+    const unsigned gh = 256;  // workgroups taking heavy pieces (they exit at once when there are none)
+    if (ws.bdir32) {
         hipLaunchKernelGGL(k_analyze<true>, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
-    else
+        hipLaunchKernelGGL(k_analyze_heavy<true>, dim3(gh), dim3(kAnHeavyTpb), 0, stream, op, ws);
+    } else {
         hipLaunchKernelGGL(k_analyze<false>, dim3((unsigned)blocks), dim3(kAnalyzeTpb), 0, stream, op, ws);
+        hipLaunchKernelGGL(k_analyze_heavy<false>, dim3(gh), dim3(kAnHeavyTpb), 0, stream, op, ws);
+    }
     hipLaunchKernelGGL(k_analyze_reduce, dim3(1), dim3(kReduceTpb), 0, stream, ws.part, blocks, ws);
 }
 

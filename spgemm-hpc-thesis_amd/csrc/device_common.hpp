@@ -256,6 +256,45 @@ __device__ __forceinline__ void upper_entries(const T *off, int n, const T (&s)[
     }
 }
 
+// Entries of products dealt to a wave in groups of 64 consecutive products:
+// lane l holds product g0 + j * stride + l of group j (g0 and stride uniform
+// across the wave, which calls this with every lane; off[0] <= g0, off
+// nondecreasing, n entries).  Lane j finds the entry holding group j's first
+// product by a binary search (one search per lane, not one per item), and
+// every entry starting inside the group marks its first lane in `mark` (this
+// wave's 64 ints; among entries starting at one product, empty ones first,
+// the largest index wins): a prefix maximum over the lanes gives each lane
+// its entry.  One LDS read per lane and group instead of a search per item.
+template <int ITEMS, typename T>
+__device__ __forceinline__ void group_entries(const T *off, int n, T g0, T stride, int *mark, int (&e)[ITEMS]) {
+    const int lane = lane_id();
+    int first;
+    {
+        const T q = g0 + (T)min(lane, ITEMS - 1) * stride;
+        T qa[1] = {q};
+        int ea[1];
+        upper_entries<1>(off, n, qa, ea);
+        first = ea[0];
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const int e0 = __builtin_amdgcn_readlane(first, j);
+        const T g = g0 + (T)j * stride;
+        mark[lane] = -1;
+        wave_sync();
+        const T lim = g + 64;
+        for (int c0 = e0 + 1; c0 < n; c0 += 64) {
+            const int c = c0 + lane;
+            const T o = c < n ? off[c] : lim;
+            if (o < lim) atomicMax(&mark[(int)(o - g)], c);
+            if (!((__ballot(o < lim) >> 63) & 1ull)) break;  // lane 63's entry starts past the group: no later one starts inside
+        }
+        wave_sync();
+        e[j] = max(e0, wave_incl_max_dpp(mark[lane]));
+        wave_sync();  // every lane has read the marks before they are reset
+    }
+}
+
 // Where the products of a row go, slot by slot: slot j of thread tid holds the
 // product (A entry e[j], B offset t[j]) when ok[j].  Entry-major when every
 // referenced B row fits a power-of-two lane group (L lanes per A entry, no
@@ -429,9 +468,9 @@ __device__ __forceinline__ void meta_to_lds(const Operands &op, const Workspace 
 }
 
 template <typename K>
-int occupancy_grid(K kernel, int tpb, int cus, long long want) {
+int occupancy_grid(K kernel, int tpb, int cus, long long want, size_t dyn_lds = 0) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, tpb, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, tpb, dyn_lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
     long long g = (long long)per_cu * cus;
     if (want > 0 && want < g) g = want;

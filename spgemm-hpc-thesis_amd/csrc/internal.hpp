@@ -151,6 +151,8 @@ struct Control {
     unsigned long long part_units;    // partitioned large rows: batches of kCapLarge A entries of all large rows
     int part_next;                    // next batch of the partition kernel
     int fold_next;                    // next fold task
+    int heavy_units;                  // analyze: pieces of heavy row tiles listed
+    int heavy_next;                   // analyze: next piece to take
 };
 
 // Per-block partial sums of the analyze pass, reduced by one small kernel

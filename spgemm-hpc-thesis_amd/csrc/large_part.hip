@@ -114,6 +114,7 @@ __global__ __launch_bounds__(TPB) void k_large_part(Operands op, Workspace ws, P
     __shared__ int64_t s_dex[kPartMaxWin];  // exclusive scan of the distinct columns per window
     __shared__ int s_cnt[NW][kPartMaxWin];  // per-wave counts, then the waves' offsets inside a chunk
     __shared__ int s_tot[kPartMaxWin];      // a chunk's items per window
+    __shared__ int s_gmark[NW][64];         // each wave's entry marks (group_entries)
     __shared__ int s_next;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
     const RowRec *list = large_list(ws);
@@ -241,12 +242,13 @@ __global__ __launch_bounds__(TPB) void k_large_part(Operands op, Workspace ws, P
             auto issue = [&](int64_t c0) {
                 int64_t sx[ITEMS];
                 int ent[ITEMS];
+                const int64_t g0 = uni64(c0 + (int64_t)wv * (64 * ITEMS));
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
-                    sx[j] = c0 + (int64_t)wv * (64 * ITEMS) + j * 64 + lane;
+                    sx[j] = g0 + j * 64 + lane;
                     nok[j] = sx[j] < Pb;
                 }
-                upper_entries<ITEMS>(s_off, nbt, sx, ent);
+                group_entries<ITEMS>(s_off, nbt, g0, (int64_t)64, s_gmark[wv], ent);
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
                     const int e = ent[j];

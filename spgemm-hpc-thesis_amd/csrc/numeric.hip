@@ -294,15 +294,17 @@ __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Wo
 
 // Products s = c0 + tid + j*TPB of the current A-entry batch: B column, B
 // value and the batch entry e (its A value is s_av[e]); column -1 past Pb.
+// Every lane of the wave calls it (groups of 64 consecutive products; `mark`
+// is the wave's 64 ints).
 template <int TPB, int CAP, int ITEMS>
 __device__ __forceinline__ void load_chunk(const Operands &op, const int64_t *s_bst, const int64_t *s_off, int nbt,
-                                           int64_t Pb, int64_t c0, int (&col)[ITEMS], double (&bv)[ITEMS],
-                                           int (&ent)[ITEMS]) {
+                                           int64_t Pb, int64_t c0, int *mark, int (&col)[ITEMS],
+                                           double (&bv)[ITEMS], int (&ent)[ITEMS]) {
     int64_t g[ITEMS];
     int e[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) g[j] = c0 + threadIdx.x + j * TPB;
-    upper_entries<ITEMS>(s_off, nbt, g, e);
+    group_entries<ITEMS>(s_off, nbt, uni64(c0 + (int64_t)(threadIdx.x - lane_id())), (int64_t)TPB, mark, e);
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         col[j] = -1;
@@ -348,6 +350,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
         for (int64_t q = tid; q < W / 32; q += TPB) s_lbits[q] = 0u;
     }
     __shared__ int s_next;
+    __shared__ int s_gmark[TPB / 64][64];  // each wave's entry marks (group_entries)
     for (int i; (i = next_large_row(&ws.ctrl->large_next[1], &s_next)) < count;) {
         const int row = list[i].row;
         if (ws.snnz[row] <= skip_le) continue;  // taken by the compact kernel
@@ -406,7 +409,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                 // Each chunk's products are loaded one chunk ahead.
                 int pcol[ITEMS], pent[ITEMS];
                 double pbv[ITEMS];
-                load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, 0, pcol, pbv, pent);
+                load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, 0, s_gmark[tid >> 6], pcol, pbv, pent);
                 for (int64_t c0 = 0; c0 < Pb; c0 += CAP) {
                     int icol[ITEMS];
                     double ival[ITEMS];
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                             ++nvalid;
                         }
                     }
-                    load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, pcol, pbv, pent);
+                    load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, s_gmark[tid >> 6], pcol, pbv, pent);
                     const int V = block_sum<TPB>(nvalid, s_wsi);
                     if (V == 0) continue;  // uniform across the block; a pending chunk stays pending
                     uint64_t *const sk = s_key[cbuf];
@@ -650,6 +653,7 @@ __global__ __launch_bounds__(TPB) void k_large_compact(Operands op, Workspace ws
     __shared__ int s_wsi[2 * (TPB / 64) + 2];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
     __shared__ int s_next;
+    __shared__ int s_gmark[TPB / 64][64];  // each wave's entry marks (group_entries)
     const int tid = threadIdx.x;
     const RowRec *list = large_list(ws);
     const int count = ws.ctrl->bin_count[kBinLarge];
@@ -693,7 +697,7 @@ __global__ __launch_bounds__(TPB) void k_large_compact(Operands op, Workspace ws
                 // Each chunk's products are loaded one chunk ahead.
                 int pcol[ITEMS], pent[ITEMS];
                 double pbv[ITEMS];
-                load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, 0, pcol, pbv, pent);
+                load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, 0, s_gmark[tid >> 6], pcol, pbv, pent);
                 for (int64_t c0 = 0; c0 < Pb; c0 += CAP) {
                     int rk[ITEMS];
                     double ival[ITEMS];
@@ -719,7 +723,7 @@ __global__ __launch_bounds__(TPB) void k_large_compact(Operands op, Workspace ws
                             ++nvalid;
                         }
                     }
-                    load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, pcol, pbv, pent);
+                    load_chunk<TPB, CAP>(op, s_bst, s_off, nbt, Pb, c0 + CAP, s_gmark[tid >> 6], pcol, pbv, pent);
                     const int V = block_sum<TPB>(nvalid, s_wsi);
                     if (V == 0) continue;  // uniform across the block
                     block_minmax<TPB>(rmn, rmx, s_wsi);

@@ -239,12 +239,13 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
     __shared__ int64_t s_off[CAP + 1];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
     __shared__ uint32_t s_pw[kPartMaxWin];  // products per partition window (32-bit: rows past 2^31 are not partitioned)
-    const int tid = threadIdx.x;
+    __shared__ int s_gmark[TPB / 64][64];   // each wave's entry marks (group_entries)
+    const int tid = threadIdx.x, lane = tid & 63;
     const RowRec *list = large_list(ws);
     const int count = ws.ctrl->bin_count[kBinLarge];
     if (count == 0) return;  // launched before the host knows the bins; no row takes a ticket
     const int64_t W = ws.lwidth;
-    uint32_t *bits = ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);
+    uint32_t *bits = LDSB ? nullptr : ws.lbits + (size_t)blockIdx.x * (size_t)(W / 32);  // HBM bitmap: grid is lblocks
     unsigned flags = 0;
     unsigned long long sym = 0;
     unsigned long long part_sum = 0, part_max = 0;
@@ -275,12 +276,15 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                 // then every bit claim, in flight together.  With the bitmap
                 // in LDS the next step's column loads are issued before this
                 // step's claims (LDS atomics do not wait on them).
+                // Lane l of wave w holds products s0 + u * TPB (u < UNROLL),
+                // s0 = 64 w + l + k * TPB * UNROLL: groups of 64 consecutive
+                // products, whose entries group_entries finds together.
                 auto load_step = [&](int64_t s0, int (&c)[UNROLL]) {
                     int64_t sx[UNROLL];
                     int e[UNROLL];
 #pragma unroll
                     for (int u = 0; u < UNROLL; ++u) sx[u] = s0 + (int64_t)u * TPB;
-                    upper_entries<UNROLL>(s_off, nbt, sx, e);
+                    group_entries<UNROLL>(s_off, nbt, uni64(s0 - lane), (int64_t)TPB, s_gmark[tid >> 6], e);
 #pragma unroll
                     for (int u = 0; u < UNROLL; ++u) {
                         c[u] = -1;
@@ -290,7 +294,7 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                 constexpr int64_t kStep = (int64_t)TPB * UNROLL;
                 int cn[UNROLL];
                 if (LDSB) load_step(tid, cn);
-                for (int64_t s0 = tid; s0 < Pb; s0 += kStep) {
+                for (int64_t s0 = tid; s0 - lane < Pb; s0 += kStep) {  // uniform per wave
                     int c[UNROLL];
                     if (LDSB) {
 #pragma unroll
@@ -314,10 +318,8 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                     if (part) {
                         // consecutive lanes hold consecutive products (a B
                         // row's columns ascend): one add per run of lanes in
-                        // the same window
-                        // (lanes past the batch's last product have left the
-                        // loop: they end runs too)
-                        const int lane = threadIdx.x & 63;
+                        // the same window (lanes past the batch's last
+                        // product end runs)
                         const uint64_t active = __ballot(true);
 #pragma unroll
                         for (int u = 0; u < UNROLL; ++u) {
@@ -432,10 +434,14 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
         ps.skip_le = -1;
         if (part && part->pw && part->pd && part->pseg && part->segbase && part_available(op.bcols, ws.lwidth))
             ps = *part;
-        if (ws.lwidth <= kLdsBitsCols)
-            hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, true>), dim3(ws.lblocks), dim3(kTpbLarge),
-                               (unsigned)(ws.lwidth / 8), stream, op, ws, op.bcols <= ws.lwidth ? lsave : nullptr,
-                               ps);
+        if (ws.lwidth <= kLdsBitsCols) {
+            // The bitmap is in LDS, so the grid is not bound to the per-block
+            // HBM windows: every workgroup that fits takes rows.
+            const size_t dyn = (size_t)(ws.lwidth / 8);
+            const int g = occupancy_grid(k_large_symbolic<kTpbLarge, kCapLarge, true>, kTpbLarge, cus, 0, dyn);
+            hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, true>), dim3(g), dim3(kTpbLarge),
+                               (unsigned)dyn, stream, op, ws, op.bcols <= ws.lwidth ? lsave : nullptr, ps);
+        }
         else if (ws.lwidth <= kLdsBitsColsWide && g_wide_lds.load(std::memory_order_relaxed))
             hipLaunchKernelGGL((k_large_symbolic<kTpbLargeWide, kCapLarge, true>), dim3(ws.lblocks),
                                dim3(kTpbLargeWide), (unsigned)(ws.lwidth / 8), stream, op, ws,

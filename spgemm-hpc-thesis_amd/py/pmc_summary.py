@@ -3,6 +3,7 @@
     python spgemm-hpc-thesis_amd/py/pmc_summary.py counters <counter_collection.csv> [--kernel SUBSTR]
         [--dispatch last|all|median|N] [--segment MARKER:K] [--factor COUNTER=F ...]
     python spgemm-hpc-thesis_amd/py/pmc_summary.py trace <kernel_stats.csv> [--top N]
+    python spgemm-hpc-thesis_amd/py/pmc_summary.py timeline <kernel_trace.csv> --segment MARKER:K [--min-us U]
 
 counters: one row per (kernel, counter) with the value per dispatch of the
   chosen dispatches (`last`: the last dispatch of each kernel, which in a
@@ -20,6 +21,11 @@ counters: one row per (kernel, counter) with the value per dispatch of the
   SQ_WAIT_ANY / SQ_WAVE_CYCLES.
 trace: the --kernel-trace --stats summary (calls, average and total
   milliseconds, share) as a markdown table.
+timeline: the dispatches of one segment of the --kernel-trace record (cut as
+  for counters, by the dispatches of MARKER), in start order, with start and
+  end in milliseconds from the segment's first start, so that kernels that
+  overlap on two streams and the critical path are visible; dispatches
+  shorter than --min-us microseconds are left out.
 """
 from __future__ import annotations
 
@@ -104,6 +110,27 @@ def trace(path, top=20):
     return "\n".join(lines)
 
 
+def timeline(path, segment, min_us=0.0):
+    with open(path, newline="") as f:
+        recs = list(csv.DictReader(f))
+    recs.sort(key=lambda r: int(r["Start_Timestamp"]))
+    marker, k = segment.rsplit(":", 1)
+    starts = [i for i, r in enumerate(recs) if marker in _short(r["Kernel_Name"])]
+    lo = starts[int(k)]
+    nxt = [i for i in starts if i > lo]
+    seg = recs[lo:nxt[0]] if nxt else recs[lo:]
+    t0 = int(seg[0]["Start_Timestamp"])
+    t_end = max(int(r["End_Timestamp"]) for r in seg)
+    lines = [f"segment {segment}: {len(seg)} dispatches, first start to last end {(t_end - t0) / 1e6:.3f} ms", "",
+             "| kernel | start ms | end ms | ms |", "|---|---|---|---|"]
+    for r in seg:
+        a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        if (b - a) / 1e3 < min_us:
+            continue
+        lines.append(f"| `{_short(r['Kernel_Name'])}` | {(a - t0) / 1e6:.3f} | {(b - t0) / 1e6:.3f} | {(b - a) / 1e6:.3f} |")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -116,8 +143,14 @@ def main():
     t = sub.add_parser("trace")
     t.add_argument("csv")
     t.add_argument("--top", type=int, default=20)
+    tl = sub.add_parser("timeline")
+    tl.add_argument("csv")
+    tl.add_argument("--segment", required=True)
+    tl.add_argument("--min-us", type=float, default=0.0)
     a = ap.parse_args()
-    if a.mode == "counters":
+    if a.mode == "timeline":
+        print(timeline(a.csv, a.segment, a.min_us))
+    elif a.mode == "counters":
         fac = {}
         for item in a.factor:
             name, val = item.split("=")
