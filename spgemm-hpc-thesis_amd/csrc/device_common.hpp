@@ -265,8 +265,9 @@ __device__ __forceinline__ void upper_entries(const T *off, int n, const T (&s)[
 // wave's 64 ints; among entries starting at one product, empty ones first,
 // the largest index wins): a prefix maximum over the lanes gives each lane
 // its entry.  One LDS read per lane and group instead of a search per item.
-template <int ITEMS, typename T>
+template <int ITEMS, int MARKS = 1, typename T>
 __device__ __forceinline__ void group_entries(const T *off, int n, T g0, T stride, int *mark, int (&e)[ITEMS]) {
+    static_assert(ITEMS % MARKS == 0, "groups are marked MARKS at a time");
     const int lane = lane_id();
     int first;
     {
@@ -276,21 +277,30 @@ __device__ __forceinline__ void group_entries(const T *off, int n, T g0, T strid
         upper_entries<1>(off, n, qa, ea);
         first = ea[0];
     }
+    // MARKS groups at a time, each in its own 64 marks (mark holds MARKS * 64
+    // ints), so that their LDS round trips overlap.
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const int e0 = __builtin_amdgcn_readlane(first, j);
-        const T g = g0 + (T)j * stride;
-        mark[lane] = -1;
+    for (int j0 = 0; j0 < ITEMS; j0 += MARKS) {
+#pragma unroll
+        for (int m = 0; m < MARKS; ++m) mark[m * 64 + lane] = -1;
         wave_sync();
-        const T lim = g + 64;
-        for (int c0 = e0 + 1; c0 < n; c0 += 64) {
-            const int c = c0 + lane;
-            const T o = c < n ? off[c] : lim;
-            if (o < lim) atomicMax(&mark[(int)(o - g)], c);
-            if (!((__ballot(o < lim) >> 63) & 1ull)) break;  // lane 63's entry starts past the group: no later one starts inside
+#pragma unroll
+        for (int m = 0; m < MARKS; ++m) {
+            const int j = j0 + m;
+            const int e0 = __builtin_amdgcn_readlane(first, j);
+            const T g = g0 + (T)j * stride;
+            const T lim = g + 64;
+            for (int c0 = e0 + 1; c0 < n; c0 += 64) {
+                const int c = c0 + lane;
+                const T o = c < n ? off[c] : lim;
+                if (o < lim) atomicMax(&mark[m * 64 + (int)(o - g)], c);
+                if (!((__ballot(o < lim) >> 63) & 1ull)) break;  // lane 63's entry starts past the group: no later one starts inside
+            }
         }
         wave_sync();
-        e[j] = max(e0, wave_incl_max_dpp(mark[lane]));
+#pragma unroll
+        for (int m = 0; m < MARKS; ++m)
+            e[j0 + m] = max(__builtin_amdgcn_readlane(first, j0 + m), wave_incl_max_dpp(mark[m * 64 + lane]));
         wave_sync();  // every lane has read the marks before they are reset
     }
 }

@@ -57,6 +57,11 @@ constexpr int64_t kLdsBitsColsWide = int64_t(1) << 20;
 constexpr int kTpbLargeWide = 1024;
 // The value kernel fits such a bitmap beside 512-product chunk arrays.
 constexpr int kCapLargeWideNum = 512;
+// Large rows of at least kCountHeavy products among the first
+// kCountHeavyRows of the heaviest-first list are counted by one 1024-thread
+// workgroup each before the others (narrow bitmaps, number of large rows known).
+constexpr int64_t kCountHeavy = int64_t(1) << 18;
+constexpr int kCountHeavyRows = 256;
 constexpr int kTpbLargeWideNum = 512;
 constexpr int kCapLargeHbm = 2048;
 constexpr int kTpbLargeHbm = 512;
@@ -277,8 +282,10 @@ constexpr unsigned kAllBins = kSmallBins | kBigBins;
 // kPartCols columns (pw, pd), the products per window before each batch of
 // kCapLarge A entries (pseg), and sums the products and the largest product
 // count of the rows with snnz > part->skip_le into the control block.
+// nlarge: the number of large rows when the host knows it (-1 otherwise); it
+// lets the heaviest be counted by a launch of their own.
 void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream,
-                     uint32_t *lsave = nullptr, const PartState *part = nullptr);
+                     uint32_t *lsave = nullptr, const PartState *part = nullptr, int nlarge = -1);
 // Batches of kCapLarge A entries of every large row and their exclusive scan
 // (ps.segbase; total into ctrl->part_units), before the large-row count
 // (large_part.hip).

@@ -305,19 +305,21 @@ __device__ __forceinline__ void load_chunk(const Operands &op, const int64_t *s_
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) g[j] = c0 + threadIdx.x + j * TPB;
     group_entries<ITEMS>(s_off, nbt, uni64(c0 + (int64_t)(threadIdx.x - lane_id())), (int64_t)TPB, mark, e);
+    // Unconditional loads (products past the batch read B entry 0; the
+    // caller only reads columns of products inside it, see chunk_col), so
+    // that the compiler waits for exactly one chunk's loads.
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-        col[j] = -1;
-        bv[j] = 0.0;
-        ent[j] = 0;
-        if (g[j] < Pb) {
-            const int64_t idx = s_bst[e[j]] + (g[j] - s_off[e[j]]);
-            col[j] = op.bcol[idx];
-            bv[j] = op.bval[idx];
-            ent[j] = e[j];
-        }
+        const bool in = g[j] < Pb;
+        const int64_t idx = in ? s_bst[e[j]] + (g[j] - s_off[e[j]]) : 0;
+        col[j] = op.bcol[idx];
+        bv[j] = op.bval[idx];
+        ent[j] = in ? e[j] : -1;
     }
 }
+
+// Column of a loaded product, -1 past the batch (ent < 0).
+__device__ __forceinline__ int chunk_col(int col, int ent) { return ent < 0 ? -1 : col; }
 
 // LDSB: the window's presence bitmap lives in dynamic LDS (lwidth / 8
 // bytes), so a head knows whether its column is present before touching HBM
@@ -417,7 +419,7 @@ __global__ __launch_bounds__(TPB) void k_large_numeric(Operands op, Workspace ws
                     int cmn = INT_MAX, cmx = INT_MIN, nvalid = 0;
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) {
-                        const int c = pcol[j];
+                        const int c = chunk_col(pcol[j], pent[j]);
                         ok[j] = c >= w0 && c < wend;  // -1 past the batch
                         icol[j] = ok[j] ? c : 0;
                         ival[j] = ok[j] ? s_av[pent[j]] * pbv[j] : 0.0;
@@ -705,7 +707,7 @@ __global__ __launch_bounds__(TPB) void k_large_compact(Operands op, Workspace ws
                     int rmn = INT_MAX, rmx = INT_MIN, nvalid = 0;
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) {
-                        const int c = pcol[j];  // -1 past the batch
+                        const int c = chunk_col(pcol[j], pent[j]);  // -1 past the batch
                         bool in = c >= 0 && c < op.bcols;
                         int r = 0;
                         if (in) {
@@ -856,10 +858,18 @@ bool compact_large_available(int64_t lwidth) {
     return acc >= kCapLargeCompact;
 }
 
+// The compact kernel takes rows of at most this many distinct columns: its
+// LDS accumulator is then small enough for the partition and fold kernels to
+// run beside it on the same CUs (R-MAT x B(65536 columns): 27.7 ms with the
+// whole accumulator, 28.7 with no compact rows, 25.5 at 4096, 26.9 at 8192).
+#ifndef SPG_COMPACT_TAKE_MAX
+#define SPG_COMPACT_TAKE_MAX 4096
+#endif
 int64_t compact_take_le(int64_t lwidth) {
     if (!compact_large_available(lwidth)) return -1;
     const int dyn = g_compact_dyn.load(std::memory_order_relaxed);
-    return (int64_t)((int)(((int64_t)dyn - lwidth / 4) / 8) & ~63);
+    const int64_t cap = (int64_t)((int)(((int64_t)dyn - lwidth / 4) / 8) & ~63);
+    return cap < (int64_t)SPG_COMPACT_TAKE_MAX ? cap : (int64_t)SPG_COMPACT_TAKE_MAX;
 }
 
 bool prepare_numeric_kernels(size_t lds_per_block) {

@@ -191,22 +191,26 @@ __global__ __launch_bounds__(TPB, (hash_min_waves<TPB, CAP>())) void k_symbolic_
     if (flags) atomicOr(&ws.ctrl->flags, (unsigned long long)flags);
 }
 
+// A batch of nbt A entries: each entry's B row start and exclusive product
+// offset (s_off[nbt] = the batch's products, returned), and the longest B
+// row of the batch (maxlen).
 template <int TPB, int CAP>
 __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Workspace &ws, int64_t e0, int nbt,
-                                                    int64_t *s_bst, double *s_av, int64_t *s_off, int64_t *s_ws64,
-                                                    bool want_values) {
+                                                    int64_t *s_bst, int64_t *s_off, int64_t *s_ws64, int *s_wsi,
+                                                    int &maxlen) {
     const int tid = threadIdx.x;
     const int ipt = (nbt + TPB - 1) / TPB;
     const int f = tid * ipt;
     int64_t mylen = 0;
+    int mx = 0;
     for (int j = 0; j < ipt; ++j) {
         const int e = f + j;
         if (e < nbt) {
-            const int64_t len = ws.elen[e0 + e];
+            const int len = ws.elen[e0 + e];
             s_bst[e] = entry_bstart_any(ws, e0 + e);
-            if (want_values) s_av[e] = op.aval[e0 + e];
             s_off[e] = len;
             mylen += len;
+            mx = max(mx, len);
         }
     }
     int64_t Pb;
@@ -219,7 +223,8 @@ __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Wo
             base += l;
         }
     }
-    __syncthreads();
+    if (tid == 0) s_off[nbt] = Pb;
+    maxlen = block_max<TPB>(mx, s_wsi);  // its barriers also publish s_off
     return Pb;
 }
 
@@ -231,15 +236,26 @@ __device__ __forceinline__ int64_t large_batch_meta(const Operands &op, const Wo
 // rows) each row's products and distinct columns per window of kPartCols
 // columns go to pw[i * nwin + k] and pd[i * nwin + k], and the rows with
 // snnz > part_skip_le add their product counts into the control block.
+//
+// Rows of at least heavy_ge products among the first heavy_n of the list
+// (which is sorted heaviest first) can be counted by a separate launch of
+// heavy_n workgroups, one per row (heavy_only: workgroup b takes list row b
+// if it is that heavy), so that each of them has the waves of a 1024-thread
+// workgroup; the launch over the other rows (heavy_only = 0) skips them.
 template <int TPB, int CAP, bool LDSB>
-__global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace ws, uint32_t *lsave, PartState ps) {
-    constexpr int UNROLL = LDSB ? 2 * kLargeUnroll : kLargeUnroll;
+__global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace ws, uint32_t *lsave, PartState ps,
+                                                        int64_t heavy_ge, int heavy_n, int heavy_only) {
+    constexpr int UNROLL = LDSB ? 2 * kLargeUnroll : kLargeUnroll;  // 4x measured slower (registers)
     extern __shared__ uint32_t s_lbits[];
     __shared__ int64_t s_bst[CAP];
     __shared__ int64_t s_off[CAP + 1];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
     __shared__ uint32_t s_pw[kPartMaxWin];  // products per partition window (32-bit: rows past 2^31 are not partitioned)
-    __shared__ int s_gmark[TPB / 64][64];   // each wave's entry marks (group_entries)
+    // each wave's entry marks (group_entries), all groups of a step at once
+    // where the LDS allows it
+    constexpr int MARKS = TPB >= 1024 ? 2 : UNROLL;
+    __shared__ int s_gmark[TPB / 64][MARKS * 64];
+    __shared__ int s_wsi[TPB / 64];
     const int tid = threadIdx.x, lane = tid & 63;
     const RowRec *list = large_list(ws);
     const int count = ws.ctrl->bin_count[kBinLarge];
@@ -259,7 +275,17 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
         for (int k = tid; k < kPartMaxWin; k += TPB) s_pw[k] = 0u;
     }
     __shared__ int s_next;
-    for (int i; (i = next_large_row(&ws.ctrl->large_next[0], &s_next)) < count;) {
+    for (int i0 = heavy_only ? (int)blockIdx.x : 0;;) {
+        int i;
+        if (heavy_only) {
+            if (i0 < 0 || i0 >= count || i0 >= heavy_n || ws.ub[list[i0].row] < heavy_ge) break;
+            i = i0;
+            i0 = -1;  // one row per workgroup
+        } else {
+            i = next_large_row(&ws.ctrl->large_next[0], &s_next);
+            if (i >= count) break;
+            if (i < heavy_n && ws.ub[list[i].row] >= heavy_ge) continue;  // the heavy launch's
+        }
         const int row = list[i].row;
         const int64_t a0 = op.aptr[row], a1 = op.aptr[row + 1];
         int64_t cnt = 0;
@@ -271,7 +297,18 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                     const int64_t u = ps.segbase[count + 1 + i] + (e0 - a0) / CAP;
                     for (int k = tid; k < nwin; k += TPB) ps.pseg[(size_t)u * (size_t)nwin + k] = s_pw[k];
                 }
-                const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, nullptr, s_off, s_ws64, false);
+                int maxlen;
+                const int64_t Pb = large_batch_meta<TPB, CAP>(op, ws, e0, nbt, s_bst, s_off, s_ws64, s_wsi, maxlen);
+                // Batches whose B rows all have at most 64 entries are taken
+                // entry-major (EM): Lp lanes (a power of two >= the longest
+                // row) per A entry, G = 64 / Lp entries per wave and item, no
+                // entry search.  Item u of step k of wave w holds entries
+                // ((k * UNROLL + u) * NW + w) * G + lane / Lp.
+                const bool em = LDSB && maxlen <= 64;
+                int logLp = 0;
+                while ((1 << logLp) < maxlen) ++logLp;
+                const int G = 64 >> logLp;
+                constexpr int64_t kStep = (int64_t)TPB * UNROLL;
                 // UNROLL products per thread per step: every column load,
                 // then every bit claim, in flight together.  With the bitmap
                 // in LDS the next step's column loads are issued before this
@@ -279,42 +316,75 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                 // Lane l of wave w holds products s0 + u * TPB (u < UNROLL),
                 // s0 = 64 w + l + k * TPB * UNROLL: groups of 64 consecutive
                 // products, whose entries group_entries finds together.
-                auto load_step = [&](int64_t s0, int (&c)[UNROLL]) {
-                    int64_t sx[UNROLL];
-                    int e[UNROLL];
+                // The loads are unconditional (products past the batch read
+                // B entry 0, the batch has products): with no branch around
+                // them the compiler waits for exactly the previous step's
+                // loads, not for the next step's too.
+                const int wv = tid >> 6;
+                auto load_step = [&](int64_t k, int (&c)[UNROLL], bool (&v)[UNROLL]) {
+                    if (em) {
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) sx[u] = s0 + (int64_t)u * TPB;
-                    group_entries<UNROLL>(s_off, nbt, uni64(s0 - lane), (int64_t)TPB, s_gmark[tid >> 6], e);
+                        for (int u = 0; u < UNROLL; ++u) {
+                            const int64_t e = ((k * UNROLL + u) * (TPB / 64) + wv) * G + (lane >> logLp);
+                            const int sub = lane & ((1 << logLp) - 1);
+                            const int ec = (int)min(e, (int64_t)nbt - 1);
+                            v[u] = e < nbt && sub < (int)(s_off[ec + 1] - s_off[ec]);
+                            c[u] = op.bcol[v[u] ? s_bst[ec] + sub : 0];
+                        }
+                    } else {
+                        const int64_t s0 = k * kStep + tid;
+                        int64_t sx[UNROLL];
+                        int e[UNROLL];
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) {
-                        c[u] = -1;
-                        if (sx[u] < Pb) c[u] = op.bcol[s_bst[e[u]] + (sx[u] - s_off[e[u]])];
+                        for (int u = 0; u < UNROLL; ++u) sx[u] = s0 + (int64_t)u * TPB;
+                        group_entries<UNROLL, MARKS>(s_off, nbt, uni64(s0 - lane), (int64_t)TPB, s_gmark[wv], e);
+#pragma unroll
+                        for (int u = 0; u < UNROLL; ++u) {
+                            v[u] = sx[u] < Pb;
+                            c[u] = op.bcol[v[u] ? s_bst[e[u]] + (sx[u] - s_off[e[u]]) : 0];
+                        }
                     }
                 };
-                constexpr int64_t kStep = (int64_t)TPB * UNROLL;
+                // steps of the wave: while its first product (EM: first entry)
+                // is inside the batch (uniform per wave)
+                const int64_t nsteps =
+                    em ? ((int64_t)nbt + (int64_t)G * (TPB / 64) * UNROLL - 1) / ((int64_t)G * (TPB / 64) * UNROLL)
+                       : (Pb + kStep - 1) / kStep;
                 int cn[UNROLL];
-                if (LDSB) load_step(tid, cn);
-                for (int64_t s0 = tid; s0 - lane < Pb; s0 += kStep) {  // uniform per wave
+                bool vn[UNROLL];
+                if (LDSB) load_step(0, cn, vn);
+                for (int64_t k = 0; k < nsteps; ++k) {
                     int c[UNROLL];
+                    bool cv[UNROLL];
                     if (LDSB) {
 #pragma unroll
-                        for (int u = 0; u < UNROLL; ++u) c[u] = cn[u];
-                        load_step(s0 + kStep, cn);
+                        for (int u = 0; u < UNROLL; ++u) {
+                            c[u] = cn[u];
+                            cv[u] = vn[u];
+                        }
+                        if (k + 1 < nsteps) load_step(k + 1, cn, vn);
                     } else {
-                        load_step(s0, c);
+                        load_step(k, c, cv);
                     }
                     uint32_t old[UNROLL], bit[UNROLL];
                     bool in[UNROLL];
 #pragma unroll
                     for (int u = 0; u < UNROLL; ++u) {
-                        in[u] = c[u] >= w0 && c[u] < wend;
+                        in[u] = cv[u] && c[u] >= w0 && c[u] < wend;
                         const uint32_t rel = in[u] ? (uint32_t)(c[u] - w0) : 0u;
                         bit[u] = 1u << (rel & 31u);
-                        uint32_t *wp = LDSB ? &s_lbits[rel >> 5] : &bits[rel >> 5];
-                        old[u] = in[u] ? atomicOr(wp, bit[u]) : bit[u];
+                        if (LDSB) {
+                            // no return value to wait for: the row's count is
+                            // the bitmap's popcount when it is cleared
+                            if (in[u]) atomicOr(&s_lbits[rel >> 5], bit[u]);
+                        } else {
+                            old[u] = in[u] ? atomicOr(&bits[rel >> 5], bit[u]) : bit[u];
+                        }
                     }
+                    if (!LDSB) {
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) cnt += in[u] && !(old[u] & bit[u]);
+                        for (int u = 0; u < UNROLL; ++u) cnt += in[u] && !(old[u] & bit[u]);
+                    }
                     if (part) {
                         // consecutive lanes hold consecutive products (a B
                         // row's columns ascend): one add per run of lanes in
@@ -355,7 +425,9 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
             uint32_t *save = LDSB && lsave ? lsave + (size_t)i * (size_t)(W / 32) : nullptr;
             for (int64_t q = tid; q < nwords; q += TPB) {
                 if (LDSB) {
-                    if (save) save[q] = s_lbits[q];
+                    const uint32_t wd = s_lbits[q];
+                    cnt += __popc(wd);
+                    if (save) save[q] = wd;
                     s_lbits[q] = 0u;
                 } else {
                     st_bits(&bits[q], 0u);
@@ -399,8 +471,12 @@ hipError_t prepare_kernels(size_t lds_per_block) {
     // dynamic LDS beside its static arrays; without that much LDS per
     // workgroup it is not used.
     const size_t dyn = (size_t)(kLdsBitsColsWide / 8);
-    const size_t stat = sizeof(int64_t) * (2 * kCapLarge + 1) + sizeof(int64_t) * (kTpbLargeWide / 64 + 1) +
-                        sizeof(uint32_t) * kPartMaxWin + 64;
+    hipFuncAttributes attr{};
+    const bool have =
+        hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&k_large_symbolic<kTpbLargeWide, kCapLarge, true>)) ==
+        hipSuccess;
+    if (!have) (void)hipGetLastError();
+    const size_t stat = have ? (size_t)attr.sharedSizeBytes : lds_per_block;  // unknown: not used
     bool ok = lds_per_block >= dyn + stat;
     if (ok) {
         const hipError_t e = hipFuncSetAttribute(
@@ -422,7 +498,7 @@ bool part_available(int64_t bcols, int64_t lwidth) {
 }
 
 void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream,
-                     uint32_t *lsave, const PartState *part) {
+                     uint32_t *lsave, const PartState *part, int nlarge) {
     if (op.arows <= 0) return;
     if (bins & (1u << kBinW1)) launch_hash<64, kCapW1>(op, ws, kBinW1, cus, stream);
     if (bins & (1u << kBinW2)) launch_hash<64, kCapW2>(op, ws, kBinW2, cus, stream);
@@ -438,17 +514,29 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
             // The bitmap is in LDS, so the grid is not bound to the per-block
             // HBM windows: every workgroup that fits takes rows.
             const size_t dyn = (size_t)(ws.lwidth / 8);
+            uint32_t *save = op.bcols <= ws.lwidth ? lsave : nullptr;
+            // With the number of large rows known, the heaviest (a row of
+            // 1.5 M products otherwise holds one 256-thread workgroup for most
+            // of the kernel) are counted first by 1024-thread workgroups.
+            int heavy_n = 0;
+            if (nlarge > 0 && g_wide_lds.load(std::memory_order_relaxed)) {
+                heavy_n = nlarge < kCountHeavyRows ? nlarge : kCountHeavyRows;
+                hipLaunchKernelGGL((k_large_symbolic<kTpbLargeWide, kCapLarge, true>), dim3(heavy_n),
+                                   dim3(kTpbLargeWide), (unsigned)dyn, stream, op, ws, save, ps, kCountHeavy, heavy_n,
+                                   1);
+            }
             const int g = occupancy_grid(k_large_symbolic<kTpbLarge, kCapLarge, true>, kTpbLarge, cus, 0, dyn);
             hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, true>), dim3(g), dim3(kTpbLarge),
-                               (unsigned)dyn, stream, op, ws, op.bcols <= ws.lwidth ? lsave : nullptr, ps);
+                               (unsigned)dyn, stream, op, ws, save, ps, kCountHeavy, heavy_n, 0);
         }
         else if (ws.lwidth <= kLdsBitsColsWide && g_wide_lds.load(std::memory_order_relaxed))
             hipLaunchKernelGGL((k_large_symbolic<kTpbLargeWide, kCapLarge, true>), dim3(ws.lblocks),
                                dim3(kTpbLargeWide), (unsigned)(ws.lwidth / 8), stream, op, ws,
-                               (uint32_t *)nullptr, ps);
+                               (uint32_t *)nullptr, ps, (int64_t)0, 0, 0);
         else
             hipLaunchKernelGGL((k_large_symbolic<kTpbLargeHbm, kCapLargeHbm, false>), dim3(ws.lblocks),
-                               dim3(kTpbLargeHbm), 0, stream, op, ws, (uint32_t *)nullptr, PartState{});
+                               dim3(kTpbLargeHbm), 0, stream, op, ws, (uint32_t *)nullptr, PartState{}, (int64_t)0, 0,
+                               0);
     }
 }
 
