@@ -133,3 +133,43 @@ def test_partition_keep_zeros_accumulator(engine, fmt):
         engine.set_format(S.FORMAT_AUTO)
     assert out[S.LARGE_AUTO] == out[S.LARGE_WINDOWS]
     assert out[S.LARGE_AUTO] == O.csr_digest(O.accumulate(6, cols, [(a, b), (a, b)]))
+
+
+def test_heavy_rows_analyze_pieces_and_count_launch(engine):
+    # A first row tile of about 70,000 A entries (cut into analyze pieces),
+    # rows of 480 K and 960 K products (counted by the heavy-row launch: C of
+    # 65,536 columns, a narrow count bitmap), hub columns, rows the compact
+    # kernel takes (at most 4096 distinct columns), an empty row inside the
+    # heavy tile, and B rows of 24 entries (entry-major count batches) as well
+    # as of up to 90 entries (group lookups).
+    rng = np.random.Generator(np.random.PCG64(74))
+    cols, nb = 65_536, 65_536
+    for blen_hi in (24, 90):
+        # (B rows of 0 entries: A entries whose products are none, several
+        # entries starting at one product)
+        blen = np.full(nb, 24) if blen_hi == 24 else rng.integers(0, blen_hi + 1, size=nb)
+        bnnz = int(blen.sum())
+        bcol = rng.integers(0, cols, size=bnnz)
+        bcol[::7] = 5  # a hub column
+        b = _csr(nb, cols, blen, bcol, rng.uniform(-1, 1, size=bnnz))
+        lens = np.full(200, 8)
+        lens[0], lens[1], lens[3] = 20_000, 40_000, 9_000
+        lens[2] = 0
+        lens[100:110] = 160  # ~3,800 products each: compact rows
+        acol = rng.integers(0, nb, size=int(lens.sum()))
+        a = _csr(len(lens), nb, lens, acol, rng.uniform(-1, 1, size=int(lens.sum())))
+        want = O.scalar_products(a, b)
+        assert want > 2 * (1 << 18)  # rows 0 and 1 take the heavy-row count launch
+        seen = {}
+        for mode, st, c in _both_modes(engine, a, b, paths=(S.PATH_AUTO,)):
+            assert st["products"] == want
+            assert st["rows_per_bin"][6] >= 13  # rows 0, 1, 3 and the compact rows are large
+            seen[mode] = st["partitioned"]
+        assert seen[S.LARGE_AUTO]
+        # the last row tile partial (rows 64-69) and heavy: its pieces are
+        # at most one per row
+        lens2 = np.full(70, 4)
+        lens2[64:70] = 10_000
+        a2 = _csr(70, nb, lens2, rng.integers(0, nb, size=int(lens2.sum())), rng.uniform(-1, 1, size=int(lens2.sum())))
+        for mode, st, c in _both_modes(engine, a2, b, paths=(S.PATH_AUTO,)):
+            assert st["products"] == O.scalar_products(a2, b)
