@@ -251,6 +251,7 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
     __shared__ int64_t s_off[CAP + 1];
     __shared__ int64_t s_ws64[TPB / 64 + 1];
     __shared__ uint32_t s_pw[kPartMaxWin];  // products per partition window (32-bit: rows past 2^31 are not partitioned)
+    __shared__ int s_pd[kPartMaxWin];       // distinct columns per partition window (zero between rows)
     // each wave's entry marks (group_entries), all groups of a step at once
     // where the LDS allows it
     constexpr int MARKS = TPB >= 1024 ? 2 : UNROLL;
@@ -272,9 +273,16 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
         for (int64_t q = tid; q < W / 32; q += TPB) s_lbits[q] = 0u;
     }
     if (part) {
-        for (int k = tid; k < kPartMaxWin; k += TPB) s_pw[k] = 0u;
+        for (int k = tid; k < kPartMaxWin; k += TPB) {
+            s_pw[k] = 0u;
+            s_pd[k] = 0;
+        }
     }
     __shared__ int s_next;
+    // The next row's ticket is taken when a row starts (thread 0), so that the
+    // counter's round trip overlaps the row's work.
+    int ticket = 0;
+    if (!heavy_only && tid == 0) ticket = atomicAdd(&ws.ctrl->large_next[0], 1);
     for (int i0 = heavy_only ? (int)blockIdx.x : 0;;) {
         int i;
         if (heavy_only) {
@@ -282,8 +290,12 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
             i = i0;
             i0 = -1;  // one row per workgroup
         } else {
-            i = next_large_row(&ws.ctrl->large_next[0], &s_next);
+            if (tid == 0) s_next = ticket;
+            __syncthreads();
+            i = s_next;
+            __syncthreads();  // every thread has read the slot before it is reused
             if (i >= count) break;
+            if (tid == 0) ticket = atomicAdd(&ws.ctrl->large_next[0], 1);
             if (i < heavy_n && ws.ub[list[i].row] >= heavy_ge) continue;  // the heavy launch's
         }
         const int row = list[i].row;
@@ -408,33 +420,45 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
                 __syncthreads();  // batch metadata is overwritten next
             }
             const int64_t nwords = (wend - w0 + 31) / 32;
-            if (part) {
-                // per window: its products, and its distinct columns (the set
-                // bits of its kPartCols / 32 bitmap words)
-                constexpr int64_t kWords = kPartCols / 32;
-                for (int k = tid; k < nwin; k += TPB) {
-                    int d = 0;
-                    const int64_t q1 = min((int64_t)(k + 1) * kWords, nwords);
-                    for (int64_t q = (int64_t)k * kWords; q < q1; ++q) d += __popc(s_lbits[q]);
-                    ps.pw[(size_t)i * (size_t)nwin + k] = (int64_t)s_pw[k];
-                    ps.pd[(size_t)i * (size_t)nwin + k] = d;
-                    s_pw[k] = 0u;
-                }
-                __syncthreads();  // bitmap read before it is cleared below
-            }
             uint32_t *save = LDSB && lsave ? lsave + (size_t)i * (size_t)(W / 32) : nullptr;
-            for (int64_t q = tid; q < nwords; q += TPB) {
-                if (LDSB) {
-                    const uint32_t wd = s_lbits[q];
-                    cnt += __popc(wd);
-                    if (save) save[q] = wd;
-                    s_lbits[q] = 0u;
-                } else {
-                    st_bits(&bits[q], 0u);
+            if (LDSB) {
+                // The row's count is the bitmap's popcount, taken as it is
+                // cleared; with partitions each wave's 64 words lie in one
+                // window of kPartCols / 32 words, whose distinct columns get
+                // the wave's sum.
+                static_assert(kPartCols % (32 * 64) == 0, "a wave's 64 bitmap words lie in one partition window");
+                for (int64_t q0 = tid - lane; q0 < nwords; q0 += TPB) {
+                    const int64_t q = q0 + lane;
+                    const uint32_t wd = q < nwords ? s_lbits[q] : 0u;
+                    const int pc = __popc(wd);
+                    cnt += pc;
+                    if (q < nwords) {
+                        if (save) save[q] = wd;
+                        s_lbits[q] = 0u;
+                    }
+                    if (part) {
+                        const int d = wave_last(wave_incl_sum_dpp(pc));
+                        if (lane == 0 && d) atomicAdd(&s_pd[q0 / (kPartCols / 32)], d);
+                    }
                 }
+                if (part) {
+                    __syncthreads();  // every window's sum is in
+                    for (int k = tid; k < nwin; k += TPB) {
+                        ps.pw[(size_t)i * (size_t)nwin + k] = (int64_t)s_pw[k];
+                        ps.pd[(size_t)i * (size_t)nwin + k] = s_pd[k];
+                        s_pw[k] = 0u;
+                        s_pd[k] = 0;
+                    }
+                }
+                // The bitmap is LDS: the next window's or row's claims are
+                // ordered by the barrier alone (a device-scope fence would
+                // write back and invalidate the L2 at every row).
+                __syncthreads();
+            } else {
+                for (int64_t q = tid; q < nwords; q += TPB) st_bits(&bits[q], 0u);
+                __threadfence();  // the HBM bitmap's clears reach the L2 before the next claims
+                __syncthreads();
             }
-            __threadfence();
-            __syncthreads();
         }
         const int64_t total = block_sum<TPB>(cnt, s_ws64);
         if (tid == 0) ws.snnz[row] = total;
