@@ -4,6 +4,7 @@
 #include "internal.hpp"
 
 #include <climits>
+#include <type_traits>
 
 namespace spg {
 namespace dev {
@@ -20,34 +21,122 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// DPP forms of the wave scans and reductions (no LDS traffic): row_shr 1, 2,
+// 4, 8 inside each 16-lane row, then row_bcast 15 and 31 carry the row totals
+// upward (row masks 0xa and 0xc).  Lanes whose source is outside the row read
+// the identity.  Every lane of the wave must be active.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i(int identity, int v) {
+    return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWMASK, 0xf, false);
+}
+// The same move of a 64-bit value (both halves from the same lane).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t identity, uint64_t v) {
+    const uint32_t lo = (uint32_t)dpp_i<CTRL, ROWMASK>((int)(uint32_t)identity, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)dpp_i<CTRL, ROWMASK>((int)(uint32_t)(identity >> 32), (int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int wave_incl_sum_dpp(int v) {
+    v += dpp_i<0x111, 0xf>(0, v);
+    v += dpp_i<0x112, 0xf>(0, v);
+    v += dpp_i<0x114, 0xf>(0, v);
+    v += dpp_i<0x118, 0xf>(0, v);
+    v += dpp_i<0x142, 0xa>(0, v);
+    v += dpp_i<0x143, 0xc>(0, v);
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_incl_sum_dpp64(uint64_t v) {
+    v += dpp_u64<0x111, 0xf>(0, v);
+    v += dpp_u64<0x112, 0xf>(0, v);
+    v += dpp_u64<0x114, 0xf>(0, v);
+    v += dpp_u64<0x118, 0xf>(0, v);
+    v += dpp_u64<0x142, 0xa>(0, v);
+    v += dpp_u64<0x143, 0xc>(0, v);
+    return v;
+}
+
+// Lane 63 of an inclusive scan holds the wave's result (a scalar register).
+__device__ __forceinline__ int wave_last(int v) { return __builtin_amdgcn_readlane(v, 63); }
+__device__ __forceinline__ uint64_t wave_last64(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <typename T>
+constexpr bool kDppInt = std::is_integral<T>::value && (sizeof(T) == 4 || sizeof(T) == 8);
+
+// Inclusive scan over the wave's lanes (all active).  Integers take DPP moves;
+// other types (none in the hot paths) take lane shuffles.
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
+    if constexpr (kDppInt<T> && sizeof(T) == 4) {
+        return (T)wave_incl_sum_dpp((int)v);
+    } else if constexpr (kDppInt<T>) {
+        return (T)wave_incl_sum_dpp64((uint64_t)v);
+    } else {
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        T t = __shfl_up(v, d, 64);
-        if (lane_id() >= d) v += t;
+        for (int d = 1; d < 64; d <<= 1) {
+            T t = __shfl_up(v, d, 64);
+            if (lane_id() >= d) v += t;
+        }
+        return v;
     }
-    return v;
+}
+
+// Lane 63's value, in every lane.
+template <typename T>
+__device__ __forceinline__ T wave_bcast_last(T v) {
+    if constexpr (kDppInt<T> && sizeof(T) == 4) return (T)wave_last((int)v);
+    else if constexpr (kDppInt<T>) return (T)wave_last64((uint64_t)v);
+    else return __shfl(v, 63, 64);
 }
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
+    if constexpr (kDppInt<T>) {
+        return wave_bcast_last(wave_incl_scan(v));
+    } else {
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+        return v;
+    }
+}
+
+__device__ __forceinline__ int wave_max_dpp(int v) {
+    v = max(v, dpp_i<0x111, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x112, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x114, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x118, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x142, 0xa>(INT_MIN, v));
+    v = max(v, dpp_i<0x143, 0xc>(INT_MIN, v));
+    return wave_last(v);
+}
+
+// Inclusive prefix maximum over the lanes (lane l gets max of lanes 0..l).
+__device__ __forceinline__ int wave_incl_max_dpp(int v) {
+    v = max(v, dpp_i<0x111, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x112, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x114, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x118, 0xf>(INT_MIN, v));
+    v = max(v, dpp_i<0x142, 0xa>(INT_MIN, v));
+    v = max(v, dpp_i<0x143, 0xc>(INT_MIN, v));
     return v;
 }
 
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
-    return v;
+__device__ __forceinline__ int wave_min_dpp(int v) {
+    v = min(v, dpp_i<0x111, 0xf>(INT_MAX, v));
+    v = min(v, dpp_i<0x112, 0xf>(INT_MAX, v));
+    v = min(v, dpp_i<0x114, 0xf>(INT_MAX, v));
+    v = min(v, dpp_i<0x118, 0xf>(INT_MAX, v));
+    v = min(v, dpp_i<0x142, 0xa>(INT_MAX, v));
+    v = min(v, dpp_i<0x143, 0xc>(INT_MAX, v));
+    return wave_last(v);
 }
 
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
-    return v;
-}
+__device__ __forceinline__ int wave_min(int v) { return wave_min_dpp(v); }
+__device__ __forceinline__ int wave_max(int v) { return wave_max_dpp(v); }
 
 // Exclusive scan across a TPB-thread block; `ws` holds TPB/64 elements.
 // Has barriers when TPB > 64 and must then be reached by every thread.
@@ -56,7 +145,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T *ws, T &total) {
     constexpr int NW = TPB / 64;
     const T inc = wave_incl_scan(v);
     if constexpr (NW == 1) {
-        total = __shfl(inc, 63, 64);
+        total = wave_bcast_last(inc);
         return inc - v;
     } else {
         const int w = threadIdx.x >> 6;
@@ -140,59 +229,6 @@ __device__ __forceinline__ void block_minmax(int &mn, int &mx, int *ws) {
         mn = a;
         mx = b;
     }
-}
-
-// DPP forms of the wave scans and reductions (no LDS traffic): row_shr 1, 2,
-// 4, 8 inside each 16-lane row, then row_bcast 15 and 31 carry the row totals
-// upward (row masks 0xa and 0xc).  Lanes whose source is outside the row read
-// the identity.
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ int dpp_i(int identity, int v) {
-    return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWMASK, 0xf, false);
-}
-
-__device__ __forceinline__ int wave_incl_sum_dpp(int v) {
-    v += dpp_i<0x111, 0xf>(0, v);
-    v += dpp_i<0x112, 0xf>(0, v);
-    v += dpp_i<0x114, 0xf>(0, v);
-    v += dpp_i<0x118, 0xf>(0, v);
-    v += dpp_i<0x142, 0xa>(0, v);
-    v += dpp_i<0x143, 0xc>(0, v);
-    return v;
-}
-
-// Lane 63 of an inclusive scan holds the wave's result (a scalar register).
-__device__ __forceinline__ int wave_last(int v) { return __builtin_amdgcn_readlane(v, 63); }
-
-__device__ __forceinline__ int wave_max_dpp(int v) {
-    v = max(v, dpp_i<0x111, 0xf>(INT_MIN, v));
-    v = max(v, dpp_i<0x112, 0xf>(INT_MIN, v));
-    v = max(v, dpp_i<0x114, 0xf>(INT_MIN, v));
-    v = max(v, dpp_i<0x118, 0xf>(INT_MIN, v));
-    v = max(v, dpp_i<0x142, 0xa>(INT_MIN, v));
-    v = max(v, dpp_i<0x143, 0xc>(INT_MIN, v));
-    return wave_last(v);
-}
-
-// Inclusive prefix maximum over the lanes (lane l gets max of lanes 0..l).
-__device__ __forceinline__ int wave_incl_max_dpp(int v) {
-    v = max(v, dpp_i<0x111, 0xf>(INT_MIN, v));
-    v = max(v, dpp_i<0x112, 0xf>(INT_MIN, v));
-    v = max(v, dpp_i<0x114, 0xf>(INT_MIN, v));
-    v = max(v, dpp_i<0x118, 0xf>(INT_MIN, v));
-    v = max(v, dpp_i<0x142, 0xa>(INT_MIN, v));
-    v = max(v, dpp_i<0x143, 0xc>(INT_MIN, v));
-    return v;
-}
-
-__device__ __forceinline__ int wave_min_dpp(int v) {
-    v = min(v, dpp_i<0x111, 0xf>(INT_MAX, v));
-    v = min(v, dpp_i<0x112, 0xf>(INT_MAX, v));
-    v = min(v, dpp_i<0x114, 0xf>(INT_MAX, v));
-    v = min(v, dpp_i<0x118, 0xf>(INT_MAX, v));
-    v = min(v, dpp_i<0x142, 0xa>(INT_MAX, v));
-    v = min(v, dpp_i<0x143, 0xc>(INT_MAX, v));
-    return wave_last(v);
 }
 
 // Block-uniform value held in a scalar register (the value must already be
@@ -397,10 +433,10 @@ __device__ __forceinline__ int rec_word(const RowRec *list, int i) {
 }
 __device__ __forceinline__ RowRec rec_decode(int w) {
     RowRec r;
-    r.row = uni(__shfl(w, 0, 64));
-    r.na = uni(__shfl(w, 1, 64));
-    const uint32_t lo = (uint32_t)uni(__shfl(w, 2, 64));
-    const uint32_t hi = (uint32_t)uni(__shfl(w, 3, 64));
+    r.row = __builtin_amdgcn_readlane(w, 0);
+    r.na = __builtin_amdgcn_readlane(w, 1);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(w, 2);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(w, 3);
     r.a0 = (int64_t)(((uint64_t)hi << 32) | lo);
     return r;
 }

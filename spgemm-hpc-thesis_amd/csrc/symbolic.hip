@@ -406,7 +406,7 @@ __global__ __launch_bounds__(TPB) void k_large_symbolic(Operands op, Workspace w
 #pragma unroll
                         for (int u = 0; u < UNROLL; ++u) {
                             const uint32_t wk = in[u] ? (uint32_t)(c[u] - w0) >> kPartLog : 0xffffffffu;
-                            const uint32_t prev = (uint32_t)__shfl_up((int)wk, 1, 64);
+                            const uint32_t prev = (uint32_t)dpp_i<0x138, 0xf>(0, (int)wk);  // lane - 1 (wave_shr 1)
                             const bool start = in[u] && (lane == 0 || prev != wk);
                             const uint64_t ends = __ballot(start) | __ballot(!in[u]) | ~active;
                             if (start) {
