@@ -179,15 +179,17 @@ __device__ __forceinline__ unsigned tile_entries(const Operands &op, const Works
             const int row = max(seed, wave_incl_max_dpp(s_mark[lane]));
             // The lengths of a row's lanes (consecutive) are summed in the
             // wave first: one LDS atomic per row of the chunk, not one per
-            // entry (64 on one address inside a long row).
+            // entry (64 on one address inside a long row).  The inclusive
+            // sum never decreases, so a run's base (the sum before its first
+            // lane) is the prefix maximum of the bases marked at run starts;
+            // every move is a DPP lane shift.
             const int64_t v = inv[h] ? lv[h] : 0;
             if (__ballot(v >= ((int64_t)1 << 24)) == 0) {
                 const int incl = wave_incl_sum_dpp((int)v);
-                const int nrow = __shfl_down(row, 1, 64);
-                const int64_t ar = __shfl(a0, row, 64);
-                const int st = (int)max((int64_t)0, ar - cb);  // the row's first lane in the chunk
-                const int before = __shfl(incl, max(st, 1) - 1, 64);
-                const int64_t sum = (int64_t)incl - (st > 0 ? before : 0);
+                const int prow = dpp_i<0x138, 0xf>(INT_MIN, row);  // lane - 1 (wave_shr 1); lane 0: none
+                const int nrow = dpp_i<0x130, 0xf>(INT_MIN, row);  // lane + 1 (wave_shl 1); lane 63: none
+                const int base = wave_incl_max_dpp(prow != row ? incl - (int)v : INT_MIN);
+                const int sum = incl - base;
                 if ((lane == 63 || nrow != row) && sum) atomicAdd(&s_ub[row], (unsigned long long)sum);
             } else if (inv[h]) {
                 atomicAdd(&s_ub[row], (unsigned long long)lv[h]);
@@ -228,7 +230,7 @@ __device__ __forceinline__ void tile_finish(const Operands &op, const Workspace 
         if (!listed) continue;
         int basei = 0;
         if (lane == 0) basei = atomicAdd(&ws.ctrl->bin_count[bb], (int)__popcll(m));
-        basei = __shfl(basei, 0, 64);
+        basei = __builtin_amdgcn_readlane(basei, 0);
         if (b == bb) {
             RowRec rec;
             rec.row = (int32_t)r;
@@ -255,8 +257,8 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
     const int last = has ? (int)min((int64_t)63, (int64_t)op.arows - 1 - tile * 64) : 0;
     const int64_t a0 = valid ? op.aptr[r] : 0;
     const int64_t a1 = valid ? op.aptr[r + 1] : 0;
-    const int64_t E0 = __shfl(a0, 0, 64);
-    const int64_t E1 = __shfl(a1, last, 64);
+    const int64_t E0 = lane_value64(a0, 0);
+    const int64_t E1 = lane_value64(a1, last);
     s_ub[w][lane] = 0ull;
     unsigned flags = 0;
     if (has && op.nnz_a <= 0 && E1 > E0) flags |= kErrInternal;  // row_ptr inconsistent with nnz(A)
@@ -271,7 +273,7 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
         const int np = (int)heavy_pieces(E1 - E0, last + 1);
         int base = 0;
         if (lane == 0) base = atomicAdd(&ws.ctrl->heavy_units, np);
-        base = __shfl(base, 0, 64);
+        base = __builtin_amdgcn_readlane(base, 0);
         RowRec *hl = ws.bin_recs + (size_t)kBinEmpty * (size_t)ws.list_stride;
         for (int k = lane; k < np; k += 64) {
             RowRec rec;
@@ -287,10 +289,8 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
         wave_sync();
         tile_finish(op, ws, tile, a0, a1, s_ub[w], s_part[w]);
     }
-    unsigned long long fl = flags;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) fl |= __shfl_xor(fl, d, 64);
-    if (lane == 0 && fl) atomicOr(&ws.ctrl->flags, fl);
+    const unsigned fl = wave_or(flags);
+    if (lane == 0 && fl) atomicOr(&ws.ctrl->flags, (unsigned long long)fl);
     __syncthreads();
     if (threadIdx.x < kAnalyzeFields) {
         int64_t v = 0;
@@ -360,10 +360,8 @@ __global__ __launch_bounds__(kAnHeavyTpb) void k_analyze_heavy(Operands op, Work
         }
         __syncthreads();  // piece state is reset for the next piece
     }
-    unsigned long long fl = flags;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) fl |= __shfl_xor(fl, d, 64);
-    if (lane == 0 && fl) atomicOr(&ws.ctrl->flags, fl);
+    const unsigned fl = wave_or(flags);
+    if (lane == 0 && fl) atomicOr(&ws.ctrl->flags, (unsigned long long)fl);
 }
 
 // Large-row work classes: two per power of two of P_row (its leading bit

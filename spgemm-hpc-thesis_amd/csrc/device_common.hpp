@@ -136,6 +136,24 @@ __device__ __forceinline__ int wave_min_dpp(int v) {
 }
 
 __device__ __forceinline__ int wave_min(int v) { return wave_min_dpp(v); }
+
+// OR over the wave's lanes (all active), in every lane.
+__device__ __forceinline__ unsigned wave_or(unsigned v) {
+    v |= (unsigned)dpp_i<0x111, 0xf>(0, (int)v);
+    v |= (unsigned)dpp_i<0x112, 0xf>(0, (int)v);
+    v |= (unsigned)dpp_i<0x114, 0xf>(0, (int)v);
+    v |= (unsigned)dpp_i<0x118, 0xf>(0, (int)v);
+    v |= (unsigned)dpp_i<0x142, 0xa>(0, (int)v);
+    v |= (unsigned)dpp_i<0x143, 0xc>(0, (int)v);
+    return (unsigned)wave_last((int)v);
+}
+
+// Lane k's 64-bit value (k uniform), in every lane.
+__device__ __forceinline__ int64_t lane_value64(int64_t v, int k) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), k);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 __device__ __forceinline__ int wave_max(int v) { return wave_max_dpp(v); }
 
 // Exclusive scan across a TPB-thread block; `ws` holds TPB/64 elements.
