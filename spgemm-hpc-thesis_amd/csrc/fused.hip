@@ -385,11 +385,19 @@ __device__ __forceinline__ int sort_fold(FusedWave<SMALLB> &w, int P, const RowI
     bool anycont = false;
 #pragma unroll
     for (int j = 0; j < kFItems; ++j) {
+        cc[j] = (int)w.key[lane + 64 * j];
+        vv[j] = w.val[lane + 64 * j];
+    }
+    // Neighbouring positions' columns by DPP lane shifts; position 64j - 1
+    // (lane 0) is lane 63 of slot j - 1, position 64j + 64 (lane 63) lane 0
+    // of slot j + 1.
+#pragma unroll
+    for (int j = 0; j < kFItems; ++j) {
         const int x = lane + 64 * j;
-        cc[j] = (int)w.key[x];
-        vv[j] = w.val[x];
-        const int pc = (int)w.key[x > 0 ? x - 1 : 0];
-        const int nc = (int)w.key[x + 1];
+        const int pl = j > 0 ? __builtin_amdgcn_readlane(cc[j > 0 ? j - 1 : 0], 63) : 0;
+        const int nl = j + 1 < kFItems ? __builtin_amdgcn_readlane(cc[j + 1 < kFItems ? j + 1 : j], 0) : 0;
+        const int pc = dpp_i<0x138, 0xf>(pl, cc[j]);  // wave_shr 1: lane - 1, lane 0 keeps pl
+        const int nc = dpp_i<0x130, 0xf>(nl, cc[j]);  // wave_shl 1: lane + 1, lane 63 keeps nl
         const bool valid = x < P;
         head[j] = valid && (x == 0 || pc != cc[j]);
         cont[j] = x + 1 < P && nc == cc[j];
