@@ -199,10 +199,10 @@ __device__ __forceinline__ unsigned tile_entries(const Operands &op, const Works
     return flags;
 }
 
-// Classification of a tile's rows from their summed bounds: ub, bins, bin
-// lists and the wave's partial sums.
-__device__ __forceinline__ void tile_finish(const Operands &op, const Workspace &ws, int64_t tile, int64_t a0,
-                                            int64_t a1, const unsigned long long *s_ub, int64_t *s_part) {
+// Classification of a tile's rows from their summed bounds: ub, bins and the
+// wave's partial sums.  Returns the lane's bin (-1 past the last row).
+__device__ __forceinline__ int tile_classify(const Operands &op, const Workspace &ws, int64_t tile, int64_t a0,
+                                             int64_t a1, const unsigned long long *s_ub, int64_t *s_part) {
     const int lane = lane_id();
     const int64_t r = tile * 64 + lane;
     const bool valid = r < op.arows;
@@ -224,21 +224,52 @@ __device__ __forceinline__ void tile_finish(const Operands &op, const Workspace 
 #pragma unroll
     for (int bb = 0; bb < kNumBins; ++bb) {
         const unsigned long long m = __ballot(b == bb);
-        if (!m) continue;
-        if (lane == 0) s_part[2 + bb] = (int64_t)__popcll(m);
-        const bool listed = bb != kBinEmpty && ((ws.mode & kWsListSmall) || (bb != kBinW1 && bb != kBinW2));
-        if (!listed) continue;
-        int basei = 0;
-        if (lane == 0) basei = atomicAdd(&ws.ctrl->bin_count[bb], (int)__popcll(m));
-        basei = __builtin_amdgcn_readlane(basei, 0);
+        if (m && lane == 0) s_part[2 + bb] = (int64_t)__popcll(m);
+    }
+    return b;
+}
+
+// Whether bin bb's rows are listed (the single pass takes bins W1 and W2 by
+// row range unless the lists are asked for).
+__device__ __forceinline__ bool bin_listed(const Workspace &ws, int bb) {
+    return bb != kBinEmpty && ((ws.mode & kWsListSmall) || (bb != kBinW1 && bb != kBinW2));
+}
+
+// The tile's rows of listed bins, in lane order from list position base[bb].
+__device__ __forceinline__ void tile_list(const Workspace &ws, int64_t tile, int64_t a0, int64_t a1, int b,
+                                          const int *base) {
+    const int64_t r = tile * 64 + lane_id();
+#pragma unroll
+    for (int bb = 0; bb < kNumBins; ++bb) {
+        const unsigned long long m = __ballot(b == bb);
+        if (!m || !bin_listed(ws, bb)) continue;
         if (b == bb) {
             RowRec rec;
             rec.row = (int32_t)r;
             rec.na = (int32_t)min(a1 - a0, (int64_t)INT_MAX);
             rec.a0 = a0;
-            ws.bin_recs[(size_t)bb * (size_t)ws.list_stride + (size_t)(basei + (int)__popcll(m & lanemask_lt()))] = rec;
+            ws.bin_recs[(size_t)bb * (size_t)ws.list_stride + (size_t)(base[bb] + (int)__popcll(m & lanemask_lt()))] =
+                rec;
         }
     }
+}
+
+// A tile classified and listed by one wave alone: one list reservation per
+// bin that holds rows.
+__device__ __forceinline__ void tile_finish(const Operands &op, const Workspace &ws, int64_t tile, int64_t a0,
+                                            int64_t a1, const unsigned long long *s_ub, int64_t *s_part) {
+    const int b = tile_classify(op, ws, tile, a0, a1, s_ub, s_part);
+    int base[kNumBins];
+#pragma unroll
+    for (int bb = 0; bb < kNumBins; ++bb) {
+        const unsigned long long m = __ballot(b == bb);
+        base[bb] = 0;
+        if (!m || !bin_listed(ws, bb)) continue;
+        int bi = 0;
+        if (lane_id() == 0) bi = atomicAdd(&ws.ctrl->bin_count[bb], (int)__popcll(m));
+        base[bb] = __builtin_amdgcn_readlane(bi, 0);
+    }
+    tile_list(ws, tile, a0, a1, b, base);
 }
 
 template <bool DIR32>
@@ -247,6 +278,7 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
     __shared__ unsigned long long s_ub[NW][64];
     __shared__ int s_mark[NW][64];
     __shared__ int64_t s_part[NW][kAnalyzeFields];
+    __shared__ int s_lcnt[NW][kNumBins];  // listed rows per tile and bin, then the tile's first list position
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
     const int64_t tile = (int64_t)blockIdx.x * NW + w;
@@ -284,14 +316,36 @@ __global__ __launch_bounds__(kAnalyzeTpb) void k_analyze(Operands op, Workspace 
         }
     }
     wave_sync();
+    int b = -1;
     if (has && !heavy) {
         flags |= tile_entries<DIR32, kAnChunks>(op, ws, last, a0, E0, E1, s_ub[w], s_mark[w]);
         wave_sync();
-        tile_finish(op, ws, tile, a0, a1, s_ub[w], s_part[w]);
+        b = tile_classify(op, ws, tile, a0, a1, s_ub[w], s_part[w]);
     }
+    // List positions are reserved once per bin for the block's four tiles
+    // (one device-scope atomic on a bin's counter per block, not per wave).
+#pragma unroll
+    for (int bb = 0; bb < kNumBins; ++bb) {
+        const unsigned long long m = __ballot(b == bb);
+        if (lane == 0) s_lcnt[w][bb] = bin_listed(ws, bb) ? (int)__popcll(m) : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNumBins) {
+        int run = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) run += s_lcnt[i][threadIdx.x];
+        int base = run ? atomicAdd(&ws.ctrl->bin_count[threadIdx.x], run) : 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const int c = s_lcnt[i][threadIdx.x];
+            s_lcnt[i][threadIdx.x] = base;
+            base += c;
+        }
+    }
+    __syncthreads();
+    if (has && !heavy) tile_list(ws, tile, a0, a1, b, s_lcnt[w]);
     const unsigned fl = wave_or(flags);
     if (lane == 0 && fl) atomicOr(&ws.ctrl->flags, (unsigned long long)fl);
-    __syncthreads();
     if (threadIdx.x < kAnalyzeFields) {
         int64_t v = 0;
 #pragma unroll
