@@ -23,11 +23,14 @@
 //   2. k_large_fold (tasks on demand) folds a task's windows one at a time into
 //      a dense LDS accumulator of kPartCols sums that start at -0.0
 //      (-0.0 + p == p for every p, which is "the first product starts the sum").
-//      Each chunk of the window's products is sorted by its 12-bit window
-//      column with two stable 6-bit radix passes split the same way (so a
-//      column's products keep their serial order), and the first position of
-//      each column's run adds the run, left to right, to the column's sum.
-//      No value atomics.  The window's touched columns (a presence bitmap) are
+//      Each chunk of the window's products is split stably by the low 8 bits of
+//      its window column into 256 buckets (one per thread), so a column's
+//      products keep their serial order inside its bucket.  When no bucket
+//      holds more than kOwnerMax products, thread t adds bucket t's products,
+//      in order, to their columns' sums (every column belongs to one thread).
+//      Otherwise (hub columns) a second stable split by the high bits sorts
+//      the chunk by column, and the first position of each column's run adds
+//      the run, left to right, to the column's sum.  No value atomics.  The window's touched columns (a presence bitmap) are
 //      then written in column order at the window's place in the structural
 //      layout, and only their sums are reset.
 //   3. k_large_gapfix moves windows together in the rare rows where a window
@@ -41,6 +44,11 @@ namespace {
 using namespace dev;
 
 constexpr uint32_t kPartMask = (uint32_t)(kPartCols - 1);
+
+// Largest bucket (products of one chunk whose window columns share their low 8
+// bits) that the fold's owner walk takes; a chunk with a larger one is sorted
+// by column and walked by runs, which keeps a hub column's run in a register.
+constexpr int kOwnerMax = 32;
 
 // Bits that tell n keys apart.
 __device__ __forceinline__ int key_bits(int n) { return n <= 1 ? 0 : 32 - __clz(n - 1); }
@@ -327,17 +335,18 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
                                                     double *cval) {
     constexpr int NW = TPB / 64;
     constexpr int ITEMS = CAP / TPB;
-    constexpr int NB = 64;                        // radix buckets of a pass (6 bits)
+    constexpr int NB = TPB;                       // buckets of a chunk: one per thread
+    constexpr int NBITS = 8;                      // the window column's low bits name the bucket
     constexpr int WORDS = (int)(kPartCols / 32);  // presence bitmap of a window
-    static_assert(NB * NW == TPB && kPartLog > 6 && kPartLog <= 12,
-                  "two passes of at most 6 bits, one counter per thread");
+    static_assert(NB == 1 << NBITS && kPartLog > NBITS && kPartLog <= 16, "one bucket per thread, 16-bit columns");
     static_assert(WORDS <= TPB && kPartCols == 16 * TPB, "extraction: half a bitmap word per thread");
     __shared__ double s_acc[kPartCols];  // running sums, -0.0 where untouched
     __shared__ uint32_t s_bits[WORDS];   // columns touched in the window
-    __shared__ uint16_t s_lc[CAP];       // a chunk's products, sorted by column (then serial order)
+    __shared__ uint16_t s_lc[CAP];       // a chunk's products by bucket (then serial order)
     __shared__ double s_lv[CAP];
     __shared__ int s_gc[NB * NW];        // per (bucket, wave) counts, then their offsets; bucket-major
     __shared__ int s_ws[NW + 1];
+    __shared__ int s_mx[NW];
     __shared__ int s_next;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
     const RowRec *list = large_list(ws);
@@ -350,33 +359,6 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
     __syncthreads();
     unsigned flags = 0;
     unsigned long long stored = 0;
-    // One stable radix pass over a chunk's items (item j of lane l of wave
-    // wv is the chunk's item wv * 64 * ITEMS + 64 j + l), by `nbits` bits of
-    // the column from `shift`, into s_lc / s_lv.
-    auto radix_pass = [&](const uint16_t (&lc)[ITEMS], const double (&v)[ITEMS], const bool (&ok)[ITEMS], int shift,
-                          int nbits) {
-        uint32_t key[ITEMS];
-        int rank[ITEMS];
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) key[j] = ((uint32_t)lc[j] >> shift) & (uint32_t)(NB - 1);
-        s_gc[lane * NW + wv] = 0;
-        wave_sync();
-        wave_multisplit<ITEMS>(key, ok, nbits, &s_gc[wv], NW, rank);
-        __syncthreads();
-        int tot;
-        const int ex = block_excl_scan<TPB>(s_gc[tid], s_ws, tot);
-        s_gc[tid] = ex;
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            if (ok[j]) {
-                const int dst = s_gc[key[j] * NW + wv] + rank[j];
-                s_lc[dst] = lc[j];
-                s_lv[dst] = v[j];
-            }
-        }
-        __syncthreads();
-    };
     for (int t; (t = next_large_row(&ws.ctrl->fold_next, &s_next)) < ntasks;) {
         const PartTask task = ps.task[t];
         const int i = task.i;
@@ -405,52 +387,156 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
                 uint16_t lc[ITEMS];
                 double v[ITEMS];
                 bool ok[ITEMS];
+                uint32_t key[ITEMS];
+                int rank[ITEMS];
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
                     lc[j] = nlc[j];
                     v[j] = nv[j];
                     ok[j] = wv * (64 * ITEMS) + j * 64 + lane < total;
+                    key[j] = (uint32_t)lc[j] & (uint32_t)(NB - 1);
                 }
                 if (c0 + CAP < n) load(c0 + CAP);
-                // stable sort by the window-local column: low 6 bits, then high
-                radix_pass(lc, v, ok, 0, 6);
+                // Stable split by bucket: item j of lane l of wave wv is the
+                // chunk's item wv * 64 * ITEMS + 64 j + l (serial order); a
+                // wave's items of one bucket are ranked by ballots, and the
+                // waves are ordered by a scan per bucket.
+                for (int k = lane; k < NB; k += 64) s_gc[k * NW + wv] = 0;
+                wave_sync();
+                wave_multisplit<ITEMS>(key, ok, NBITS, &s_gc[wv], NW, rank);
+                __syncthreads();
+                int cw[NW], cnt = 0;
+#pragma unroll
+                for (int q = 0; q < NW; ++q) {
+                    cw[q] = s_gc[tid * NW + q];
+                    cnt += cw[q];
+                }
+                const int wmx = wave_max_dpp(cnt);
+                if (lane == 0) s_mx[wv] = wmx;
+                int tot;
+                const int st = block_excl_scan<TPB>(cnt, s_ws, tot);  // its barriers also publish s_mx
+                int mx = 0;
+#pragma unroll
+                for (int q = 0; q < NW; ++q) mx = max(mx, s_mx[q]);
+                {
+                    int o = st;
+#pragma unroll
+                    for (int q = 0; q < NW; ++q) {
+                        s_gc[tid * NW + q] = o;
+                        o += cw[q];
+                    }
+                }
+                __syncthreads();
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
-                    const int x = wv * (64 * ITEMS) + j * 64 + lane;
-                    lc[j] = ok[j] ? s_lc[x] : (uint16_t)0;
-                    v[j] = ok[j] ? s_lv[x] : 0.0;
+                    if (ok[j]) {
+                        const int dst = s_gc[key[j] * NW + wv] + rank[j];
+                        s_lc[dst] = lc[j];
+                        s_lv[dst] = v[j];
+                    }
                 }
-                __syncthreads();  // read back before the second pass overwrites
-                radix_pass(lc, v, ok, 6, kPartLog - 6);
-                // Each column's run folds, in serial order, onto its running
-                // sum at the run's first position.  Position x = q * TPB + tid:
-                // a wave reads 64 consecutive sorted positions (no bank
-                // conflicts), and a run that ends inside them has its end from
-                // a ballot of the heads, so its values are read without waiting
-                // on each next column; only the wave's last run reads on.
-#pragma unroll
-                for (int q = 0; q < ITEMS; ++q) {
-                    const int x = q * TPB + tid;
-                    const bool in = x < total;
-                    const int c = in ? (int)s_lc[x] : -1;
-                    const int pc = in && x > 0 ? (int)s_lc[x - 1] : -2;
-                    const bool head = in && pc != c;
-                    const uint64_t hm = __ballot(head);
-                    if (head) {
-                        const uint64_t above = lane == 63 ? 0ull : hm & (~0ull << (lane + 1));
-                        const int end = above ? x - lane + (__ffsll((long long)above) - 1) : min(x - lane + 64, total);
-                        double acc = s_acc[c];
-                        int y = x;
-                        for (; y + 2 <= end; y += 2) {
-                            const double p0 = s_lv[y], p1 = s_lv[y + 1];
-                            acc = acc + p0;
-                            acc = acc + p1;
+                __syncthreads();
+                if (mx <= kOwnerMax) {
+                    // Thread tid adds bucket tid's products, in serial order,
+                    // to their columns' sums (every column of the bucket is
+                    // its own), two at a time; a pair of one column is added
+                    // in turn.
+                    const int en = st + cnt;
+                    for (int x = st; x < en; x += 2) {
+                        const bool two = x + 1 < en;
+                        const int ca = s_lc[x];
+                        const double va = s_lv[x];
+                        const int cb = two ? (int)s_lc[x + 1] : ca;
+                        const double vb = two ? s_lv[x + 1] : 0.0;
+                        const double aa = s_acc[ca];
+                        if (cb == ca) {
+                            s_acc[ca] = two ? (aa + va) + vb : aa + va;
+                        } else {
+                            const double ab = s_acc[cb];
+                            s_acc[ca] = aa + va;
+                            s_acc[cb] = ab + vb;
+                            atomicOr(&s_bits[cb >> 5], 1u << (cb & 31));
                         }
-                        if (y < end) acc = acc + s_lv[y];
-                        if (!above)
-                            for (y = end; y < total && (int)s_lc[y] == c; ++y) acc = acc + s_lv[y];
-                        s_acc[c] = acc;
-                        atomicOr(&s_bits[c >> 5], 1u << (c & 31));
+                        atomicOr(&s_bits[ca >> 5], 1u << (ca & 31));
+                    }
+                } else {
+                    // A bucket holds a hub column's long run: a second stable
+                    // split, by the column's high bits, sorts the chunk by
+                    // column (position x of the first split is item j of lane
+                    // l of wave wv, x = wv * 64 * ITEMS + 64 j + l).
+                    constexpr int HBITS = kPartLog - NBITS;
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        const int x = wv * (64 * ITEMS) + j * 64 + lane;
+                        lc[j] = ok[j] ? s_lc[x] : (uint16_t)0;
+                        v[j] = ok[j] ? s_lv[x] : 0.0;
+                        key[j] = (uint32_t)lc[j] >> NBITS;
+                    }
+                    for (int k = lane; k < (1 << HBITS); k += 64) s_gc[k * NW + wv] = 0;
+                    __syncthreads();  // every item is read back before the split overwrites
+                    wave_multisplit<ITEMS>(key, ok, HBITS, &s_gc[wv], NW, rank);
+                    __syncthreads();
+                    // offsets of the (1 << HBITS) high buckets, in order, and
+                    // of each wave's items inside them
+                    {
+                        int hc = 0;
+                        if (tid < (1 << HBITS)) {
+#pragma unroll
+                            for (int q = 0; q < NW; ++q) hc += s_gc[tid * NW + q];
+                        }
+                        int htot;
+                        int o = block_excl_scan<TPB>(hc, s_ws, htot);
+                        if (tid < (1 << HBITS)) {
+#pragma unroll
+                            for (int q = 0; q < NW; ++q) {
+                                const int c = s_gc[tid * NW + q];
+                                s_gc[tid * NW + q] = o;
+                                o += c;
+                            }
+                        }
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) {
+                        if (ok[j]) {
+                            const int dst = s_gc[key[j] * NW + wv] + rank[j];
+                            s_lc[dst] = lc[j];
+                            s_lv[dst] = v[j];
+                        }
+                    }
+                    __syncthreads();
+                    // Each column's run folds, in serial order, onto its
+                    // running sum at the run's first position.  Position x =
+                    // q * TPB + tid: a wave reads 64 consecutive sorted
+                    // positions, and a run that ends inside them has its end
+                    // from a ballot of the heads, so its values are read
+                    // without waiting on each next column; only the wave's
+                    // last run reads on.
+#pragma unroll
+                    for (int q = 0; q < ITEMS; ++q) {
+                        const int x = q * TPB + tid;
+                        const bool in = x < total;
+                        const int c = in ? (int)s_lc[x] : -1;
+                        const int pc = in && x > 0 ? (int)s_lc[x - 1] : -2;
+                        const bool head = in && pc != c;
+                        const uint64_t hm = __ballot(head);
+                        if (head) {
+                            const uint64_t above = lane == 63 ? 0ull : hm & (~0ull << (lane + 1));
+                            const int end =
+                                above ? x - lane + (__ffsll((long long)above) - 1) : min(x - lane + 64, total);
+                            double acc = s_acc[c];
+                            int y = x;
+                            for (; y + 2 <= end; y += 2) {
+                                const double p0 = s_lv[y], p1 = s_lv[y + 1];
+                                acc = acc + p0;
+                                acc = acc + p1;
+                            }
+                            if (y < end) acc = acc + s_lv[y];
+                            if (!above)
+                                for (y = end; y < total && (int)s_lc[y] == c; ++y) acc = acc + s_lv[y];
+                            s_acc[c] = acc;
+                            atomicOr(&s_bits[c >> 5], 1u << (c & 31));
+                        }
                     }
                 }
                 __syncthreads();
