@@ -40,9 +40,6 @@ constexpr uint64_t kStAgg = 1ull << 62;   // tile status: count of the tile only
 constexpr uint64_t kStIncl = 1ull << 63;  // tile status: count of tiles 0..t
 constexpr uint64_t kStVal = kStAgg - 1;
 constexpr int kSpinLimit = 1 << 22;       // never reached unless a tile fails to publish
-#ifndef SPG_LB_PROBE
-#define SPG_LB_PROBE 1
-#endif
 // Wait between polls of an unpublished earlier tile, in units of 64 clocks.
 // With one status per line (kStatusStride) the wait length hardly matters
 // between 2 and 16; with packed statuses polling contended for one memory
@@ -51,9 +48,16 @@ constexpr int kSpinLimit = 1 << 22;       // never reached unless a tile fails t
 #define SPG_LB_SLEEP 8
 #endif
 
-// Look-back loads per lane per probe (64 tiles each).  One: every status of a
-// probe is its own line, and wider probes measured slower.
-constexpr int kLbProbe = SPG_LB_PROBE;
+// Statuses read per look-back probe (lanes 0 .. kLbWidth-1 of wave 0, one
+// line each).  The nearest inclusive total is usually a few tiles back, and a
+// narrower probe returns sooner: on the 2^20-row 16-per-row product the
+// single-pass kernel took 1.71 ms with 64 statuses per probe, 1.57 with 16,
+// 1.54 with 12, 1.57 with 8 and 1.66 with 4.
+#ifndef SPG_LB_WIDTH
+#define SPG_LB_WIDTH 12
+#endif
+constexpr int kLbWidth = SPG_LB_WIDTH;
+static_assert(kLbWidth >= 1 && kLbWidth <= 64, "one status per lane");
 
 // A wave's LDS.  The row's entry metadata is dead once its gathers are
 // issued, so it shares storage with the sorted products.
@@ -448,7 +452,7 @@ __device__ __forceinline__ void st_status(uint64_t *p, uint64_t v) {
 
 // Exclusive entry offset of tile t (one wave): publish the tile's count, then
 // sum the counts of earlier tiles back to the nearest published inclusive
-// total, 64 * kLbProbe tiles per probe.
+// total, kLbWidth tiles per probe.
 __device__ int64_t tile_lookback(uint64_t *status, int64_t t, int64_t agg, unsigned &flags) {
     const int lane = lane_id();
     if (lane == 0) st_status(&status[t * kStatusStride], (t == 0 ? kStIncl : kStAgg) | (uint64_t)agg);
@@ -456,36 +460,25 @@ __device__ int64_t tile_lookback(uint64_t *status, int64_t t, int64_t agg, unsig
     int64_t excl = 0;
     int64_t p = t - 1;
     for (int spins = 0;;) {
-        // kLbProbe coalesced loads in flight per lane: load j reads the
-        // statuses at distances 64j + lane behind p.
-        uint64_t s[kLbProbe];
-#pragma unroll
-        for (int j = 0; j < kLbProbe; ++j) {
-            const int64_t q = p - lane - 64 * j;
-            s[j] = q >= 0 ? ld_status(&status[q * kStatusStride]) : kStIncl;  // before tile 0: total 0
-        }
+        // Lane l < kLbWidth reads the status at distance l behind p; the
+        // other lanes read nothing and stand for "not yet known", so a probe
+        // ends at distance kLbWidth.
+        const int64_t q = p - lane;
+        const uint64_t s = lane >= kLbWidth ? 0ull : q >= 0 ? ld_status(&status[q * kStatusStride]) : kStIncl;  // before tile 0: total 0
         // fn: distance of the nearest unpublished tile, fi: of the nearest
-        // inclusive total (the smallest load index j holding one wins).
-        int fn = 64 * kLbProbe, fi = 64 * kLbProbe;
-#pragma unroll
-        for (int j = kLbProbe - 1; j >= 0; --j) {
-            const uint64_t nr_m = __ballot((s[j] & (kStIncl | kStAgg)) == 0);
-            const uint64_t incl_m = __ballot((s[j] & kStIncl) != 0);
-            if (nr_m) fn = 64 * j + __ffsll((long long)nr_m) - 1;
-            if (incl_m) fi = 64 * j + __ffsll((long long)incl_m) - 1;
-        }
+        // inclusive total.
+        const uint64_t nr_m = __ballot((s & (kStIncl | kStAgg)) == 0);
+        const uint64_t incl_m = __ballot((s & kStIncl) != 0);
+        const int fn = nr_m ? __ffsll((long long)nr_m) - 1 : 64;
+        const int fi = incl_m ? __ffsll((long long)incl_m) - 1 : 64;
         const int lim = fi < fn ? fi + 1 : fn;  // statuses summed: distances < lim
-        int64_t part = 0;
-#pragma unroll
-        for (int j = 0; j < kLbProbe; ++j)
-            if (64 * j + lane < lim) part += (int64_t)(s[j] & kStVal);
-        if (lim > 0) excl += wave_sum(part);
+        if (lim > 0) excl += wave_sum(lane < lim ? (int64_t)(s & kStVal) : (int64_t)0);
         if (fi < fn) break;
         if (fn > 0) {
             p -= fn;
         } else {
             // Tile p is unpublished: wait on its status alone (every lane
-            // loads the same line) before the next full probe.
+            // loads the same line) before the next probe.
             bool ready = false;
             while (!ready && ++spins <= kSpinLimit) {
                 __builtin_amdgcn_s_sleep(SPG_LB_SLEEP);
