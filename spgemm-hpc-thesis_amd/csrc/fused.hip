@@ -500,8 +500,10 @@ __device__ __forceinline__ int64_t lane64(int64_t v, int k) {
                      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k));
 }
 
+// Waves per SIMD the classic tiles' kernel is compiled for (0: no
+// constraint); five fit its LDS and 96 VGPRs without spills.
 #ifndef SPG_FUSED_WPE
-#define SPG_FUSED_WPE 0
+#define SPG_FUSED_WPE 5
 #endif
 #if SPG_FUSED_WPE
 #define SPG_FUSED_ATTR __attribute__((amdgpu_waves_per_eu(SPG_FUSED_WPE)))
@@ -515,8 +517,8 @@ __device__ __forceinline__ int64_t lane64(int64_t v, int k) {
 // count from the staged lengths, and the tile also publishes the statistics
 // the analysis would have gathered (products, rows per bin).
 template <bool SMALLB, int R, bool DIRECT = false>
-__global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused(Operands op, Workspace ws, FusedOut fo) {
-    constexpr int NW = kFusedTpb / 64;
+__global__ __launch_bounds__(kFusedTileTpb) SPG_FUSED_ATTR void k_fused(Operands op, Workspace ws, FusedOut fo) {
+    constexpr int NW = kFusedTileTpb / 64;
     constexpr int NS = kDirectFields - 1;  // DIRECT statistics besides the distinct columns
     __shared__ FusedWave<SMALLB> s_w[NW];
     __shared__ int64_t s_wtot[NW];
@@ -910,7 +912,7 @@ __global__ void k_pack_map(const int64_t *tcnt, const int64_t *toff, int64_t nbl
 // row it owns.  Tiles run in map order, which is row order, so the look-back
 // is that of consecutive-row tiles.
 template <bool SMALLB>
-__global__ __launch_bounds__(kFusedTpb) SPG_FUSED_ATTR void k_fused_packed(Operands op, Workspace ws, FusedOut fo) {
+__global__ __launch_bounds__(kFusedTpb) void k_fused_packed(Operands op, Workspace ws, FusedOut fo) {
     constexpr int NW = kFusedTpb / 64;
     constexpr int R = 2;
     constexpr int BR = kAnalyzeRows;  // rows per block
@@ -1139,7 +1141,7 @@ void launch_pack_tiles(const int64_t *part, int64_t nblocks, int64_t *tcnt, int6
 void launch_fused_direct(const Operands &op, const Workspace &ws, const FusedOut &fo, bool small_b,
                          int rows_per_wave, hipStream_t stream) {
     if (op.arows <= 0 || fo.ntiles <= 0) return;
-    const dim3 grid((unsigned)fo.ntiles), block(kFusedTpb);
+    const dim3 grid((unsigned)fo.ntiles), block(kFusedTileTpb);
     if (rows_per_wave == 1) {
         if (small_b) hipLaunchKernelGGL((k_fused<true, 1, true>), grid, block, 0, stream, op, ws, fo);
         else hipLaunchKernelGGL((k_fused<false, 1, true>), grid, block, 0, stream, op, ws, fo);
@@ -1151,7 +1153,7 @@ void launch_fused_direct(const Operands &op, const Workspace &ws, const FusedOut
 
 int64_t fused_tiles(int32_t rows, int rows_per_wave) {
     const int64_t tile =
-        rows_per_wave == kFusedDealRows ? (int64_t)kFusedDealTile : (int64_t)(kFusedTpb / 64) * rows_per_wave;
+        rows_per_wave == kFusedDealRows ? (int64_t)kFusedDealTile : (int64_t)(kFusedTileTpb / 64) * rows_per_wave;
     return ((int64_t)rows + tile - 1) / tile;
 }
 
@@ -1167,12 +1169,15 @@ void launch_fused(const Operands &op, const Workspace &ws, const FusedOut &fo, b
         const dim3 dblock(kFusedDealTpb);
         if (small_b) hipLaunchKernelGGL((k_fused_dealt<true>), grid, dblock, 0, stream, op, ws, fo);
         else hipLaunchKernelGGL((k_fused_dealt<false>), grid, dblock, 0, stream, op, ws, fo);
-    } else if (rows_per_wave == 1) {
-        if (small_b) hipLaunchKernelGGL((k_fused<true, 1>), grid, block, 0, stream, op, ws, fo);
-        else hipLaunchKernelGGL((k_fused<false, 1>), grid, block, 0, stream, op, ws, fo);
     } else {
-        if (small_b) hipLaunchKernelGGL((k_fused<true, 2>), grid, block, 0, stream, op, ws, fo);
-        else hipLaunchKernelGGL((k_fused<false, 2>), grid, block, 0, stream, op, ws, fo);
+        const dim3 tblock(kFusedTileTpb);
+        if (rows_per_wave == 1) {
+            if (small_b) hipLaunchKernelGGL((k_fused<true, 1>), grid, tblock, 0, stream, op, ws, fo);
+            else hipLaunchKernelGGL((k_fused<false, 1>), grid, tblock, 0, stream, op, ws, fo);
+        } else {
+            if (small_b) hipLaunchKernelGGL((k_fused<true, 2>), grid, tblock, 0, stream, op, ws, fo);
+            else hipLaunchKernelGGL((k_fused<false, 2>), grid, tblock, 0, stream, op, ws, fo);
+        }
     }
 }
 

@@ -92,12 +92,20 @@ constexpr int kCapLargeCompact = 1024;
 constexpr int kTpbLargeCompact = 512;
 // Single-pass path: rows with at most kFusedCap products and A entries are
 // computed by one wave each, kFusedRows consecutive rows per wave, in tiles of
-// (kFusedTpb / 64) * kFusedRows rows (these are exactly the rows of bins W1, W2).
+// (kFusedTileTpb / 64) * kFusedRows rows (these are exactly the rows of bins
+// W1, W2).  Four-wave tiles compiled for five waves per SIMD (96 VGPRs) keep
+// 20 waves per CU: on the 2^20-row 16-per-row product their kernel took 1.515
+// ms against 1.543 ms for eight-wave tiles (16 rows, 109 VGPRs, 16 waves per
+// CU).  The packed tiles (kFusedTpb) take 16 rows per workgroup.
 constexpr int kFusedCap = kCapW2;
 #ifndef SPG_FUSED_TPB
 #define SPG_FUSED_TPB 512
 #endif
 constexpr int kFusedTpb = SPG_FUSED_TPB;
+#ifndef SPG_FUSED_TILE_TPB
+#define SPG_FUSED_TILE_TPB 256
+#endif
+constexpr int kFusedTileTpb = SPG_FUSED_TILE_TPB;
 constexpr int kFusedRows = 2;  // default rows per wave of a tile (1 or 2)
 // Dealt tiles (rows_per_wave == kFusedDealRows, a mode id): a tile is 32
 // consecutive rows on a 1024-thread workgroup, and its single-pass rows are
@@ -308,7 +316,7 @@ bool prepare_numeric_kernels(size_t lds_per_block);  // numeric.hip; false: wide
 // Single-pass kernel over all rows (fused.hip): computes the rows of bins W1,
 // W2 and reserves the symbolic counts of the others.  small_b: every B entry
 // index i has 8*i < 2^32.
-// rows_per_wave: 1 or 2 (rows of a tile per wave; the tile has kFusedTpb/64
+// rows_per_wave: 1 or 2 (rows of a tile per wave; the tile has kFusedTileTpb/64
 // waves), kFusedDealRows (dealt 32-row tiles) or kFusedPackRows (packed
 // tiles; fused_tiles does not apply, the plan launches an upper bound).
 int64_t fused_tiles(int32_t rows, int rows_per_wave);

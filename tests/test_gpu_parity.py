@@ -106,7 +106,7 @@ EDGE_SHAPES = [
 
 @pytest.mark.parametrize("path,fmt", [p[1:] for p in PATH_FORMATS], ids=[p[0] for p in PATH_FORMATS])
 def test_single_pass_tile_edges(engine, path, fmt):
-    # Block and tile boundaries of every single-pass tiling (16-row tiles,
+    # Block and tile boundaries of every single-pass tiling (8-row tiles,
     # dealt 32-row tiles, packed tiles per 256-row block) against the oracle.
     pin(engine, path, fmt)
     try:
