@@ -50,11 +50,12 @@ constexpr int kSpinLimit = 1 << 22;       // never reached unless a tile fails t
 
 // Statuses read per look-back probe (lanes 0 .. kLbWidth-1 of wave 0, one
 // line each).  The nearest inclusive total is usually a few tiles back, and a
-// narrower probe returns sooner: on the 2^20-row 16-per-row product the
-// single-pass kernel took 1.71 ms with 64 statuses per probe, 1.57 with 16,
-// 1.54 with 12, 1.57 with 8 and 1.66 with 4.
+// narrower probe returns sooner.  On the 2^20-row 16-per-row product with
+// 16-row tiles the single-pass kernel took 1.71 ms with 64 statuses per probe,
+// 1.57 with 16, 1.54 with 12, 1.57 with 8 and 1.66 with 4; with 8-row tiles
+// 1.512 ms with 12, 1.501 with 16, 1.523 with 24 and 1.568 with 8.
 #ifndef SPG_LB_WIDTH
-#define SPG_LB_WIDTH 12
+#define SPG_LB_WIDTH 16
 #endif
 constexpr int kLbWidth = SPG_LB_WIDTH;
 static_assert(kLbWidth >= 1 && kLbWidth <= 64, "one status per lane");
