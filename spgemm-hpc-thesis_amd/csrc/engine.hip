@@ -191,7 +191,8 @@ spg_status counts_deferred(spg_context *ctx, ProductPlan &p, unsigned bins, bool
             p.ps.pseg = ctx->pseg.as<uint32_t>();
             launch_part_batches(p.op, p.ws, p.ps, nlarge, st);
         }
-        launch_symbolic(p.op, p.ws, 1u << kBinLarge, ctx->cus, st, p.lsave, p.part_counts ? &p.ps : nullptr, nlarge);
+        launch_symbolic(p.op, p.ws, 1u << kBinLarge, ctx->cus, st, p.lsave, p.part_counts ? &p.ps : nullptr, nlarge,
+                        ctx->side, ctx->fork, ctx->join);
     }
     if (scan) launch_scan(p.ws.snnz, ctx->rowptr[0].as<int64_t>(), p.rows, p.ws.scan_tmp, &p.ws.ctrl->structural, st);
     if ((s = read_control(ctx)) != SPG_OK) return s;

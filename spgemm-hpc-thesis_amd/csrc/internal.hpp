@@ -291,9 +291,12 @@ constexpr unsigned kAllBins = kSmallBins | kBigBins;
 // kCapLarge A entries (pseg), and sums the products and the largest product
 // count of the rows with snnz > part->skip_le into the control block.
 // nlarge: the number of large rows when the host knows it (-1 otherwise); it
-// lets the heaviest be counted by a launch of their own.
+// lets the heaviest be counted by a launch of their own, on `side` beside the
+// other rows' count when side, fork and join are given (the side stream is
+// forked from and joined back to `stream` by the two events).
 void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream,
-                     uint32_t *lsave = nullptr, const PartState *part = nullptr, int nlarge = -1);
+                     uint32_t *lsave = nullptr, const PartState *part = nullptr, int nlarge = -1,
+                     hipStream_t side = nullptr, hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 // Batches of kCapLarge A entries of every large row and their exclusive scan
 // (ps.segbase; total into ctrl->part_units), before the large-row count
 // (large_part.hip).

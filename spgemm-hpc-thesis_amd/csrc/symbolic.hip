@@ -522,7 +522,8 @@ bool part_available(int64_t bcols, int64_t lwidth) {
 }
 
 void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int cus, hipStream_t stream,
-                     uint32_t *lsave, const PartState *part, int nlarge) {
+                     uint32_t *lsave, const PartState *part, int nlarge, hipStream_t side, hipEvent_t fork,
+                     hipEvent_t join) {
     if (op.arows <= 0) return;
     if (bins & (1u << kBinW1)) launch_hash<64, kCapW1>(op, ws, kBinW1, cus, stream);
     if (bins & (1u << kBinW2)) launch_hash<64, kCapW2>(op, ws, kBinW2, cus, stream);
@@ -541,17 +542,27 @@ void launch_symbolic(const Operands &op, const Workspace &ws, unsigned bins, int
             uint32_t *save = op.bcols <= ws.lwidth ? lsave : nullptr;
             // With the number of large rows known, the heaviest (a row of
             // 1.5 M products otherwise holds one 256-thread workgroup for most
-            // of the kernel) are counted first by 1024-thread workgroups.
+            // of the kernel) are counted by 1024-thread workgroups of their
+            // own, one per row.  They are few, so with a side stream they run
+            // beside the other rows' count (which skips them) instead of
+            // before it.
             int heavy_n = 0;
+            const bool forked = nlarge > 0 && side && fork && join;
             if (nlarge > 0 && g_wide_lds.load(std::memory_order_relaxed)) {
                 heavy_n = nlarge < kCountHeavyRows ? nlarge : kCountHeavyRows;
+                if (forked) {
+                    (void)hipEventRecord(fork, stream);
+                    (void)hipStreamWaitEvent(side, fork, 0);
+                }
                 hipLaunchKernelGGL((k_large_symbolic<kTpbLargeWide, kCapLarge, true>), dim3(heavy_n),
-                                   dim3(kTpbLargeWide), (unsigned)dyn, stream, op, ws, save, ps, kCountHeavy, heavy_n,
-                                   1);
+                                   dim3(kTpbLargeWide), (unsigned)dyn, forked ? side : stream, op, ws, save, ps,
+                                   kCountHeavy, heavy_n, 1);
+                if (forked) (void)hipEventRecord(join, side);
             }
             const int g = occupancy_grid(k_large_symbolic<kTpbLarge, kCapLarge, true>, kTpbLarge, cus, 0, dyn);
             hipLaunchKernelGGL((k_large_symbolic<kTpbLarge, kCapLarge, true>), dim3(g), dim3(kTpbLarge),
                                (unsigned)dyn, stream, op, ws, save, ps, kCountHeavy, heavy_n, 0);
+            if (forked && heavy_n > 0) (void)hipStreamWaitEvent(stream, join, 0);
         }
         else if (ws.lwidth <= kLdsBitsColsWide && g_wide_lds.load(std::memory_order_relaxed))
             hipLaunchKernelGGL((k_large_symbolic<kTpbLargeWide, kCapLarge, true>), dim3(ws.lblocks),
