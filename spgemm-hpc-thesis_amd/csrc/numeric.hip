@@ -32,8 +32,8 @@ using namespace dev;
 template <int TPB, int CAP>
 __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws, const RowRec &cur, int P, int L,
                                         const int64_t *crowptr, int32_t *ccol, double *cval, double *s_a,
-                                        int64_t *s_b64, int *s_c, int *s_wsi, unsigned long long &stored,
-                                        unsigned &flags) {
+                                        int64_t *s_b64, int *s_c, int *s_wsi, uint64_t *s_ws64,
+                                        unsigned long long &stored, unsigned &flags) {
     constexpr int ITEMS = CAP / TPB;
     // Bucketed items are keys (column - row minimum) << 32 | s: one LDS word
     // per item, ordered as (column, s), the bucket recomputed from the key.
@@ -149,35 +149,55 @@ __device__ __forceinline__ void esc_row(const Operands &op, const Workspace &ws,
     }
     __syncthreads();
 
-    // --- ordered fold at each column's first position, scan, coalesced write
+    // --- ordered fold at each column's first position, then the output: slot
+    // j of thread tid is sorted position tid + j * TPB.  Every slot's column
+    // and value are read first; a head folds its column's run in serial
+    // order; one block scan of 16-bit per-slot flags (four slots per word)
+    // gives every stored entry's place.
     if constexpr (!kEarlyRange) {
         cbase = crowptr[cur.row];
         lim = crowptr[cur.row + 1] - cbase;
     }
-    int run = 0;
-    for (int x0 = 0; x0 < P; x0 += TPB) {
-        const int x = x0 + tid;
-        int nz = 0;
-        int32_t c = 0;
-        double v = 0.0;
-        if (x < P) {
-            c = s_col[x];
-            if (x == 0 || s_col[x - 1] != c) {
-                v = s_a[x];
-                for (int y = x + 1; y < P && s_col[y] == c; ++y) v = v + s_a[y];
-                nz = op.keep_zeros || v != 0.0;
-            }
+    int32_t cc[ITEMS];
+    double vv[ITEMS];
+    bool hd[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const int x = tid + j * TPB;
+        const bool in = x < P;
+        cc[j] = in ? s_col[x] : 0;
+        vv[j] = in ? s_a[x] : 0.0;
+        const int32_t pc = in && x > 0 ? s_col[x - 1] : 0;
+        hd[j] = in && (x == 0 || pc != cc[j]);
+    }
+    uint64_t pk[(ITEMS + 3) / 4];
+#pragma unroll
+    for (int h = 0; h < (ITEMS + 3) / 4; ++h) pk[h] = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        if (hd[j]) {
+            double v = vv[j];
+            for (int y = tid + j * TPB + 1; y < P && s_col[y] == cc[j]; ++y) v = v + s_a[y];
+            vv[j] = v;
         }
-        int tot;
-        const int ex = block_excl_scan<TPB>(nz, s_wsi, tot);
-        if (nz) {
-            const int64_t o = (int64_t)run + ex;
+        const bool nz = hd[j] && (op.keep_zeros || vv[j] != 0.0);
+        hd[j] = nz;
+        pk[j / 4] |= (uint64_t)(nz ? 1u : 0u) << (16 * (j % 4));
+    }
+    uint64_t ex[(ITEMS + 3) / 4], tt[(ITEMS + 3) / 4];
+#pragma unroll
+    for (int h = 0; h < (ITEMS + 3) / 4; ++h) ex[h] = block_excl_scan<TPB>(pk[h], s_ws64, tt[h]);
+    int64_t run = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        if (hd[j]) {
+            const int64_t o = run + (int64_t)((ex[j / 4] >> (16 * (j % 4))) & 0xffffu);
             if (o < lim) {
-                __builtin_nontemporal_store(c, &ccol[cbase + o]);
-                __builtin_nontemporal_store(v, &cval[cbase + o]);
+                __builtin_nontemporal_store(cc[j], &ccol[cbase + o]);
+                __builtin_nontemporal_store(vv[j], &cval[cbase + o]);
             }
         }
-        run += tot;
+        run += (int64_t)((tt[j / 4] >> (16 * (j % 4))) & 0xffffu);
     }
     if (tid == 0) {
         ws.annz[cur.row] = run;
@@ -192,6 +212,7 @@ struct EscLds {
     int64_t b64[CAP];  // meta: B start of entry e | items: keys (column, s), then sorted columns
     int c[CAP + 1];    // meta: product offset of entry e | bucket counts, then bucket starts
     int wsi[2 * (TPB / 64) + 2];
+    uint64_t ws64[TPB / 64];  // the output scan's per-wave totals
 };
 
 // One row of the software pipeline (see hash_step in symbolic.hip).
@@ -210,16 +231,24 @@ __device__ __forceinline__ void esc_step(const Operands &op, const Workspace &ws
     if (P <= CAP) {
         if (tid == 0) sh.c[cur.na] = P;
         __syncthreads();
-        esc_row<TPB, CAP>(op, ws, cur, P, L, crowptr, ccol, cval, sh.a, sh.b64, sh.c, sh.wsi, stored, flags);
+        esc_row<TPB, CAP>(op, ws, cur, P, L, crowptr, ccol, cval, sh.a, sh.b64, sh.c, sh.wsi, sh.ws64, stored, flags);
     }
     __syncthreads();  // LDS is reused by the next row
     cur = nxt;
     i += G;
 }
 
+// Waves per SIMD the value kernels keep (1: no constraint): bin W4's wide
+// format would otherwise take 193 VGPRs (two waves), bin W1's 93 (five).
+template <int CAP>
+constexpr int esc_min_waves() {
+    return CAP == kCapW4 ? 3 : CAP == kCapW1 ? 6 : 1;
+}
+
 template <int TPB, int CAP, bool NARROW>
-__global__ __launch_bounds__(TPB) void k_numeric_esc(Operands op, Workspace ws, int binid, const int64_t *crowptr,
-                                                     int32_t *ccol, double *cval) {
+__global__ __launch_bounds__(TPB, esc_min_waves<CAP>()) void k_numeric_esc(Operands op, Workspace ws, int binid,
+                                                                            const int64_t *crowptr, int32_t *ccol,
+                                                                            double *cval) {
     static_assert(CAP % TPB == 0 && CAP <= 65536, "CAP must be a multiple of TPB and fit 16-bit indices");
     __shared__ EscLds<TPB, CAP> sh;
     const int tid = threadIdx.x;
