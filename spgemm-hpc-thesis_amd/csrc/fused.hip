@@ -65,14 +65,18 @@ static_assert(kLbWidth >= 1 && kLbWidth <= 64, "one status per lane");
 template <bool SMALLB>
 struct FusedWave {
     using bst_t = typename std::conditional<SMALLB, uint32_t, int64_t>::type;
-    uint32_t key[kFKeys];  // bucket-ordered keys (+ padding), then sorted columns
+    // The rows' entry metadata is dead once their gathers are issued (and
+    // their items are in registers), so the sort's arrays share its storage.
     union {
         struct {
             double av[kFusedCap];    // A value of each of the row's entries
             bst_t bst[kFusedCap];    // first B entry of each entry's B row
             int off[kFusedCap + 1];  // serial index of each entry's first product
         };
-        double val[kFusedCap + 64];  // products at their sorted positions (+ one dummy per lane)
+        struct {
+            uint32_t key[kFKeys];        // bucket-ordered keys (+ padding), then sorted columns
+            double val[kFusedCap + 64];  // products at their sorted positions (+ one dummy per lane)
+        };
     };
     alignas(16) int start[kFNb + 4];  // bucket counts, then bucket starts; [kFNb+1] is a dummy counter
     uint8_t sx[kFKeys];               // serial index by bucket position (wide rows)
