@@ -343,8 +343,9 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
     __shared__ double s_acc[kPartCols];  // running sums, -0.0 where untouched
     __shared__ uint32_t s_bits[WORDS];   // columns touched in the window
     __shared__ uint16_t s_lc[CAP];       // a chunk's products by bucket (then serial order)
-    __shared__ double s_lv[CAP];
+    __shared__ __align__(16) double s_lv[CAP];
     __shared__ int s_gc[NB * NW];        // per (bucket, wave) counts, then their offsets; bucket-major
+    __shared__ uint64_t s_hm[CAP / 64];  // heads of the sorted chunk's column runs
     __shared__ int s_ws[NW + 1];
     __shared__ int s_mx[NW];
     __shared__ int s_next;
@@ -506,34 +507,67 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
                     }
                     __syncthreads();
                     // Each column's run folds, in serial order, onto its
-                    // running sum at the run's first position.  Position x =
-                    // q * TPB + tid: a wave reads 64 consecutive sorted
-                    // positions, and a run that ends inside them has its end
-                    // from a ballot of the heads, so its values are read
-                    // without waiting on each next column; only the wave's
-                    // last run reads on.
+                    // running sum at the run's first position (x = q * TPB +
+                    // tid).  The runs' heads are marked first, one bitmap word
+                    // per 64 sorted positions, so that a run's end is the next
+                    // head, or the chunk's end, wherever it lies; its values
+                    // are then read eight at a time, 16 bytes per read, with
+                    // no wait on each next column.
+                    int hc[ITEMS];
+                    bool hd[ITEMS];
+                    uint64_t hm[ITEMS];
 #pragma unroll
                     for (int q = 0; q < ITEMS; ++q) {
                         const int x = q * TPB + tid;
                         const bool in = x < total;
-                        const int c = in ? (int)s_lc[x] : -1;
+                        hc[q] = in ? (int)s_lc[x] : -1;
                         const int pc = in && x > 0 ? (int)s_lc[x - 1] : -2;
-                        const bool head = in && pc != c;
-                        const uint64_t hm = __ballot(head);
-                        if (head) {
-                            const uint64_t above = lane == 63 ? 0ull : hm & (~0ull << (lane + 1));
-                            const int end =
-                                above ? x - lane + (__ffsll((long long)above) - 1) : min(x - lane + 64, total);
+                        hd[q] = in && pc != hc[q];
+                        hm[q] = __ballot(hd[q]);
+                        if (lane == 0) s_hm[x >> 6] = hm[q];
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int q = 0; q < ITEMS; ++q) {
+                        if (hd[q]) {
+                            const int x = q * TPB + tid;
+                            const int c = hc[q];
+                            const uint64_t above = lane == 63 ? 0ull : hm[q] & (~0ull << (lane + 1));
+                            int end = total;
+                            if (above) {
+                                end = x - lane + (__ffsll((long long)above) - 1);
+                            } else {
+                                for (int k = (x >> 6) + 1; k * 64 < total; ++k) {
+                                    const uint64_t h = s_hm[k];
+                                    if (h) {
+                                        end = k * 64 + (__ffsll((long long)h) - 1);
+                                        break;
+                                    }
+                                }
+                            }
                             double acc = s_acc[c];
                             int y = x;
+                            if (y & 1) acc = acc + s_lv[y++];
+                            for (; y + 8 <= end; y += 8) {
+                                const double2 p0 = *reinterpret_cast<const double2 *>(&s_lv[y]);
+                                const double2 p1 = *reinterpret_cast<const double2 *>(&s_lv[y + 2]);
+                                const double2 p2 = *reinterpret_cast<const double2 *>(&s_lv[y + 4]);
+                                const double2 p3 = *reinterpret_cast<const double2 *>(&s_lv[y + 6]);
+                                acc = acc + p0.x;
+                                acc = acc + p0.y;
+                                acc = acc + p1.x;
+                                acc = acc + p1.y;
+                                acc = acc + p2.x;
+                                acc = acc + p2.y;
+                                acc = acc + p3.x;
+                                acc = acc + p3.y;
+                            }
                             for (; y + 2 <= end; y += 2) {
-                                const double p0 = s_lv[y], p1 = s_lv[y + 1];
-                                acc = acc + p0;
-                                acc = acc + p1;
+                                const double2 p0 = *reinterpret_cast<const double2 *>(&s_lv[y]);
+                                acc = acc + p0.x;
+                                acc = acc + p0.y;
                             }
                             if (y < end) acc = acc + s_lv[y];
-                            if (!above)
-                                for (y = end; y < total && (int)s_lc[y] == c; ++y) acc = acc + s_lv[y];
                             s_acc[c] = acc;
                             atomicOr(&s_bits[c >> 5], 1u << (c & 31));
                         }

@@ -25,7 +25,8 @@
 #include <string>
 
 // 512 MiB: the workspace of a 2^20-row product with 16 entries per row and
-// 256 products per row (about 290 MB) fits, with room for big rows.
+// 256 products per row (about 295 MB, of which 8.4 MB are the
+// single-pass tiles' statuses) fits, with room for big rows.
 constexpr size_t kScratchBytes = size_t(1) << 29;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
