@@ -679,6 +679,15 @@ def run_served(args, wl):
         for a, b in pairs:
             handles.append((srv.register(a), srv.register(b)))
         big_h = (srv.register(big_a), srv.register(big_b))
+        if saturated:
+            # one untimed pass: the server's pooled batches grow to their
+            # steady sizes (growing a batch that is already tens of GB
+            # reallocates it, which stalls the device for seconds once)
+            warm = [srv.submit_resident(*(big_h if is_big[k] else handles[pick[k]])) for k in range(n_req)]
+            for r in warm:
+                srv.wait(r)
+                srv.release(r)
+        st0 = srv.stats()
         rids = [None] * n_req
         lat = np.zeros(n_req)
         nnzs = np.zeros(n_req, np.int64)
@@ -720,8 +729,9 @@ def run_served(args, wl):
             exact &= O.csr_digest(_o(got)) == O.csr_digest(_o(alone))
         srv.close()
         return {"throughput": n_req / wall, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-                "p99_ms": float(np.percentile(lat, 99) * 1e3), "wall_s": wall, "batches": st["batches"],
-                "busy_s": st["busy_s"], "nnz": nnzs, "exact": exact}
+                "p99_ms": float(np.percentile(lat, 99) * 1e3), "wall_s": wall,
+                "batches": st["batches"] - st0["batches"], "busy_s": st["busy_s"] - st0["busy_s"], "nnz": nnzs,
+                "exact": exact}
 
     log("bench: served with continuous batching")
     cb = serve(4096)
@@ -760,7 +770,8 @@ def run_served(args, wl):
                                  "products_per_s": round(sat_cb["throughput"], 1),
                                  "one_per_launch_products_per_s": round(sat_seq["throughput"], 1),
                                  "speedup_vs_one_per_launch": round(sat_cb["throughput"] / sat_seq["throughput"], 3),
-                                 "batches": sat_cb["batches"], "p99_ms": round(sat_cb["p99_ms"], 3)},
+                                 "batches": sat_cb["batches"], "p99_ms": round(sat_cb["p99_ms"], 3),
+                                 "warmup": "one untimed pass of the same requests on the same server first"},
                    "p50_speedup_vs_one_per_launch": round(seq["p50_ms"] / cb["p50_ms"], 3),
                    "p99_speedup_vs_one_per_launch": round(seq["p99_ms"] / cb["p99_ms"], 3),
                    "parity": ("bit-exact: 24 sampled requests per mode equal the product computed alone"

@@ -33,9 +33,14 @@
 //     capacity (exact for the rows the symbolic kernels count, the work bound
 //     for single-pass rows) and the large rows' accumulator windows;
 //   - after the values, the batch is measured again.
-// Requests are admitted in arrival order while the estimate fits, reusing the
-// pooled batch with the most bytes; other idle batches are trimmed when their
-// bytes are needed.  When the queue's first request does not fit by its
+// Requests are admitted in arrival order while the estimate fits and the
+// batch, past its first request, needs at most 2/5 of the bytes that the
+// registrations leave (two batches fit side by side, so one is built and
+// counted while the other runs), reusing the pooled batch with the most
+// bytes; other idle batches are trimmed when their bytes are needed.  When
+// the budget cuts a batch short while the device still has a batch to run,
+// admission waits for a batch to return to the pool instead of starting a
+// small one that would trim it.  When the queue's first request does not fit by its
 // capacity bound and nothing held can be released (no batch built or running, no
 // finished request awaiting its client), it is admitted alone by its arena
 // and workspace, and its count decides: C is reserved by the counted size,
@@ -150,6 +155,7 @@ struct spg_server {
     uint64_t used = 0;      // registrations + batches not in the pool (measured)
     uint64_t idle = 0;      // batches in the pool (measured)
     uint64_t reserved = 0;  // admitted, not yet allocated (estimates)
+    uint64_t registered = 0;  // registered matrices (part of used)
     std::map<spg_batch *, Slot> slots;
     int64_t done_unreleased = 0;  // finished requests awaiting spg_server_release
     int64_t next_id = 1;
@@ -344,18 +350,42 @@ std::vector<std::shared_ptr<Request>> admit(spg_server *s, std::unique_lock<std:
         spg_batch *cand = largest_idle(s);
         const bool resident = (bool)s->queue.front()->ra;
         const uint64_t held = s->used + s->idle + s->reserved;
+        // Past its first request, a batch needs at most 2/5 of the bytes the
+        // registrations leave: two batches then fit beside each other even
+        // after their buffers regrow (by 1/8), so the next batch is built and
+        // counted while one runs.  A batch as large as the whole budget
+        // serialises them, and the few requests that fit beside it form
+        // batches of their own.
+        const uint64_t share = (s->budget > s->registered ? s->budget - s->registered : 0) / 5 * 2;
         Shape sh;
         uint64_t est = 0;
+        bool budget_cut = false;  // the budget, not the queue or the share, ended the batch
         while (!s->queue.empty() && (int)take.size() < s->max_batch) {
             const auto &r = s->queue.front();
             if (!r->bounded || (bool)r->ra != resident) break;
             const Shape s2 = sh.plus(*r);
             const uint64_t e2 = estimate(s, cand, s2, s2.c_entries);
-            if (held + e2 > s->budget) break;
+            if (held + e2 > s->budget) {
+                budget_cut = true;
+                break;
+            }
+            if (!take.empty() && estimate(s, nullptr, s2, s2.c_entries) > share) break;
             sh = s2;
             est = e2;
             take.push_back(r);
             s->queue.pop_front();
+        }
+        if (!take.empty() && budget_cut && (s->running || !s->built.empty())) {
+            // Only part of the next requests fits beside the batches in
+            // flight.  While the device still has a batch to run, wait for a
+            // batch to come back to the pool and be reused: a batch started in
+            // the bytes left would take a pooled batch's bytes for its values
+            // (trimming it), and those buffers would then be allocated again
+            // (fresh allocations of tens of GB stall the device for seconds).
+            // With the device idle, the part that fits is admitted.
+            for (auto it = take.rbegin(); it != take.rend(); ++it) s->queue.push_front(*it);
+            take.clear();
+            return take;
         }
         if (take.empty()) {
             // other idle batches' bytes first; then wait for releases
@@ -694,6 +724,7 @@ spg_status spg_server_register(spg_server *s, const spg_csr_view *host, int64_t 
         if (m->bytes > free_bytes(s) && s->idle > 0) trim_idle(s, lk);
         if (m->bytes > free_bytes(s)) return SPG_ENOMEM;
         s->used += m->bytes;
+        s->registered += m->bytes;
     }
     spg_status st;
     {
@@ -704,6 +735,7 @@ spg_status spg_server_register(spg_server *s, const spg_csr_view *host, int64_t 
     std::lock_guard<std::mutex> lk(s->mu);
     if (st != SPG_OK) {
         s->used -= m->bytes;
+        s->registered -= m->bytes;
         s->cv.notify_all();
         return st;
     }
@@ -724,6 +756,7 @@ spg_status spg_server_unregister(spg_server *s, int64_t handle) {
         if (it->second.use_count() > 1) return SPG_EBUSY;  // referenced by a queued or running request
         m = it->second;
         s->used -= m->bytes;
+        s->registered -= m->bytes;
         s->resident.erase(it);
         s->cv.notify_all();
     }
