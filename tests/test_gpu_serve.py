@@ -180,3 +180,56 @@ def test_served_result_too_large_for_budget():
         srv.release(rid)
     finally:
         srv.close()
+
+
+def _resident_pool(srv, n, seed):
+    mats = [W.uniform_random(4096, 4096, 12, seed=seed + k) for k in range(n)]
+    return mats, [srv.register(m) for m in mats]
+
+
+def test_served_saturated_tight_budget(engine):
+    # 120 requests at once against a budget that holds a few batches of about
+    # 10 MB per request: batches are bounded by the per-batch share of the
+    # budget, and a batch the budget cuts short waits for a pooled batch
+    # while the device is busy; every result equals the product alone.
+    srv = S.Server(0, budget_bytes=128 << 20, max_batch=4096)
+    try:
+        mats, handles = _resident_pool(srv, 6, 40)
+        pairs = [(k % 6, (k * 5 + 1) % 6) for k in range(120)]
+        rids = [srv.submit_resident(handles[i], handles[j]) for i, j in pairs]
+        want = {}
+        for k, (rid, (i, j)) in enumerate(zip(rids, pairs)):
+            got = srv.result(rid)
+            if (i, j) not in want:
+                want[(i, j)] = digest(engine.multiply(mats[i], mats[j]))
+            assert digest(got) == want[(i, j)], k
+            srv.release(rid)
+        st = srv.stats()
+        assert st["completed"] == 120
+        assert st["batches"] >= 4  # the share and the budget bounded the batches
+        for h in handles:
+            srv.unregister(h)
+        assert srv.stats()["charged_bytes"] == 0
+    finally:
+        srv.close()
+
+
+def test_served_client_holds_every_result(engine):
+    # A client that releases nothing until every request is done: admission
+    # waits for pooled batches only while the device has a batch to run, so
+    # all 60 requests complete when their results fit the budget together.
+    srv = S.Server(0, budget_bytes=1 << 30, max_batch=4096)
+    try:
+        mats, handles = _resident_pool(srv, 4, 60)
+        pairs = [(k % 4, (k + 1) % 4) for k in range(60)]
+        rids = [srv.submit_resident(handles[i], handles[j]) for i, j in pairs]
+        for rid in rids:
+            srv.wait(rid)
+        for k, (rid, (i, j)) in enumerate(zip(rids, pairs)):
+            if k % 7 == 0:
+                assert digest(srv.result(rid)) == digest(engine.multiply(mats[i], mats[j])), k
+        for rid in rids:
+            srv.release(rid)
+        assert srv.stats()["completed"] == 60
+    finally:
+        srv.close()
