@@ -72,10 +72,11 @@ __device__ __forceinline__ uint64_t same_key_lanes(uint32_t key, bool valid, int
 // its counters.
 template <int ITEMS>
 __device__ __forceinline__ void wave_multisplit(const uint32_t (&key)[ITEMS], const bool (&valid)[ITEMS], int nbits,
-                                                int *cnt, int stride, int (&rank)[ITEMS]) {
+                                                int *cnt, int stride, int (&rank)[ITEMS], int nitems = ITEMS) {
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
+        if (j >= nitems) break;  // wave-uniform: items j >= nitems hold nothing
         const uint64_t m = same_key_lanes(key[j], valid[j], nbits);
         const int before = __popcll(m & lt);
         const int base = valid[j] ? cnt[key[j] * stride] : 0;
@@ -371,20 +372,32 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
             const int64_t n = ps.pw[rb + w];
             const int d = ps.pd[rb + w];
             if (n == 0) continue;
-            // each chunk's products are loaded one chunk ahead
+            // Each chunk's products are loaded one chunk ahead.  A chunk of
+            // `total` products gives each lane it = ceil(total / TPB) items
+            // (at most ITEMS), item j of lane l of wave wv being product
+            // wv * 64 * it + 64 j + l: every wave holds a contiguous range, in
+            // order, and a small chunk is spread over all waves.
             uint16_t nlc[ITEMS];
             double nv[ITEMS];
+            auto items_of = [&](int64_t c0) {
+                const int64_t rem = n - c0;
+                return rem >= CAP ? ITEMS : (int)((rem + TPB - 1) / TPB);
+            };
             auto load = [&](int64_t c0) {
+                const int it = items_of(c0);
+                const int64_t tot = min((int64_t)CAP, n - c0);
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
-                    const int64_t y = c0 + (int64_t)wv * (64 * ITEMS) + j * 64 + lane;
-                    nlc[j] = y < n ? ps.scol[s + y] : (uint16_t)0;
-                    nv[j] = y < n ? ps.sval[s + y] : 0.0;
+                    const int x = wv * (64 * it) + j * 64 + lane;
+                    const bool in = j < it && x < tot;
+                    nlc[j] = in ? ps.scol[s + c0 + x] : (uint16_t)0;
+                    nv[j] = in ? ps.sval[s + c0 + x] : 0.0;
                 }
             };
             load(0);
             for (int64_t c0 = 0; c0 < n; c0 += CAP) {
                 const int total = (int)min((int64_t)CAP, n - c0);
+                const int it = items_of(c0);
                 uint16_t lc[ITEMS];
                 double v[ITEMS];
                 bool ok[ITEMS];
@@ -394,17 +407,17 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
                 for (int j = 0; j < ITEMS; ++j) {
                     lc[j] = nlc[j];
                     v[j] = nv[j];
-                    ok[j] = wv * (64 * ITEMS) + j * 64 + lane < total;
+                    ok[j] = j < it && wv * (64 * it) + j * 64 + lane < total;
                     key[j] = (uint32_t)lc[j] & (uint32_t)(NB - 1);
                 }
                 if (c0 + CAP < n) load(c0 + CAP);
                 // Stable split by bucket: item j of lane l of wave wv is the
-                // chunk's item wv * 64 * ITEMS + 64 j + l (serial order); a
+                // chunk's item wv * 64 * it + 64 j + l (serial order); a
                 // wave's items of one bucket are ranked by ballots, and the
                 // waves are ordered by a scan per bucket.
                 for (int k = lane; k < NB; k += 64) s_gc[k * NW + wv] = 0;
                 wave_sync();
-                wave_multisplit<ITEMS>(key, ok, NBITS, &s_gc[wv], NW, rank);
+                wave_multisplit<ITEMS>(key, ok, NBITS, &s_gc[wv], NW, rank, it);
                 __syncthreads();
                 int cw[NW], cnt = 0;
 #pragma unroll
@@ -464,18 +477,18 @@ __global__ __launch_bounds__(TPB) void k_large_fold(Operands op, Workspace ws, P
                     // A bucket holds a hub column's long run: a second stable
                     // split, by the column's high bits, sorts the chunk by
                     // column (position x of the first split is item j of lane
-                    // l of wave wv, x = wv * 64 * ITEMS + 64 j + l).
+                    // l of wave wv, x = wv * 64 * it + 64 j + l).
                     constexpr int HBITS = kPartLog - NBITS;
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) {
-                        const int x = wv * (64 * ITEMS) + j * 64 + lane;
+                        const int x = wv * (64 * it) + j * 64 + lane;
                         lc[j] = ok[j] ? s_lc[x] : (uint16_t)0;
                         v[j] = ok[j] ? s_lv[x] : 0.0;
                         key[j] = (uint32_t)lc[j] >> NBITS;
                     }
                     for (int k = lane; k < (1 << HBITS); k += 64) s_gc[k * NW + wv] = 0;
                     __syncthreads();  // every item is read back before the split overwrites
-                    wave_multisplit<ITEMS>(key, ok, HBITS, &s_gc[wv], NW, rank);
+                    wave_multisplit<ITEMS>(key, ok, HBITS, &s_gc[wv], NW, rank, it);
                     __syncthreads();
                     // offsets of the (1 << HBITS) high buckets, in order, and
                     // of each wave's items inside them
