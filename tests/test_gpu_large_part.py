@@ -3,7 +3,8 @@
 serial-order oracle and identical to the HBM-window kernels
 (SPG_LARGE_WINDOWS), for rows whose windows cancel to exact zeros, rows cut
 into several fold tasks, hub columns taking most of a window's products,
-windows of many chunks, and C with more columns than the compact kernel
+windows of many chunks, windows of 8 to 2,056 products (every per-lane item
+count of a fold chunk), and C with more columns than the compact kernel
 takes."""
 import numpy as np
 import pytest
@@ -173,3 +174,41 @@ def test_heavy_rows_analyze_pieces_and_count_launch(engine):
         a2 = _csr(70, nb, lens2, rng.integers(0, nb, size=int(lens2.sum())), rng.uniform(-1, 1, size=int(lens2.sum())))
         for mode, st, c in _both_modes(engine, a2, b, paths=(S.PATH_AUTO,)):
             assert st["products"] == O.scalar_products(a2, b)
+
+
+@pytest.mark.parametrize("hub", [False, True], ids=["spread", "hub"])
+def test_partition_window_sizes(engine, hub):
+    # Rows whose windows (4096 columns each) get 8 to 2,056 products: a fold
+    # chunk of n products gives each lane ceil(n / 256) items, so every item
+    # count from 1 to 4 and partial last chunks of several-chunk windows occur.
+    # With `hub`, half of each B row's entries fall in one column of their
+    # window (the sorted path); bit-exact against the oracle.
+    rng = np.random.Generator(np.random.PCG64(72))
+    cols, per = 200_000, 8
+    nwin = (cols + 4095) // 4096
+    sizes = [1, 8, 32, 33, 64, 65, 96, 97, 128, 129, 256, 257]  # B rows per window in a row: 8x products
+    rows_b = []
+    for w in range(nwin):
+        lo, hi = w * 4096, min(cols, (w + 1) * 4096)
+        for _ in range(260):
+            c = rng.integers(lo, hi, size=per)
+            if hub:
+                c[: per // 2] = lo + 7
+            rows_b.append(c)
+    nb = len(rows_b)
+    b = _csr(nb, cols, np.full(nb, per), np.concatenate(rows_b), rng.uniform(-1, 1, size=nb * per))
+    arows = []
+    for r in range(6):
+        ws = rng.choice(nwin, size=len(sizes), replace=False)
+        ks = []
+        for w, m in zip(ws, np.roll(sizes, r)):
+            ks.append(w * 260 + rng.choice(260, size=int(m), replace=False))
+        k = np.concatenate(ks)
+        rng.shuffle(k)
+        arows.append(k)
+    a = _csr(len(arows), nb, [len(k) for k in arows], np.concatenate(arows),
+             rng.uniform(-1, 1, size=sum(len(k) for k in arows)))
+    for mode, st, c in _both_modes(engine, a, b, paths=(S.PATH_AUTO,)):
+        assert st["rows_per_bin"][6] == len(arows)
+        if mode == S.LARGE_AUTO:
+            assert st["partitioned"]
